@@ -1,0 +1,252 @@
+/*
+ * lfgpu.h -- C ABI of the MI355X-native LehrFEM++ assembly + PCG path.
+ *
+ * The library (lehrfempp_amd/liblfgpu.so) replaces the data-parallel part of
+ * LehrFEM++'s Galerkin pipeline
+ *
+ *   ReactionDiffusionElementMatrixProvider / ScalarLoadElementVectorProvider
+ *     -> AssembleMatrixLocally / AssembleVectorLocally into a COOMatrix
+ *     -> FixFlaggedSolutionComponents -> COOMatrix::makeSparse
+ *     -> sparse solve
+ *
+ * with device-resident HIP kernels.  All entry points return an int status
+ * (LFGPU_OK or one of the error codes below) and never throw; the text of the
+ * last error of the calling thread is available from lfgpu_last_error().
+ * Host buffers passed in are owned by the caller and only read/written during
+ * the call; device buffers are owned by the library's handles.
+ * All work of one context is ordered on that context's HIP stream; a context
+ * must be used by one host thread at a time.
+ *
+ * Index conventions: DOF indices are gdof_idx_t = int64 at the boundary
+ * (reference assemble/assembly_types.h:22) and int32 on the device
+ * (every supported problem has fewer than 2^31 DOFs and nonzeros).
+ */
+#ifndef LFGPU_H
+#define LFGPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define LFGPU_OK 0
+#define LFGPU_EINVAL 1      /* bad argument / size mismatch (LF_ASSERT_MSG)   */
+#define LFGPU_EDEGENERATE 2 /* degenerate triangle (tria_o1.cc:10-48)         */
+#define LFGPU_ENOCONV 3     /* iterative solver hit maxit before rtol        */
+#define LFGPU_EOOM 4        /* device allocation failed                      */
+#define LFGPU_EHIP 5        /* HIP runtime error                             */
+#define LFGPU_ENORULE 6     /* no quadrature rule / FE (LfException,
+                               uscalfe/loc_comp_ellbvp.h:278-287)             */
+#define LFGPU_ERANGE 7      /* index out of range or size limit exceeded     */
+
+/* Text of the last error on this thread ("" if none). */
+const char *lfgpu_last_error(void);
+
+/* ---- opaque handles ---------------------------------------------------- */
+typedef struct lfgpu_ctx lfgpu_ctx;   /* device + stream + event timer      */
+typedef struct lfgpu_vec lfgpu_vec;   /* device FP64 vector                 */
+typedef struct lfgpu_mesh lfgpu_mesh; /* vertex coords + cell->vertex/DOF   */
+typedef struct lfgpu_fe lfgpu_fe;     /* quadrature rule + shape tables     */
+typedef struct lfgpu_plan lfgpu_plan; /* CSR pattern + scatter records      */
+typedef struct lfgpu_csr lfgpu_csr;   /* device CSR matrix                  */
+
+/* ---- context ----------------------------------------------------------- */
+int lfgpu_ctx_create(int device, lfgpu_ctx **out);
+int lfgpu_ctx_destroy(lfgpu_ctx *ctx);
+/* Block until all work queued on the context's stream has finished. */
+int lfgpu_ctx_sync(lfgpu_ctx *ctx);
+/* Device time (hipEvent pair on the context stream) of the most recent
+ * assembly / SpMV / PCG call, in milliseconds. */
+int lfgpu_ctx_last_ms(lfgpu_ctx *ctx, double *ms);
+
+/* ---- device vectors ---------------------------------------------------- */
+int lfgpu_vec_create(lfgpu_ctx *ctx, int64_t n, const double *host_init,
+                     lfgpu_vec **out);
+int lfgpu_vec_destroy(lfgpu_vec *v);
+int lfgpu_vec_size(const lfgpu_vec *v, int64_t *n);
+int lfgpu_vec_upload(lfgpu_vec *v, const double *host, int64_t n);
+int lfgpu_vec_download(const lfgpu_vec *v, double *host, int64_t n);
+int lfgpu_vec_fill(lfgpu_vec *v, double value);
+
+/* ---- mesh + DOF handler ------------------------------------------------
+ * Replaces the cell loop's view of the mesh and of UniformFEDofHandler
+ * (assemble/dofhandler.cc:86-305): xy [nv x 2] vertex coordinates,
+ * cell_v [ncells x 3] vertex indices in the cell's local order, cell_dofs
+ * [ncells x nloc] the GlobalDofIndices of each cell (NULL for P1, where the
+ * DOFs are the vertex indices).  Degenerate cells (tria_o1.cc:10-48,
+ * tolerance 1e-8) are rejected with LFGPU_EDEGENERATE. */
+int lfgpu_mesh_create(lfgpu_ctx *ctx, int64_t nv, const double *xy,
+                      int64_t ncells, const int32_t *cell_v, int nloc,
+                      const int64_t *cell_dofs, int64_t ndofs,
+                      lfgpu_mesh **out);
+int lfgpu_mesh_destroy(lfgpu_mesh *m);
+
+/* ---- host-side topology helpers (no GPU work) ---------------------------
+ * TPTriagMeshBuilder::Build (mesh/utils/tp_triag_mesh_builder.cc:18-178):
+ * xy [(nx+1)(ny+1) x 2], cell_v [2 nx ny x 3], edge_v [E x 2] with
+ * E = nx ny + (nx+1) ny + nx (ny+1), in the builder's insertion order. */
+int lfgpu_tp_triag_mesh(int32_t nx, int32_t ny, double x0, double y0,
+                        double x1, double y1, double *xy, int32_t *cell_v,
+                        int32_t *edge_v);
+/* hybrid2d::Mesh edge numbering (mesh/hybrid2d/mesh.cc:236-575) and
+ * relative orientations (mesh/hybrid2d/triangle.cc:71-77) for a triangle
+ * mesh with npre predefined edges; *nedges receives E (edge_v_out holds
+ * capacity max_edges).  Then UniformFEDofHandler::initIndexArrays
+ * (assemble/dofhandler.cc:142-284) for {node:1, edge:ndof_edge,
+ * cell:ndof_cell}: cell_dofs [ncells x (3+3 ndof_edge+ndof_cell)]. */
+int lfgpu_mesh_edges(int64_t nv, int64_t ncells, const int32_t *cell_v,
+                     int64_t npre, const int32_t *pre_edge_v,
+                     int64_t max_edges, int32_t *edge_v_out, int32_t *cell_e,
+                     int8_t *cell_ori, int64_t *nedges);
+int lfgpu_dof_table(int64_t nv, int64_t ncells, int64_t nedges,
+                    const int32_t *cell_v, const int32_t *cell_e,
+                    const int8_t *cell_ori, int ndof_edge, int ndof_cell,
+                    int64_t *cell_dofs, int64_t *ndofs);
+
+/* ---- reference finite element + quadrature ------------------------------
+ * PrecomputedScalarReferenceFiniteElement
+ * (uscalfe/precomputed_scalar_reference_finite_element.h:72-78) for
+ * FeLagrangeO1Tria (degree 1) or FeLagrangeO3Tria (degree 3)
+ * (uscalfe/lagr_fe.h:57-150, :945-1140).  pts/w == NULL selects
+ * make_QuadRule(kTria, 2*degree) (quad/make_quad_rule.cc:38-50), which is
+ * what the providers' first constructor does (loc_comp_ellbvp.h:210-231);
+ * otherwise pts is [2 x nqp] column-major (a user quad_rule_collection_t). */
+int lfgpu_fe_create(lfgpu_ctx *ctx, int degree, int nqp, const double *pts,
+                    const double *w, lfgpu_fe **out);
+int lfgpu_fe_destroy(lfgpu_fe *fe);
+int lfgpu_fe_info(const lfgpu_fe *fe, int *nloc, int *nqp);
+
+/* ---- coefficients (MeshFunction arguments of the providers) ------------- */
+#define LFGPU_COEFF_CONST 0  /* MeshFunctionConstant(value)                  */
+#define LFGPU_COEFF_CELL 1   /* value * data[cell]  (CodimMeshDataSet)       */
+#define LFGPU_COEFF_QP 2     /* value * data[cell*nqp + k]                   */
+#define LFGPU_COEFF_SINSIN 3 /* value * sin(pi x) sin(pi y) at TriaO1::Global */
+#define LFGPU_COEFF_EXPCELL 4 /* value * exp(data[cell])                      */
+typedef struct lfgpu_coeff {
+  int kind;
+  double value;
+  const lfgpu_vec *data;
+} lfgpu_coeff;
+
+/* ---- element-level kernels (COO value streams, reference order) ---------
+ * Geometry of every cell as TriaO1 precomputes it (geometry/tria_o1.cc:50-68):
+ * J and J^{-T} row-major 2x2 per cell, |det J| per cell. */
+int lfgpu_geometry(lfgpu_ctx *ctx, const lfgpu_mesh *m, lfgpu_vec *J,
+                   lfgpu_vec *JinvT, lfgpu_vec *det);
+/* ReactionDiffusionElementMatrixProvider::Eval for all cells
+ * (uscalfe/loc_comp_ellbvp.h:266-339): out [ncells x nloc x nloc], row-major
+ * per cell; that is the triplet value stream of AssembleMatrixLocally
+ * (assemble/assembler.h:166-179) in its order. */
+int lfgpu_element_matrices(lfgpu_ctx *ctx, const lfgpu_mesh *m,
+                           const lfgpu_fe *fe, const lfgpu_coeff *alpha,
+                           const lfgpu_coeff *gamma, lfgpu_vec *out);
+/* ScalarLoadElementVectorProvider::Eval for all cells
+ * (uscalfe/loc_comp_ellbvp.h:691-746): out [ncells x nloc]. */
+int lfgpu_element_vectors(lfgpu_ctx *ctx, const lfgpu_mesh *m,
+                          const lfgpu_fe *fe, const lfgpu_coeff *f,
+                          lfgpu_vec *out);
+
+/* ---- assembly plan: sparsity pattern + deterministic scatter ------------
+ * Builds, on the device, the compressed pattern that
+ * FixFlaggedSolutionComponents (assemble/fix_dof.h:86-138) followed by
+ * COOMatrix::makeSparse (coomatrix.h:172-180) yields for the triplets of
+ * AssembleMatrixLocally: rows/cols of fixed DOFs are removed except for a
+ * unit diagonal.  fixed_flags [ndofs] (host, NULL = no fixed DOFs) is the
+ * selector of FixFlaggedSolutionComponents.  The plan also holds, per row,
+ * its contributing (cell, local row) pairs in cell order with the CSR slot
+ * of every local column, so re-assembly sums each entry in triplet order
+ * without atomics. */
+int lfgpu_plan_create(lfgpu_ctx *ctx, const lfgpu_mesh *m,
+                      const uint8_t *fixed_flags, lfgpu_plan **out);
+int lfgpu_plan_destroy(lfgpu_plan *p);
+/* Borrow the plan's CSR (values are written by lfgpu_assemble_galerkin). */
+int lfgpu_plan_csr(lfgpu_plan *p, lfgpu_csr **csr);
+
+/* Fused AssembleMatrixLocally + AssembleVectorLocally +
+ * FixFlaggedSolutionComponents + makeSparse: writes the plan's CSR values and
+ * rhs [ndofs].  fixed_vals [ndofs] are the prescribed values of the fixed
+ * DOFs (NULL = zero), applied as b -= A x_fixed, b[fixed] = value
+ * (fix_dof.h:97-114).  f may be NULL (rhs then only carries the Dirichlet
+ * terms), rhs may be NULL (matrix only). */
+int lfgpu_assemble_galerkin(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
+                            const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,
+                            const lfgpu_coeff *f, const lfgpu_vec *fixed_vals,
+                            lfgpu_vec *rhs);
+
+/* ---- CSR matrices ------------------------------------------------------ */
+int lfgpu_csr_info(const lfgpu_csr *A, int32_t *nrows, int32_t *ncols,
+                   int64_t *nnz);
+int lfgpu_csr_download(const lfgpu_csr *A, int32_t *row_ptr, int32_t *col,
+                       double *val);
+/* COOMatrix::makeSparse -> setFromTriplets (coomatrix.h:172-180) on the
+ * device for host triplets: stable sort by (row, col), duplicates summed in
+ * triplet order, explicit zeros kept. */
+int lfgpu_coo_to_csr(lfgpu_ctx *ctx, int64_t nnz, const int32_t *rows,
+                     const int32_t *cols, const double *vals, int32_t nrows,
+                     int32_t ncols, lfgpu_csr **out);
+int lfgpu_csr_destroy(lfgpu_csr *A);
+/* y = A x (FP64). */
+int lfgpu_spmv(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *x,
+               lfgpu_vec *y);
+
+/* y = A x launched reps times back to back between one hipEvent pair on the
+ * context stream; *avg_ms is the mean duration of one SpMV (measurement
+ * support for roofline figures; no reference counterpart). */
+int lfgpu_spmv_timed(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *x,
+                     lfgpu_vec *y, int reps, double *avg_ms);
+
+/* ---- Jacobi-preconditioned CG ------------------------------------------
+ * Replaces Eigen::SparseLU in the demo drivers
+ * (examples/ellbvp_linfe/homDir_linfe_demo.cc:171-174) for SPD systems.
+ * Stopping rule and iteration count follow Eigen 3.4
+ * ConjugateGradient<.., DiagonalPreconditioner>: stop when
+ * ||b - A x||_2 < rtol ||b||_2.
+ * mode LFGPU_PCG_FP64: everything in FP64.
+ * mode LFGPU_PCG_MIXED: CG recurrences, matrix and SpMV in FP32 with
+ *   reliable updates: the true residual b - A x is recomputed in FP64 and the
+ *   solution accumulated in FP64; convergence is always judged on the FP64
+ *   true residual. */
+#define LFGPU_PCG_FP64 0
+#define LFGPU_PCG_MIXED 1
+typedef struct lfgpu_pcg_opts {
+  double rtol;     /* relative residual target                    */
+  int maxit;       /* iteration cap (Eigen: 2*n by default)       */
+  int mode;        /* LFGPU_PCG_FP64 or LFGPU_PCG_MIXED            */
+  int check_every; /* iterations between host convergence checks */
+} lfgpu_pcg_opts;
+typedef struct lfgpu_pcg_info {
+  int iters;       /* Eigen's iteration count                     */
+  int converged;   /* 1 if rtol was reached                       */
+  int refinements; /* FP64 true-residual replacements (mixed)     */
+  double relres;   /* final ||b - A x|| / ||b|| (FP64)            */
+  double ms;       /* device time of the solve                    */
+} lfgpu_pcg_info;
+int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *b,
+              lfgpu_vec *x, const lfgpu_pcg_opts *opts, lfgpu_pcg_info *info);
+
+/* ---- adjoint gradient ----------------------------------------------------
+ * For the P1 problem A(theta) u = f with A(theta) = sum_e exp(theta_e) K_e
+ * (+ reaction) assembled by the plan, J(u) = 1/2 (u-u_obs)^T M (u-u_obs) and
+ * the adjoint solution lambda of A lambda = M (u - u_obs):
+ *   grad[e] = -exp(theta_e) lambda_e^T K_e u_e,
+ * K_e the unit-coefficient element stiffness of
+ * ReactionDiffusionElementMatrixProvider::Eval (alpha = 1, gamma = 0). */
+int lfgpu_adjoint_gradient(lfgpu_ctx *ctx, const lfgpu_mesh *m,
+                           const lfgpu_fe *fe, const lfgpu_vec *theta,
+                           const lfgpu_vec *u, const lfgpu_vec *lam,
+                           lfgpu_vec *grad);
+
+/* ---- small BLAS-1 helpers used by drivers ------------------------------- */
+/* y = a*x + b*y */
+int lfgpu_axpby(lfgpu_ctx *ctx, double a, const lfgpu_vec *x, double b,
+                lfgpu_vec *y);
+/* *out = x . y (FP64, deterministic order) */
+int lfgpu_dot(lfgpu_ctx *ctx, const lfgpu_vec *x, const lfgpu_vec *y,
+              double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LFGPU_H */

@@ -1,0 +1,130 @@
+// Internal declarations of liblfgpu (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/lfgpu.h"
+
+namespace lfgpu {
+
+void set_error(const std::string &msg);
+void clear_error();
+
+// Returns a non-OK status with the HIP error text recorded.
+int hip_fail(hipError_t e, const char *what);
+
+#define LFG_HIP(expr)                                     \
+  do {                                                    \
+    hipError_t lfg_e_ = (expr);                           \
+    if (lfg_e_ != hipSuccess) return lfgpu::hip_fail(lfg_e_, #expr); \
+  } while (0)
+
+#define LFG_REQUIRE(cond, code, msg) \
+  do {                               \
+    if (!(cond)) {                   \
+      lfgpu::set_error(msg);         \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+#define LFG_TRY(expr)               \
+  do {                              \
+    int lfg_rc_ = (expr);           \
+    if (lfg_rc_ != LFGPU_OK) return lfg_rc_; \
+  } while (0)
+
+// Device allocation that records an LFGPU_EOOM on failure.
+int dmalloc(void **p, size_t bytes, const char *what);
+template <class T>
+int dalloc(T **p, size_t count, const char *what) {
+  return dmalloc(reinterpret_cast<void **>(p), count * sizeof(T), what);
+}
+void dfree(void *p);
+
+// Coefficient as seen by kernels (lfgpu_coeff resolved to a device pointer).
+struct CoeffD {
+  int kind;
+  double value;
+  const double *data;
+};
+int resolve_coeff(const lfgpu_coeff *c, int64_t ncells, int nqp, bool allow_null,
+                  CoeffD *out, const char *name);
+
+// Device-wide exclusive scan of int32 counts into int32 offsets (n+1 outputs).
+int exclusive_scan_i32(hipStream_t s, const int32_t *in, int32_t *out, int64_t n,
+                       int64_t *total);
+
+}  // namespace lfgpu
+
+struct lfgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  // pinned host mirror of small solver status words
+  void *pinned = nullptr;
+  // reusable device scratch for reductions (partials + scalars)
+  double *red = nullptr;
+  size_t red_count = 0;
+};
+
+struct lfgpu_vec {
+  lfgpu_ctx *ctx = nullptr;
+  double *d = nullptr;
+  int64_t n = 0;
+};
+
+struct lfgpu_mesh {
+  lfgpu_ctx *ctx = nullptr;
+  int64_t nv = 0, ncells = 0, ndofs = 0;
+  int nloc = 0;
+  double *xy = nullptr;         // [nv x 2]
+  int32_t *cell_v = nullptr;    // [ncells x 3]
+  int32_t *cell_dofs = nullptr; // [ncells x nloc]; aliases cell_v for P1
+  bool dofs_alias = false;
+};
+
+struct lfgpu_fe {
+  lfgpu_ctx *ctx = nullptr;
+  int degree = 0, nloc = 0, nqp = 0;
+  // device table: w[nqp] | pts[2 nqp] | N[nloc nqp] | dN[nloc 2 nqp]
+  double *tab = nullptr;
+  int tab_len = 0;
+};
+
+struct lfgpu_solver_cache;  // lfgpu_solver.hip
+
+struct lfgpu_csr {
+  lfgpu_ctx *ctx = nullptr;
+  int32_t nrows = 0, ncols = 0;
+  int64_t nnz = 0;
+  int32_t *row_ptr = nullptr;
+  int32_t *col = nullptr;
+  double *val = nullptr;
+  bool owned_by_plan = false;
+  lfgpu_solver_cache *cache = nullptr;  // built lazily by the solver
+  uint64_t val_version = 0;             // bumped whenever val changes
+};
+
+struct lfgpu_plan {
+  lfgpu_ctx *ctx = nullptr;
+  const lfgpu_mesh *mesh = nullptr;
+  lfgpu_csr csr;
+  int32_t *rec_ptr = nullptr;  // [ndofs + 1]
+  int32_t *rec_key = nullptr;  // [R] cell*nloc + i, cell order within a row
+  uint8_t *rec_meta = nullptr; // [R x meta_stride]: i, slot of each local col
+  int meta_stride = 0;
+  int64_t nrec = 0;
+  uint8_t *fixed = nullptr;    // [ndofs]
+  bool has_fixed = false;
+  int block = 256;             // rows per workgroup of the assembly kernel
+  int max_block_nnz = 0;       // LDS doubles needed by one workgroup
+};
+
+namespace lfgpu {
+void solver_cache_free(lfgpu_csr *A);
+inline void csr_values_changed(lfgpu_csr *A) { A->val_version++; }
+}  // namespace lfgpu

@@ -1,0 +1,176 @@
+/*
+ * lfref.h -- CPU oracle for the LehrFEM++ assembly + solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain C++ restatement of the
+ * reference's algorithms, used as the checker for the HIP path.  Only the
+ * test suite, the smoke check and the CPU-baseline leg of the benchmark may
+ * load it.  The product library (lehrfempp_amd/liblfgpu.so) never links,
+ * loads or calls anything here.
+ *
+ * Every function cites the reference file:line it restates.  Conventions:
+ *   - index arrays are int32 unless stated (gdof_idx_t is int64 in the
+ *     reference, assemble/assembly_types.h:22; cell DOF tables are passed as
+ *     int64 for that reason);
+ *   - matrices produced per cell are row-major nloc x nloc;
+ *   - per-quadrature-point coefficient arrays are [ncells x nqp] row-major.
+ *   - compiled with -ffp-contract=off so that no FMA contraction changes the
+ *     rounding of the restated expressions.
+ */
+#ifndef LFREF_H
+#define LFREF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Quadrature rules on the reference triangle, degree 2 (3 points) and 6
+ * (12 points): quad/quad_rules_tria.cc:35-51, :113-156, selected by
+ * quad/make_quad_rule.cc:38-50.  pts is [2 x nqp] column-major (point k is
+ * pts[2k], pts[2k+1]).  Returns nqp, or -1 for an unsupported degree. */
+int lfref_quad_rule(int degree, double *pts, double *w);
+
+/* Reference shape functions of FeLagrangeO1Tria / FeLagrangeO3Tria at the
+ * given points (uscalfe/lagr_fe.h:104-129, :1010-1101).
+ * N is [nsf x npts] column-major, dN is [nsf x 2*npts] column-major with
+ * columns 2k, 2k+1 the x1/x2 derivatives at point k.  Returns nsf. */
+int lfref_shape_functions(int degree, int npts, const double *pts, double *N,
+                          double *dN);
+
+/* TPTriagMeshBuilder::Build on [x0,x1]x[y0,y1] with nx x ny cells
+ * (mesh/utils/tp_triag_mesh_builder.cc:18-178).  Output arrays sized by the
+ * caller: xy[2*(nx+1)*(ny+1)], cell_v[3*2*nx*ny],
+ * edge_v[2*(nx*ny+(nx+1)*ny+nx*(ny+1))]. */
+void lfref_tp_triag_mesh(int nx, int ny, double x0, double y0, double x1,
+                         double y1, double *xy, int32_t *cell_v,
+                         int32_t *edge_v);
+
+/* Edge numbering of hybrid2d::Mesh for a triangle mesh
+ * (mesh/hybrid2d/mesh.cc:236-274 predefined edges keep their position,
+ * :302-430 edges found from cells, :517-575 created edges numbered after the
+ * predefined ones in (min,max)-endpoint order), the cell->edge table and the
+ * relative orientations (mesh/hybrid2d/triangle.cc:71-77).
+ * Returns the total number of edges E; edge_v_out must hold 2*E ints
+ * (caller passes capacity max_edges).  Returns -1 on overflow. */
+int lfref_mesh_edges(int nv, int ncells, const int32_t *cell_v, int npre,
+                     const int32_t *pre_edge_v, int max_edges,
+                     int32_t *edge_v_out, int32_t *cell_e, int8_t *cell_ori);
+
+/* UniformFEDofHandler::initIndexArrays for triangle meshes with
+ * {node: 1, edge: ndof_edge, cell: ndof_cell} local DOFs
+ * (assemble/dofhandler.cc:142-284).  cell_dofs is [ncells x nloc] int64.
+ * Returns the number of DOFs. */
+int64_t lfref_dof_table(int nv, int ncells, int nedges, const int32_t *cell_v,
+                        const int32_t *cell_e, const int8_t *cell_ori,
+                        int ndof_edge, int ndof_cell, int64_t *cell_dofs);
+
+/* flagEntitiesOnBoundary (mesh/utils/special_entity_sets.cc:40-62) for
+ * nodes and edges: an edge with exactly one adjacent cell is a boundary edge
+ * and its endpoints are boundary nodes. */
+void lfref_boundary_flags(int nv, int ncells, int nedges,
+                          const int32_t *cell_e, const int32_t *edge_v,
+                          uint8_t *node_flag, uint8_t *edge_flag);
+
+/* TriaO1 constructor (geometry/tria_o1.cc:10-68): per cell J (row-major
+ * 2x2), J^{-T} (row-major 2x2) and |det J|.  Returns 0, or -(1+cell) for the
+ * first degenerate cell. */
+int lfref_geometry(int ncells, const double *xy, const int32_t *cell_v,
+                   double *J, double *JinvT, double *det);
+
+/* ReactionDiffusionElementMatrixProvider::Eval for every cell
+ * (uscalfe/loc_comp_ellbvp.h:266-339) with the degree-2*p rule.
+ * alpha, gamma: [ncells x nqp].  mats: [ncells x nloc x nloc] row-major.
+ * Returns 0 or a negative error (degenerate cell). */
+int lfref_element_matrices(int degree, int ncells, const double *xy,
+                           const int32_t *cell_v, const double *alpha,
+                           const double *gamma, double *mats);
+
+/* ScalarLoadElementVectorProvider::Eval for every cell
+ * (uscalfe/loc_comp_ellbvp.h:691-746).  f_kind: 0 = constant f_const,
+ * 1 = sin(pi x) sin(pi y) at TriaO1::Global(qp) (geometry/tria_o1.cc:70-74),
+ * 2 = per-QP values f_qp[ncells x nqp]. */
+int lfref_element_vectors(int degree, int ncells, const double *xy,
+                          const int32_t *cell_v, int f_kind, double f_const,
+                          const double *f_qp, double *vecs);
+
+/* Global quadrature point coordinates TriaO1::Global (tria_o1.cc:70-74),
+ * [ncells x nqp x 2]. */
+int lfref_qp_coords(int degree, int ncells, const double *xy,
+                    const int32_t *cell_v, double *out);
+
+/* AssembleMatrixLocally triplet loop (assemble/assembler.h:114-186,
+ * coomatrix.h:87-91): triplets in cell order, then i, then j. */
+void lfref_assemble_coo(int ncells, int nloc, const int64_t *cell_dofs,
+                        const double *mats, int32_t *rows, int32_t *cols,
+                        double *vals);
+
+/* AssembleVectorLocally (assemble/assembler.h:298-327) into a zeroed b. */
+void lfref_assemble_vector(int ncells, int nloc, const int64_t *cell_dofs,
+                           const double *vecs, int64_t ndofs, double *b);
+
+/* COOMatrix::MatVecMult(alpha, x, y): y += v*(alpha*x[col]) in triplet order
+ * (assemble/coomatrix.h:248-260). */
+void lfref_coo_matvec(int64_t nnz, const int32_t *rows, const int32_t *cols,
+                      const double *vals, double alpha, const double *x,
+                      double *y);
+
+/* FixFlaggedSolutionComponents (assemble/fix_dof.h:86-138) on a triplet
+ * list held in caller arrays of capacity cap >= nnz + ndofs.  Returns the new
+ * triplet count. */
+int64_t lfref_fix_flagged(int64_t nnz, int32_t *rows, int32_t *cols,
+                          double *vals, int64_t ndofs, const uint8_t *flags,
+                          const double *fixed_vals, double *b);
+
+/* COOMatrix::makeSparse -> SparseMatrix::setFromTriplets (coomatrix.h:172-180,
+ * Eigen 3.4.0): duplicates summed in triplet order, column indices sorted,
+ * explicit zeros kept.  Output as CSR of the same matrix; col/val need
+ * capacity nnz.  Returns the compressed nnz. */
+int64_t lfref_set_from_triplets(int64_t nnz, const int32_t *rows,
+                                const int32_t *cols, const double *vals,
+                                int32_t nrows, int32_t *row_ptr, int32_t *col,
+                                double *val);
+
+/* Direct FP64 solve of an SPD CSR system by banded Cholesky (the role of
+ * Eigen::SparseLU in examples/ellbvp_linfe/homDir_linfe_demo.cc:171-174).
+ * Returns 0, -1 if not positive definite, -2 if the band is too large. */
+int lfref_band_cholesky_solve(int32_t n, const int32_t *row_ptr,
+                              const int32_t *col, const double *val,
+                              const double *b, double *x);
+
+/* CSR y = A x (plain row loop). */
+void lfref_csr_spmv(int32_t n, const int32_t *row_ptr, const int32_t *col,
+                    const double *val, const double *x, double *y);
+
+/* Jacobi-preconditioned conjugate gradients with the semantics of Eigen
+ * 3.4.0 ConjugateGradient + DiagonalPreconditioner (x0 given in x; stop when
+ * ||r||^2 < tol^2 ||b||^2).  Returns iterations; *err = ||r||/||b||. */
+int lfref_pcg(int32_t n, const int32_t *row_ptr, const int32_t *col,
+              const double *val, const double *b, double *x, double tol,
+              int maxit, double *err);
+
+/* Reference-faithful CPU baseline of the whole P1/P3 path on one thread:
+ * per-cell Eval into heap-allocated matrices, triplet push_back, Dirichlet
+ * fix on the triplets, setFromTriplets.  Outputs the CSR (capacities as for
+ * lfref_set_from_triplets with nnz = ncells*nloc^2 + ndofs) and b. */
+int64_t lfref_cpu_path_assemble(int degree, int ncells, const double *xy,
+                                const int32_t *cell_v, const int64_t *cell_dofs,
+                                int64_t ndofs, const double *alpha_cell,
+                                double gamma_const, int f_kind, double f_const,
+                                const uint8_t *fixed, int32_t *row_ptr,
+                                int32_t *col, double *val, double *b);
+
+/* Adjoint gradient of J = 1/2 (u-u_obs)^T M (u-u_obs) with respect to
+ * theta_e (alpha_e = exp(theta_e)) for the P1 problem
+ * -div(alpha grad u) = f, u = 0 on the boundary: given u (the forward
+ * solution) and lambda (the adjoint solution), g_e = -exp(theta_e) *
+ * lambda_e^T K1_e u_e with K1_e the unit-coefficient element stiffness
+ * (uscalfe/loc_comp_ellbvp.h:266-339 with alpha = 1, gamma = 0). */
+int lfref_adjoint_gradient_p1(int ncells, const double *xy,
+                              const int32_t *cell_v, const double *theta,
+                              const double *u, const double *lam,
+                              double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,336 @@
+"""numpy front end of the CPU oracle (oracle/liblfref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke``
+and by the CPU-baseline leg of ``bench.py``.  Never used by the product
+library.  See oracle/lfref.h for the reference file:line each function
+restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64 = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i8 = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liblfref.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        sig = {
+            "lfref_quad_rule": (C.c_int, [C.c_int, _d, _d]),
+            "lfref_shape_functions": (C.c_int, [C.c_int, C.c_int, _d, _d, _d]),
+            "lfref_tp_triag_mesh": (None, [C.c_int, C.c_int, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, _d, _i32, _i32]),
+            "lfref_mesh_edges": (C.c_int, [C.c_int, C.c_int, _i32, C.c_int, _i32, C.c_int,
+                                           _i32, _i32, _i8]),
+            "lfref_dof_table": (C.c_int64, [C.c_int, C.c_int, C.c_int, _i32, _i32, _i8,
+                                            C.c_int, C.c_int, _i64]),
+            "lfref_boundary_flags": (None, [C.c_int, C.c_int, C.c_int, _i32, _i32, _u8, _u8]),
+            "lfref_geometry": (C.c_int, [C.c_int, _d, _i32, _d, _d, _d]),
+            "lfref_element_matrices": (C.c_int, [C.c_int, C.c_int, _d, _i32, _d, _d, _d]),
+            "lfref_element_vectors": (C.c_int, [C.c_int, C.c_int, _d, _i32, C.c_int,
+                                                C.c_double, C.c_void_p, _d]),
+            "lfref_qp_coords": (C.c_int, [C.c_int, C.c_int, _d, _i32, _d]),
+            "lfref_assemble_coo": (None, [C.c_int, C.c_int, _i64, _d, _i32, _i32, _d]),
+            "lfref_assemble_vector": (None, [C.c_int, C.c_int, _i64, _d, C.c_int64, _d]),
+            "lfref_coo_matvec": (None, [C.c_int64, _i32, _i32, _d, C.c_double, _d, _d]),
+            "lfref_fix_flagged": (C.c_int64, [C.c_int64, _i32, _i32, _d, C.c_int64, _u8,
+                                              _d, _d]),
+            "lfref_set_from_triplets": (C.c_int64, [C.c_int64, _i32, _i32, _d, C.c_int32,
+                                                    _i32, _i32, _d]),
+            "lfref_band_cholesky_solve": (C.c_int, [C.c_int32, _i32, _i32, _d, _d, _d]),
+            "lfref_csr_spmv": (None, [C.c_int32, _i32, _i32, _d, _d, _d]),
+            "lfref_pcg": (C.c_int, [C.c_int32, _i32, _i32, _d, _d, _d, C.c_double, C.c_int,
+                                    C.POINTER(C.c_double)]),
+            "lfref_cpu_path_assemble": (C.c_int64, [C.c_int, C.c_int, _d, _i32, _i64,
+                                                    C.c_int64, _d, C.c_double, C.c_int,
+                                                    C.c_double, _u8, _i32, _i32, _d, _d]),
+            "lfref_adjoint_gradient_p1": (C.c_int, [C.c_int, _d, _i32, _d, _d, _d, _d]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def nqp_for_degree(degree: int) -> int:
+    return {1: 3, 3: 12}[degree]
+
+
+def quad_rule(qdeg: int):
+    pts = np.zeros(24)
+    w = np.zeros(12)
+    n = lib().lfref_quad_rule(qdeg, pts, w)
+    if n < 0:
+        raise ValueError("no triangle rule of degree %d in the oracle" % qdeg)
+    return pts[: 2 * n].reshape(n, 2).copy(), w[:n].copy()
+
+
+def shape_functions(degree: int, pts: np.ndarray):
+    """Values [nsf x npts] and gradients [nsf x 2*npts] (Eigen column layout)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    npts = pts.shape[0]
+    nsf = 3 if degree == 1 else 10
+    N = np.zeros(nsf * npts)
+    dN = np.zeros(nsf * 2 * npts)
+    lib().lfref_shape_functions(degree, npts, pts.ravel(), N, dN)
+    return N.reshape(npts, nsf).T.copy(), dN.reshape(2 * npts, nsf).T.copy()
+
+
+class TriMesh:
+    """Triangle mesh arrays: xy [nv,2], cells [ne,3], edges [E,2], cell_e, cell_ori."""
+
+    def __init__(self, xy, cells, pre_edges=None):
+        self.xy = np.ascontiguousarray(xy, dtype=np.float64)
+        self.cells = np.ascontiguousarray(cells, dtype=np.int32)
+        self.nv = self.xy.shape[0]
+        self.ne = self.cells.shape[0]
+        pre = (np.zeros((0, 2), np.int32) if pre_edges is None
+               else np.ascontiguousarray(pre_edges, dtype=np.int32))
+        cap = pre.shape[0] + 3 * self.ne
+        ev = np.zeros(2 * cap, np.int32)
+        ce = np.zeros(3 * self.ne, np.int32)
+        co = np.zeros(3 * self.ne, np.int8)
+        E = lib().lfref_mesh_edges(self.nv, self.ne, self.cells.ravel(), pre.shape[0],
+                                   pre.ravel() if pre.size else np.zeros(2, np.int32),
+                                   cap, ev, ce, co)
+        assert E >= 0
+        self.nedges = E
+        self.edges = ev[: 2 * E].reshape(E, 2).copy()
+        self.cell_e = ce.reshape(self.ne, 3)
+        self.cell_ori = co.reshape(self.ne, 3)
+
+    def boundary_flags(self):
+        nf = np.zeros(self.nv, np.uint8)
+        ef = np.zeros(self.nedges, np.uint8)
+        lib().lfref_boundary_flags(self.nv, self.ne, self.nedges, self.cell_e.ravel(),
+                                   self.edges.ravel(), nf, ef)
+        return nf, ef
+
+    def dofs(self, degree: int):
+        """(cell_dofs [ne, nloc] int64, ndofs) of UniformFEDofHandler."""
+        ndof_edge, ndof_cell = (0, 0) if degree == 1 else (2, 1)
+        nloc = 3 + 3 * ndof_edge + ndof_cell
+        cd = np.zeros(self.ne * nloc, np.int64)
+        nd = lib().lfref_dof_table(self.nv, self.ne, self.nedges, self.cells.ravel(),
+                                   self.cell_e.ravel(), self.cell_ori.ravel(), ndof_edge,
+                                   ndof_cell, cd)
+        return cd.reshape(self.ne, nloc), int(nd)
+
+    def dirichlet_flags(self, degree: int):
+        """Boundary DOF flags: boundary nodes, plus interior DOFs of boundary edges."""
+        nf, ef = self.boundary_flags()
+        if degree == 1:
+            return nf.copy()
+        _, nd = self.dofs(degree)
+        fl = np.zeros(nd, np.uint8)
+        fl[: self.nv] = nf
+        be = np.nonzero(ef)[0]
+        fl[self.nv + 2 * be] = 1
+        fl[self.nv + 2 * be + 1] = 1
+        return fl
+
+
+def tp_triag_mesh(nx: int, ny: int | None = None, box=(0.0, 0.0, 1.0, 1.0)) -> TriMesh:
+    ny = nx if ny is None else ny
+    nv = (nx + 1) * (ny + 1)
+    ne = 2 * nx * ny
+    E = nx * ny + (nx + 1) * ny + nx * (ny + 1)
+    xy = np.zeros(2 * nv)
+    cells = np.zeros(3 * ne, np.int32)
+    edges = np.zeros(2 * E, np.int32)
+    lib().lfref_tp_triag_mesh(nx, ny, box[0], box[1], box[2], box[3], xy, cells, edges)
+    return TriMesh(xy.reshape(nv, 2), cells.reshape(ne, 3), edges.reshape(E, 2))
+
+
+def geometry(mesh: TriMesh):
+    J = np.zeros(4 * mesh.ne)
+    Jit = np.zeros(4 * mesh.ne)
+    det = np.zeros(mesh.ne)
+    rc = lib().lfref_geometry(mesh.ne, mesh.xy.ravel(), mesh.cells.ravel(), J, Jit, det)
+    if rc != 0:
+        raise ValueError("degenerate cell %d" % (-rc - 1))
+    return J.reshape(-1, 2, 2), Jit.reshape(-1, 2, 2), det
+
+
+def _qp_array(v, ne, nqp):
+    v = np.asarray(v, dtype=np.float64)
+    if v.ndim == 0:
+        return np.full(ne * nqp, float(v))
+    if v.size == ne:
+        return np.ascontiguousarray(np.repeat(v.ravel(), nqp))
+    assert v.size == ne * nqp
+    return np.ascontiguousarray(v.ravel())
+
+
+def element_matrices(mesh: TriMesh, degree: int, alpha, gamma):
+    q = nqp_for_degree(degree)
+    nsf = 3 if degree == 1 else 10
+    out = np.zeros(mesh.ne * nsf * nsf)
+    rc = lib().lfref_element_matrices(degree, mesh.ne, mesh.xy.ravel(), mesh.cells.ravel(),
+                                      _qp_array(alpha, mesh.ne, q),
+                                      _qp_array(gamma, mesh.ne, q), out)
+    if rc != 0:
+        raise ValueError("element matrices failed: %d" % rc)
+    return out.reshape(mesh.ne, nsf, nsf)
+
+
+F_CONST, F_SINSIN, F_QP = 0, 1, 2
+
+
+def element_vectors(mesh: TriMesh, degree: int, f_kind=F_CONST, f_const=1.0, f_qp=None):
+    nsf = 3 if degree == 1 else 10
+    out = np.zeros(mesh.ne * nsf)
+    fq = None
+    if f_kind == F_QP:
+        fq = np.ascontiguousarray(f_qp, dtype=np.float64).ravel()
+    rc = lib().lfref_element_vectors(degree, mesh.ne, mesh.xy.ravel(), mesh.cells.ravel(),
+                                     f_kind, f_const,
+                                     None if fq is None else fq.ctypes.data, out)
+    if rc != 0:
+        raise ValueError("element vectors failed: %d" % rc)
+    return out.reshape(mesh.ne, nsf)
+
+
+def qp_coords(mesh: TriMesh, degree: int):
+    q = nqp_for_degree(degree)
+    out = np.zeros(mesh.ne * q * 2)
+    lib().lfref_qp_coords(degree, mesh.ne, mesh.xy.ravel(), mesh.cells.ravel(), out)
+    return out.reshape(mesh.ne, q, 2)
+
+
+def assemble_coo(cell_dofs, mats):
+    ne, nloc = cell_dofs.shape
+    nnz = ne * nloc * nloc
+    r = np.zeros(nnz, np.int32)
+    c = np.zeros(nnz, np.int32)
+    v = np.zeros(nnz)
+    lib().lfref_assemble_coo(ne, nloc, np.ascontiguousarray(cell_dofs, np.int64).ravel(),
+                             np.ascontiguousarray(mats).ravel(), r, c, v)
+    return r, c, v
+
+
+def assemble_vector(cell_dofs, vecs, ndofs):
+    ne, nloc = cell_dofs.shape
+    b = np.zeros(ndofs)
+    lib().lfref_assemble_vector(ne, nloc, np.ascontiguousarray(cell_dofs, np.int64).ravel(),
+                                np.ascontiguousarray(vecs).ravel(), ndofs, b)
+    return b
+
+
+def coo_matvec(rows, cols, vals, alpha, x, y):
+    lib().lfref_coo_matvec(len(rows), rows, cols, vals, alpha,
+                           np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def fix_flagged(rows, cols, vals, flags, fixed_vals, b):
+    ndofs = len(flags)
+    nnz = len(rows)
+    cap = nnz + ndofs
+    R = np.zeros(cap, np.int32)
+    Cc = np.zeros(cap, np.int32)
+    V = np.zeros(cap)
+    R[:nnz], Cc[:nnz], V[:nnz] = rows, cols, vals
+    b = np.array(b, dtype=np.float64)
+    n = lib().lfref_fix_flagged(nnz, R, Cc, V, ndofs, np.ascontiguousarray(flags, np.uint8),
+                                np.ascontiguousarray(fixed_vals, np.float64), b)
+    return R[:n].copy(), Cc[:n].copy(), V[:n].copy(), b
+
+
+def set_from_triplets(rows, cols, vals, nrows):
+    nnz = len(rows)
+    rp = np.zeros(nrows + 1, np.int32)
+    col = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1))
+    n = lib().lfref_set_from_triplets(nnz, np.ascontiguousarray(rows, np.int32),
+                                      np.ascontiguousarray(cols, np.int32),
+                                      np.ascontiguousarray(vals, np.float64), nrows, rp,
+                                      col, val)
+    return rp, col[:n].copy(), val[:n].copy()
+
+
+def band_cholesky_solve(rp, col, val, b):
+    x = np.zeros(len(rp) - 1)
+    rc = lib().lfref_band_cholesky_solve(len(rp) - 1, rp, col, val,
+                                         np.ascontiguousarray(b, np.float64), x)
+    if rc != 0:
+        raise ValueError("band Cholesky failed: %d" % rc)
+    return x
+
+
+def csr_spmv(rp, col, val, x):
+    y = np.zeros(len(rp) - 1)
+    lib().lfref_csr_spmv(len(rp) - 1, rp, col, val, np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def pcg(rp, col, val, b, tol=1e-10, maxit=100000, x0=None):
+    n = len(rp) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    err = C.c_double(0.0)
+    it = lib().lfref_pcg(n, rp, col, val, np.ascontiguousarray(b, np.float64), x, tol,
+                         maxit, C.byref(err))
+    return x, it, err.value
+
+
+def cpu_path_assemble(mesh: TriMesh, degree, cell_dofs, ndofs, alpha_cell, gamma_const,
+                      f_kind, f_const, fixed):
+    """Reference-faithful single-thread path: Eval -> triplets -> fix -> CSR."""
+    nloc = cell_dofs.shape[1]
+    cap = mesh.ne * nloc * nloc + ndofs
+    rp = np.zeros(ndofs + 1, np.int32)
+    col = np.zeros(cap, np.int32)
+    val = np.zeros(cap)
+    b = np.zeros(ndofs)
+    n = lib().lfref_cpu_path_assemble(degree, mesh.ne, mesh.xy.ravel(), mesh.cells.ravel(),
+                                      np.ascontiguousarray(cell_dofs, np.int64).ravel(),
+                                      ndofs, np.ascontiguousarray(alpha_cell, np.float64),
+                                      gamma_const, f_kind, f_const,
+                                      np.ascontiguousarray(fixed, np.uint8), rp, col, val, b)
+    if n < 0:
+        raise ValueError("cpu path failed: %d" % n)
+    return rp, col[:n].copy(), val[:n].copy(), b
+
+
+def adjoint_gradient_p1(mesh: TriMesh, theta, u, lam):
+    g = np.zeros(mesh.ne)
+    rc = lib().lfref_adjoint_gradient_p1(mesh.ne, mesh.xy.ravel(), mesh.cells.ravel(),
+                                         np.ascontiguousarray(theta, np.float64),
+                                         np.ascontiguousarray(u, np.float64),
+                                         np.ascontiguousarray(lam, np.float64), g)
+    assert rc == 0
+    return g
+
+
+def galerkin_system(mesh: TriMesh, degree: int, alpha, gamma, f_kind=F_CONST, f_const=1.0,
+                    f_qp=None, flags=None, fixed_vals=None):
+    """The homDir_linfe_demo.cc:130-170 pipeline: element matrices -> COO ->
+    load vector -> FixFlaggedSolutionComponents -> setFromTriplets (CSR)."""
+    cd, nd = mesh.dofs(degree)
+    mats = element_matrices(mesh, degree, alpha, gamma)
+    r, c, v = assemble_coo(cd, mats)
+    vecs = element_vectors(mesh, degree, f_kind, f_const, f_qp)
+    b = assemble_vector(cd, vecs, nd)
+    if flags is None:
+        flags = mesh.dirichlet_flags(degree)
+    if fixed_vals is None:
+        fixed_vals = np.zeros(nd)
+    r, c, v, b = fix_flagged(r, c, v, flags, fixed_vals, b)
+    rp, col, val = set_from_triplets(r, c, v, nd)
+    return rp, col, val, b

@@ -1,0 +1,339 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: integer/index data (patterns, DOF maps, row pointers) bit-exact;
+element-level FP64 values bit-exact where both sides evaluate the same
+expression sequence (the kernels are built with -ffp-contract=off), and
+otherwise within the tolerance written in each test (1e-12 relative to the
+row's infinity norm for assembled matrices, as the element couplings across
+right angles are cancellation-prone near-zeros).
+"""
+import numpy as np
+import pytest
+
+from lehrfempp_amd import capi
+from oracle import lfref as L
+
+pytestmark = pytest.mark.gpu
+
+
+def jittered_mesh(n, amount=0.2, seed=11):
+    m = L.tp_triag_mesh(n)
+    xy = m.xy.copy()
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    inner = (xy[:, 0] > 0) & (xy[:, 0] < 1) & (xy[:, 1] > 0) & (xy[:, 1] < 1)
+    xy[inner] += rng.uniform(-amount * h, amount * h, size=(inner.sum(), 2))
+    return L.TriMesh(xy, m.cells, m.edges)
+
+
+def gpu_mesh(ctx, m: L.TriMesh, degree):
+    if degree == 1:
+        return capi.Mesh(ctx, m.xy, m.cells)
+    cd, nd = m.dofs(degree)
+    return capi.Mesh(ctx, m.xy, m.cells, cd, nd)
+
+
+def csr_to_dense_rows(rp, col, val):
+    return {(r, int(c)): v for r in range(len(rp) - 1)
+            for c, v in zip(col[rp[r]:rp[r + 1]], val[rp[r]:rp[r + 1]])}
+
+
+def row_scaled_err(rp, val_a, val_b):
+    err = 0.0
+    for r in range(len(rp) - 1):
+        a = val_a[rp[r]:rp[r + 1]]
+        b = val_b[rp[r]:rp[r + 1]]
+        if len(a):
+            scale = max(np.abs(b).max(), 1e-300)
+            err = max(err, np.abs(a - b).max() / scale)
+    return err
+
+
+# ------------------------------------------------------------- element level
+def test_geometry_bitexact(ctx):
+    m = jittered_mesh(16)
+    g = gpu_mesh(ctx, m, 1)
+    J, Jit, det = capi.geometry(ctx, g)
+    Jo, Jito, deto = L.geometry(m)
+    assert np.array_equal(J, Jo) and np.array_equal(Jit, Jito) and np.array_equal(det, deto)
+    g.close()
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+@pytest.mark.parametrize("kind", ["const", "cell", "qp"])
+def test_element_matrices_bitexact(ctx, degree, kind):
+    m = jittered_mesh(8 if degree == 3 else 16)
+    g = gpu_mesh(ctx, m, degree)
+    fe = capi.FE(ctx, degree)
+    rng = np.random.default_rng(3)
+    q = fe.nqp
+    if kind == "const":
+        a_h, c_h = 1.7, 0.3
+        ca, cg = capi.coeff(capi.COEFF_CONST, a_h), capi.coeff(capi.COEFF_CONST, c_h)
+        vecs = []
+    elif kind == "cell":
+        a_h = np.exp(rng.standard_normal(m.ne))
+        c_h = rng.uniform(0, 2, m.ne)
+        va, vc = capi.Vec(ctx, a_h), capi.Vec(ctx, c_h)
+        ca, cg = capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CELL, 1.0, vc)
+        vecs = [va, vc]
+    else:
+        a_h = rng.uniform(0.5, 2, (m.ne, q))
+        c_h = rng.uniform(0, 1, (m.ne, q))
+        va, vc = capi.Vec(ctx, a_h), capi.Vec(ctx, c_h)
+        ca, cg = capi.coeff(capi.COEFF_QP, 1.0, va), capi.coeff(capi.COEFF_QP, 1.0, vc)
+        vecs = [va, vc]
+    Kg = capi.element_matrices(ctx, g, fe, ca, cg)
+    Ko = L.element_matrices(m, degree, a_h, c_h)
+    assert np.array_equal(Kg, Ko), np.abs(Kg - Ko).max()
+    for v in vecs:
+        v.close()
+    fe.close()
+    g.close()
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_element_vectors(ctx, degree):
+    m = jittered_mesh(8)
+    g = gpu_mesh(ctx, m, degree)
+    fe = capi.FE(ctx, degree)
+    v = capi.element_vectors(ctx, g, fe, capi.coeff(capi.COEFF_CONST, 2.5))
+    assert np.array_equal(v, L.element_vectors(m, degree, L.F_CONST, 2.5))
+    vs = capi.element_vectors(ctx, g, fe, capi.coeff(capi.COEFF_SINSIN, 1.0))
+    vo = L.element_vectors(m, degree, L.F_SINSIN)
+    # device sin vs glibc sin may differ in the last bit
+    assert np.abs(vs - vo).max() <= 1e-15 * np.abs(vo).max()
+    fe.close()
+    g.close()
+
+
+# ---------------------------------------------------------------- assembly
+def _assemble_both(ctx, m, degree, alpha_cell, gamma, f_kind, fixed_vals=None):
+    cd, nd = m.dofs(degree)
+    flags = m.dirichlet_flags(degree)
+    g = gpu_mesh(ctx, m, degree)
+    fe = capi.FE(ctx, degree)
+    plan = capi.Plan(ctx, g, flags)
+    va = capi.Vec(ctx, alpha_cell)
+    rhs = capi.Vec(ctx, nd)
+    fv = capi.Vec(ctx, fixed_vals) if fixed_vals is not None else None
+    f = capi.coeff(capi.COEFF_SINSIN if f_kind == L.F_SINSIN else capi.COEFF_CONST, 1.0)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, gamma),
+                  f, rhs, fv)
+    rp, col, val = plan.csr.download()
+    b = rhs.numpy()
+    ro = L.galerkin_system(m, degree, alpha_cell, gamma, f_kind, 1.0, flags=flags,
+                           fixed_vals=fixed_vals)
+    for h in (va, rhs, fv, plan, fe, g):
+        if h is not None:
+            h.close()
+    return (rp, col, val, b), ro
+
+
+@pytest.mark.parametrize("degree,n", [(1, 64), (1, 37), (3, 8), (3, 13)])
+def test_galerkin_csr_matches_oracle(ctx, degree, n):
+    m = jittered_mesh(n) if degree == 3 else L.tp_triag_mesh(n)
+    rng = np.random.default_rng(n)
+    alpha = np.exp(rng.standard_normal(m.ne))
+    (rp, col, val, b), (rpo, colo, valo, bo) = _assemble_both(ctx, m, degree, alpha, 1.0,
+                                                              L.F_CONST)
+    assert np.array_equal(rp, rpo) and np.array_equal(col, colo)  # pattern bit-exact
+    # every entry is summed in triplet order from bit-identical element values
+    assert np.array_equal(val, valo), row_scaled_err(rp, val, valo)
+    assert np.array_equal(b, bo)
+
+
+def test_galerkin_sinsin_rhs_tolerance(ctx):
+    m = L.tp_triag_mesh(48)
+    alpha = np.exp(np.random.default_rng(0).standard_normal(m.ne))
+    (rp, col, val, b), (rpo, colo, valo, bo) = _assemble_both(ctx, m, 1, alpha, 1.0, L.F_SINSIN)
+    assert np.array_equal(val, valo)
+    assert np.abs(b - bo).max() <= 1e-15 * np.abs(bo).max() * 8
+
+
+def test_nonzero_dirichlet_values(ctx):
+    # FixFlaggedSolutionComponents with prescribed values (fix_dof.h:97-114)
+    m = L.tp_triag_mesh(20)
+    rng = np.random.default_rng(9)
+    alpha = np.exp(rng.standard_normal(m.ne))
+    fl = m.dirichlet_flags(1)
+    gvals = rng.standard_normal(m.nv) * fl
+    (rp, col, val, b), (rpo, colo, valo, bo) = _assemble_both(ctx, m, 1, alpha, 0.5, L.F_CONST,
+                                                              fixed_vals=gvals)
+    assert np.array_equal(col, colo) and np.array_equal(val, valo)
+    assert np.array_equal(b, bo), np.abs(b - bo).max()
+
+
+def test_assembly_is_deterministic(ctx):
+    m = L.tp_triag_mesh(96)
+    g = gpu_mesh(ctx, m, 1)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, m.dirichlet_flags(1))
+    va = capi.Vec(ctx, np.exp(np.random.default_rng(1).standard_normal(m.ne)))
+    rhs = capi.Vec(ctx, m.nv)
+    outs = []
+    for _ in range(3):
+        plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 1.0),
+                      capi.coeff(capi.COEFF_SINSIN, 1.0), rhs)
+        outs.append((plan.csr.download()[2].copy(), rhs.numpy()))
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+    for h in (va, rhs, plan, fe, g):
+        h.close()
+
+
+# ------------------------------------------------------------ COO -> CSR
+def test_coo_to_csr_golden(ctx, golden):
+    g = golden["test_mesh0"]
+    rows, cols, vals = [], [], []
+    for c, cell in enumerate(g["cells"]):
+        for i in cell:
+            for j in cell:
+                rows.append(i)
+                cols.append(j)
+                vals.append(float(c) if i == j else -float(c))
+    A = capi.coo_to_csr(ctx, rows, cols, vals, 10, 10)
+    rp, col, val = A.download()
+    dense = np.zeros((10, 10))
+    for r in range(10):
+        dense[r, col[rp[r]:rp[r + 1]]] = val[rp[r]:rp[r + 1]]
+    ref = np.array(golden["test_assembler_matrix"]["matrix"], dtype=float).reshape(10, 10)
+    assert np.array_equal(dense, ref)
+    rpo, colo, valo = L.set_from_triplets(rows, cols, vals, 10)
+    assert np.array_equal(rp, rpo) and np.array_equal(col, colo) and np.array_equal(val, valo)
+    A.close()
+
+
+def test_coo_to_csr_insertion_order_and_random(ctx):
+    A = capi.coo_to_csr(ctx, [0, 0, 0], [1, 1, 1], [1e16, 1.0, -1e16], 2, 2)
+    assert A.download()[2][0] == (1e16 + 1.0) + (-1e16)
+    A.close()
+    rng = np.random.default_rng(2)
+    nnz = 200000
+    r = rng.integers(0, 3000, nnz).astype(np.int32)
+    c = rng.integers(0, 2500, nnz).astype(np.int32)
+    v = rng.standard_normal(nnz)
+    A = capi.coo_to_csr(ctx, r, c, v, 3000, 2500)
+    got = A.download()
+    exp = L.set_from_triplets(r, c, v, 3000)
+    for a, b in zip(got, exp):
+        assert np.array_equal(a, b)
+    A.close()
+    with pytest.raises(capi.LfgpuError) as e:
+        capi.coo_to_csr(ctx, [0, 5], [0, 0], [1.0, 1.0], 3, 3)
+    assert e.value.code == capi.ERANGE
+
+
+# ---------------------------------------------------------- SpMV and PCG
+def _c1_system(ctx, n=64, alpha=None, gamma=0.0, f_kind=L.F_CONST):
+    m = L.tp_triag_mesh(n)
+    if alpha is None:
+        alpha = np.ones(m.ne)
+    g = gpu_mesh(ctx, m, 1)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, m.dirichlet_flags(1))
+    va = capi.Vec(ctx, alpha)
+    b = capi.Vec(ctx, m.nv)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, gamma),
+                  capi.coeff(capi.COEFF_SINSIN if f_kind == L.F_SINSIN else capi.COEFF_CONST, 1.0),
+                  b)
+    return m, g, fe, plan, va, b
+
+
+def test_spmv_matches_oracle(ctx):
+    m, g, fe, plan, va, b = _c1_system(ctx, 80, np.exp(np.random.default_rng(4).standard_normal(2 * 80 * 80)))
+    rp, col, val = plan.csr.download()
+    x = np.random.default_rng(5).standard_normal(m.nv)
+    xv, yv = capi.Vec(ctx, x), capi.Vec(ctx, m.nv)
+    capi.spmv(ctx, plan.csr, xv, yv)
+    y = yv.numpy()
+    yo = L.csr_spmv(rp, col, val, x)
+    assert np.abs(y - yo).max() <= 1e-14 * np.abs(yo).max()
+    for h in (xv, yv, va, b, plan, fe, g):
+        h.close()
+
+
+def test_pcg_fp64_c1_against_direct_solve(ctx):
+    # -Laplace u = 1, 64x64, zero Dirichlet; SparseLU's role is
+    # played by the oracle's banded Cholesky
+    m, g, fe, plan, va, b = _c1_system(ctx)
+    rp, col, val = plan.csr.download()
+    bh = b.numpy()
+    x_direct = L.band_cholesky_solve(rp, col, val, bh)
+    x = capi.Vec(ctx, m.nv)
+    info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10)
+    xo, ito, erro = L.pcg(rp, col, val, bh, tol=1e-10)
+    assert info.converged and info.relres < 1e-10
+    assert abs(info.iters - ito) <= 2  # same algorithm, different summation order
+    xg = x.numpy()
+    assert np.linalg.norm(xg - x_direct) <= 1e-8 * np.linalg.norm(x_direct)
+    # tighter solve reaches the direct solution to ~1e-12
+    x.fill(0.0)
+    info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-14)
+    assert np.linalg.norm(x.numpy() - x_direct) <= 1e-12 * np.linalg.norm(x_direct)
+    for h in (x, va, b, plan, fe, g):
+        h.close()
+
+
+def test_pcg_mixed_c1_against_direct_solve(ctx):
+    alpha = np.exp(np.random.default_rng(8).standard_normal(2 * 64 * 64))
+    m, g, fe, plan, va, b = _c1_system(ctx, 64, alpha, 1.0, L.F_SINSIN)
+    rp, col, val = plan.csr.download()
+    x_direct = L.band_cholesky_solve(rp, col, val, b.numpy())
+    x = capi.Vec(ctx, m.nv)
+    info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, mode=capi.PCG_MIXED)
+    assert info.converged and info.relres < 1e-10 and info.refinements >= 1
+    xg = x.numpy()
+    r = b.numpy() - L.csr_spmv(rp, col, val, xg)  # FP64 residual certificate
+    assert np.linalg.norm(r) < 1e-10 * np.linalg.norm(b.numpy())
+    assert np.linalg.norm(xg - x_direct) <= 1e-7 * np.linalg.norm(x_direct)
+    for h in (x, va, b, plan, fe, g):
+        h.close()
+
+
+def test_pcg_error_paths(ctx):
+    m, g, fe, plan, va, b = _c1_system(ctx, 32)
+    x = capi.Vec(ctx, m.nv)
+    info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, maxit=3, raise_on_noconv=False)
+    assert not info.converged and info.iters == 3
+    with pytest.raises(capi.LfgpuError) as e:
+        capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, maxit=3)
+    assert e.value.code == capi.ENOCONV
+    # zero rhs gives x = 0 in zero iterations (Eigen semantics)
+    z = capi.Vec(ctx, m.nv)
+    x.fill(1.0)
+    info = capi.pcg(ctx, plan.csr, z, x, rtol=1e-10)
+    assert info.iters == 0 and np.all(x.numpy() == 0.0)
+    for h in (z, x, va, b, plan, fe, g):
+        h.close()
+
+
+def test_error_codes(ctx):
+    with pytest.raises(capi.LfgpuError) as e:
+        capi.FE(ctx, 2)
+    assert e.value.code == capi.ENORULE
+    with pytest.raises(capi.LfgpuError) as e:
+        capi.Mesh(ctx, np.array([[0, 0], [1, 1], [2, 2.0]]), np.array([[0, 1, 2]]))
+    assert e.value.code == capi.EDEGENERATE
+    with pytest.raises(capi.LfgpuError) as e:
+        capi.Mesh(ctx, np.array([[0, 0], [1, 0], [0, 1.0]]), np.array([[0, 1, 3]]))
+    assert e.value.code == capi.EDEGENERATE
+
+
+# ------------------------------------------------------------------ adjoint
+def test_adjoint_gradient_kernel_matches_oracle(ctx):
+    m = jittered_mesh(24)
+    rng = np.random.default_rng(12)
+    theta = rng.standard_normal(m.ne)
+    u = rng.standard_normal(m.nv)
+    lam = rng.standard_normal(m.nv)
+    g = gpu_mesh(ctx, m, 1)
+    fe = capi.FE(ctx, 1)
+    vt, vu, vl, vg = (capi.Vec(ctx, theta), capi.Vec(ctx, u), capi.Vec(ctx, lam),
+                      capi.Vec(ctx, m.ne))
+    capi.adjoint_gradient(ctx, g, fe, vt, vu, vl, vg)
+    go = L.adjoint_gradient_p1(m, theta, u, lam)
+    # exp on the device may differ from glibc's in the last bit
+    assert np.abs(vg.numpy() - go).max() <= 4e-16 * np.abs(go).max() * 4
+    for h in (vt, vu, vl, vg, fe, g):
+        h.close()

@@ -1,0 +1,391 @@
+"""Pins the CPU oracle (oracle/lfref.cc) against the reference's own golden
+vectors, closed forms and invariants.  CPU only.
+
+Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a transposed Jacobian, a wrong index or summation order) fails one of
+them; the reference test each one mirrors is cited.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import lfref as L
+
+
+# ---------------------------------------------------------------- quadrature
+@pytest.mark.parametrize("qdeg", [2, 6])
+def test_quad_rule_exactness(qdeg):
+    # quad/test/make_quad_rule_tests.cc: monomials x^i y^j, i + j <= degree
+    pts, w = L.quad_rule(qdeg)
+    assert abs(w.sum() - 0.5) < 1e-15
+    for i in range(qdeg + 1):
+        for j in range(qdeg + 1 - i):
+            exact = math.factorial(i) * math.factorial(j) / math.factorial(i + j + 2)
+            approx = float(np.sum(w * pts[:, 0] ** i * pts[:, 1] ** j))
+            assert abs(approx - exact) < 1e-14, (i, j)
+    # one degree higher is not integrated exactly by the degree-2 rule
+    if qdeg == 2:
+        errs = [abs(np.sum(w * pts[:, 0] ** i * pts[:, 1] ** (3 - i)) -
+                    math.factorial(i) * math.factorial(3 - i) / math.factorial(5))
+                for i in range(4)]
+        assert max(errs) > 1e-4
+
+
+def test_unsupported_rule():
+    with pytest.raises(ValueError):
+        L.quad_rule(3)
+
+
+# ----------------------------------------------------------- shape functions
+def test_p1_shape_known_values(golden):
+    g = golden["p1_shape_values"]
+    N, dN = L.shape_functions(1, np.array(g["points"]))
+    # EXPECT_DOUBLE_EQ: equal to within 4 ulp
+    assert np.allclose(N.T, np.array(g["values"]), rtol=4 * 2.2e-16, atol=0)
+    assert np.array_equal(dN[:, :2], [[-1, -1], [1, 0], [0, 1]])
+
+
+def _p3_nodes():
+    t = 1.0 / 3.0
+    return np.array([[0, 0], [1, 0], [0, 1], [t, 0], [2 * t, 0], [2 * t, t], [t, 2 * t],
+                     [0, 2 * t], [0, t], [t, t]])
+
+
+def test_p3_cardinal_basis():
+    # lagr_fe_tests.cc:293-305: N_i(node_j) = delta_ij
+    N, _ = L.shape_functions(3, _p3_nodes())
+    assert np.allclose(N, np.eye(10), atol=1e-14)
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_gradients_match_finite_differences(degree):
+    rng = np.random.default_rng(4)
+    p = rng.uniform(0.05, 0.45, size=(7, 2))
+    _, dN = L.shape_functions(degree, p)
+    h = 1e-6
+    for d in range(2):
+        e = np.zeros(2)
+        e[d] = h
+        Np, _ = L.shape_functions(degree, p + e)
+        Nm, _ = L.shape_functions(degree, p - e)
+        fd = (Np - Nm) / (2 * h)
+        assert np.allclose(dN[:, d::2], fd, atol=1e-7)
+    N, _ = L.shape_functions(degree, p)
+    assert np.allclose(N.sum(axis=0), 1.0, atol=1e-14)  # partition of unity
+
+
+# ------------------------------------------------------------------ geometry
+def test_tria_o1_reference_triangle(golden):
+    nodes = np.array(golden["tria_o1_nodes"]["nodes"], dtype=float)
+    m = L.TriMesh(nodes, np.array([[0, 1, 2]]))
+    J, Jit, det = L.geometry(m)
+    assert np.array_equal(J[0], [[3.0, 2.0], [1.0, 4.0]])
+    assert det[0] == 10.0
+    assert np.allclose(Jit[0], np.linalg.inv(J[0]).T, rtol=1e-15)
+    # JacobianInverseGramian == J (J^T J)^-1 (check_jacobian_inverse_gramian.cc)
+    assert np.allclose(Jit[0], J[0] @ np.linalg.inv(J[0].T @ J[0]), rtol=1e-14)
+
+
+def test_geometry_random_triangles():
+    rng = np.random.default_rng(0)
+    xy = rng.uniform(-2, 3, size=(60, 2))
+    cells = np.arange(60, dtype=np.int32).reshape(20, 3)
+    m = L.TriMesh(xy, cells)
+    J, Jit, det = L.geometry(m)
+    for c in range(20):
+        a, b, d = xy[cells[c]]
+        Jn = np.column_stack([b - a, d - a])
+        assert np.allclose(J[c], Jn, rtol=0, atol=1e-15)
+        assert np.allclose(Jit[c], np.linalg.inv(Jn).T, rtol=1e-13)
+        assert abs(det[c] - abs(np.linalg.det(Jn))) < 1e-13 * abs(np.linalg.det(Jn))
+
+
+def test_degenerate_triangle_rejected():
+    m = L.TriMesh(np.array([[0, 0], [1, 1], [2, 2.0]]), np.array([[0, 1, 2]]))
+    with pytest.raises(ValueError):
+        L.geometry(m)
+
+
+# ----------------------------------------------------------- mesh / dofs
+def test_tp_mesh_counts():
+    # mesh/utils/test/tp_triag_mesh_builder_tests.cc: 2x2 -> 8 cells, 16 edges, 9 nodes
+    m = L.tp_triag_mesh(2)
+    assert (m.ne, m.nedges, m.nv) == (8, 16, 9)
+    nf, ef = m.boundary_flags()
+    assert nf.sum() == 8 and ef.sum() == 8
+
+
+def test_tp_mesh_closed_forms():
+    n = 3
+    m = L.tp_triag_mesh(n)
+    V = lambda i, j: i + j * (n + 1)  # noqa: E731
+    for i in range(n):
+        for j in range(n):
+            assert list(m.cells[2 * (i * n + j)]) == [V(i, j), V(i + 1, j + 1), V(i, j + 1)]
+            assert list(m.cells[2 * (i * n + j) + 1]) == [V(i, j), V(i + 1, j), V(i + 1, j + 1)]
+    # predefined edges keep the builder's order: horizontal, vertical, diagonal
+    assert list(m.edges[0]) == [V(0, 0), V(1, 0)]
+    assert list(m.edges[n * (n + 1)]) == [V(0, 0), V(0, 1)]
+    assert list(m.edges[n * (n + 1) + (n + 1) * n]) == [V(0, 0), V(1, 1)]
+    # up cell: e0 = diagonal (+), e1 = horizontal (i, j+1) (-), e2 = vertical (-)
+    assert list(m.cell_ori[0]) == [1, -1, -1]
+    assert list(m.cell_ori[1]) == [1, 1, -1]
+
+
+def test_test_mesh0_counts(golden):
+    # assembly_tests.cc:154-161 (the triangle-only part: quads are not
+    # represented by the triangle oracle, so count the triangles' edges)
+    g = golden["test_mesh0"]
+    assert g["num_cells"] == len(g["cells"]) and g["num_nodes"] == len(g["points"])
+    edges = set()
+    for c in g["cells"]:
+        for k in range(len(c)):
+            a, b = c[k], c[(k + 1) % len(c)]
+            edges.add((min(a, b), max(a, b)))
+    assert len(edges) == g["num_edges"]
+
+
+@pytest.mark.parametrize("n", [2, 3, 5])
+def test_p3_dofs_and_pattern_counts(n):
+    m = L.tp_triag_mesh(n)
+    cd, nd = m.dofs(3)
+    assert nd == m.nv + 2 * m.nedges + m.ne
+    assert set(np.unique(cd)) == set(range(nd))
+    rp, col, val, b = L.galerkin_system(m, 3, 1.0, 1.0)
+    assert len(col) == 153 * n * n - 234 * n + 109  # nnz after the Dirichlet fix
+    # every cell's DOF set is shared consistently: interior edge DOFs are seen
+    # in opposite order by the two adjacent cells
+    fl = m.dirichlet_flags(3)
+    assert fl.sum() == 4 * n * 3
+
+
+def test_p1_pattern_counts():
+    m = L.tp_triag_mesh(64)
+    rp, col, val, b = L.galerkin_system(m, 1, 1.0, 0.0)
+    assert len(col) == 27537 and m.dirichlet_flags(1).sum() == 256
+
+
+# ----------------------------------------------------------- element level
+def test_p1_element_matrix_closed_form():
+    # uscalfe/lin_fe.cc:60-84: area * grad(lambda)^T grad(lambda); mass
+    # (area/12)(1 + delta_ij)
+    rng = np.random.default_rng(1)
+    xy = rng.uniform(0, 1, size=(30, 2))
+    cells = np.arange(30, dtype=np.int32).reshape(10, 3)
+    m = L.TriMesh(xy, cells)
+    alpha = rng.uniform(0.5, 2, size=10)
+    gamma = rng.uniform(0.5, 2, size=10)
+    K = L.element_matrices(m, 1, alpha, gamma)
+    for c in range(10):
+        a, b, d = xy[cells[c]]
+        area = 0.5 * abs(np.linalg.det(np.column_stack([b - a, d - a])))
+        X = np.column_stack([np.ones(3), xy[cells[c]]])
+        grads = np.linalg.inv(X)[1:, :]  # 2 x 3
+        ref = alpha[c] * area * grads.T @ grads + gamma[c] * area / 12 * (np.ones((3, 3)) + np.eye(3))
+        assert np.allclose(K[c], ref, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_element_invariants(degree):
+    # bvp_fe_tests.cc:57-60 (diffusion rows sum to 0), loc_comp_test.cc:46-84
+    # (1^T M 1 = |K|)
+    rng = np.random.default_rng(2)
+    xy = rng.uniform(0, 1, size=(24, 2))
+    cells = np.arange(24, dtype=np.int32).reshape(8, 3)
+    m = L.TriMesh(xy, cells)
+    _, _, det = L.geometry(m)
+    A = L.element_matrices(m, degree, 1.0, 0.0)
+    assert np.abs(A.sum(axis=2)).max() < 1e-10
+    M = L.element_matrices(m, degree, 0.0, 1.0)
+    assert np.allclose(M.sum(axis=(1, 2)), det / 2, rtol=1e-13)
+    assert np.allclose(M, M.transpose(0, 2, 1), rtol=1e-14, atol=1e-16)
+
+
+def test_p3_stiffness_energy_of_quadratic():
+    # the P3 interpolant of v = x^2 is exact; v^T K v = int |grad v|^2 = 4/3
+    m = L.tp_triag_mesh(4)
+    cd, nd = m.dofs(3)
+    rp, col, val, _ = L.set_from_triplets(*L.assemble_coo(cd, L.element_matrices(m, 3, 1.0, 0.0)),
+                                          nd) + (None,)
+    # DOF locations: evaluate the shape-function nodes through each cell's map
+    loc = np.zeros((nd, 2))
+    nodes = _p3_nodes()
+    for c in range(m.ne):
+        a, b, d = m.xy[m.cells[c]]
+        for k in range(10):
+            loc[cd[c, k]] = a + nodes[k, 0] * (b - a) + nodes[k, 1] * (d - a)
+    v = loc[:, 0] ** 2
+    Av = L.csr_spmv(rp, col, val, v)
+    assert abs(v @ Av - 4.0 / 3.0) < 1e-12
+
+
+def test_load_vector_equals_mass_times_linear_f():
+    # f linear: degree-2 quadrature is exact, so the load vector is M f_I
+    rng = np.random.default_rng(3)
+    xy = rng.uniform(0, 1, size=(18, 2))
+    cells = np.arange(18, dtype=np.int32).reshape(6, 3)
+    m = L.TriMesh(xy, cells)
+    q = L.qp_coords(m, 1)
+    fq = 1 + 2 * q[..., 0] + 3 * q[..., 1]
+    v = L.element_vectors(m, 1, L.F_QP, f_qp=fq)
+    M = L.element_matrices(m, 1, 0.0, 1.0)
+    fv = 1 + 2 * xy[cells][..., 0] + 3 * xy[cells][..., 1]
+    assert np.allclose(v, np.einsum("cij,cj->ci", M, fv), rtol=1e-13)
+    # constant f: entries sum to f |K|
+    _, _, det = L.geometry(m)
+    v1 = L.element_vectors(m, 1, L.F_CONST, 2.5)
+    assert np.allclose(v1.sum(axis=1), 2.5 * det / 2, rtol=1e-14)
+    # sin(pi x) sin(pi y) kind equals the same values passed per QP
+    vs = L.element_vectors(m, 1, L.F_SINSIN)
+    vq = L.element_vectors(m, 1, L.F_QP, f_qp=np.sin(np.pi * q[..., 0]) * np.sin(np.pi * q[..., 1]))
+    assert np.allclose(vs, vq, rtol=1e-15, atol=1e-18)
+
+
+# ------------------------------------------------- COO / setFromTriplets / fix
+def test_golden_assembly_matrix(golden):
+    # assembly_tests.cc:219-285 with TestAssembler on test mesh 0
+    g = golden["test_mesh0"]
+    rows, cols, vals = [], [], []
+    for c, cell in enumerate(g["cells"]):
+        for i in cell:
+            for j in cell:
+                rows.append(i)
+                cols.append(j)
+                vals.append(float(c) if i == j else -float(c))
+    rp, col, val = L.set_from_triplets(rows, cols, vals, 10)
+    dense = np.zeros((10, 10))
+    for r in range(10):
+        dense[r, col[rp[r]:rp[r + 1]]] = val[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(col[rp[r]:rp[r + 1]]) > 0)
+    ref = np.array(golden["test_assembler_matrix"]["matrix"], dtype=float).reshape(10, 10)
+    assert np.array_equal(dense, ref)
+    # explicit zeros from cell 0 are kept in the pattern
+    assert len(col) == len(set(zip(rows, cols)))
+
+
+def test_set_from_triplets_sums_in_insertion_order():
+    # three values whose sum depends on the order: (1e16 + 1) - 1e16 vs ...
+    rows = [0, 0, 0]
+    cols = [1, 1, 1]
+    vals = [1e16, 1.0, -1e16]
+    rp, col, val = L.set_from_triplets(rows, cols, vals, 2)
+    assert val[0] == (1e16 + 1.0) + (-1e16)
+    rp, col, val = L.set_from_triplets(rows, cols, [1.0, 1e16, -1e16], 2)
+    assert val[0] == (1.0 + 1e16) + (-1e16)
+
+
+def test_coo_matvec_golden(golden):
+    g = golden["coo_matvec"]
+    t = np.array(g["triplets_1based"])
+    rows = (t[:, 0] - 1).astype(np.int32)
+    cols = (t[:, 1] - 1).astype(np.int32)
+    vals = t[:, 2].astype(float)
+    x = np.linspace(0.5, 5.0, 10) + np.linspace(9.0, 0.0, 10)
+    y = L.coo_matvec(rows, cols, vals, 1.0, x, np.zeros(8))
+    assert np.array_equal(y, np.array(g["exact"], dtype=float))
+
+
+def test_fix_flagged_golden(golden):
+    g = golden["fix_dof"]
+    n = g["n"]
+    rows, cols, vals = [], [], []
+    for k in range(n):
+        if k > 0:
+            rows.append(k); cols.append(k - 1); vals.append(-1.0)
+        if k < n - 1:
+            rows.append(k); cols.append(k + 1); vals.append(-1.0)
+        rows.append(k); cols.append(k); vals.append(2.0)
+    b = np.arange(1, n + 1, dtype=float)
+    flags = np.zeros(n, np.uint8)
+    fv = np.ones(n)
+    for k, v in g["fixed"].items():
+        flags[int(k)] = 1
+        fv[int(k)] = v
+    r, c, v, b2 = L.fix_flagged(np.array(rows, np.int32), np.array(cols, np.int32),
+                                np.array(vals), flags, fv, b)
+    rp, col, val = L.set_from_triplets(r, c, v, n)
+    x = L.band_cholesky_solve(rp, col, val, b2)
+    assert np.linalg.norm(x - np.array(g["exact"])) < 1e-12
+
+
+# --------------------------------------------------------- global operator
+def test_energies_unit_square(golden):
+    # full_gal_tests.cc:49-236 on the unit square
+    m = L.tp_triag_mesh(8)
+    q = L.qp_coords(m, 1)
+    x, y = q[..., 0], q[..., 1]
+    coeffs = {"0": 0.0 * x, "1": 1.0 + 0.0 * x, "1+x": 1.0 + x, "1+x^2+y^2": 1 + x * x + y * y}
+    cd, nd = m.dofs(1)
+    for case in golden["energies_unit_square"]["cases"]:
+        A = L.element_matrices(m, 1, coeffs[case["alpha"]], coeffs[case["gamma"]])
+        rp, col, val = L.set_from_triplets(*L.assemble_coo(cd, A), nd)
+        v = np.ones(nd) if case["v"] == "1" else 2 * m.xy[:, 0] + m.xy[:, 1]
+        e = v @ L.csr_spmv(rp, col, val, v)
+        assert abs(e - case["energy"]) < 1e-12, case
+
+
+def test_manufactured_solution_converges():
+    # homDir_linfe_demo.cc:45-57: u = sin(pi x) sin(pi y), f = 2 pi^2 u
+    errs = []
+    for n in (16, 32):
+        m = L.tp_triag_mesh(n)
+        q = L.qp_coords(m, 1)
+        f = 2 * np.pi ** 2 * np.sin(np.pi * q[..., 0]) * np.sin(np.pi * q[..., 1])
+        rp, col, val, b = L.galerkin_system(m, 1, 1.0, 0.0, L.F_QP, f_qp=f)
+        u = L.band_cholesky_solve(rp, col, val, b)
+        ex = np.sin(np.pi * m.xy[:, 0]) * np.sin(np.pi * m.xy[:, 1])
+        errs.append(np.abs(u - ex).max())
+    assert 3.5 < errs[0] / errs[1] < 4.5  # O(h^2) nodal error
+
+
+def test_pcg_matches_direct_solve_c1():
+    m = L.tp_triag_mesh(64)
+    rp, col, val, b = L.galerkin_system(m, 1, 1.0, 0.0)
+    x = L.band_cholesky_solve(rp, col, val, b)
+    y, it, err = L.pcg(rp, col, val, b, tol=1e-12)
+    assert err < 1e-12 and 50 < it < 1000
+    assert np.abs(x - y).max() < 1e-12 * np.abs(x).max() * 100
+    # residual certificate of the direct solve
+    assert np.linalg.norm(L.csr_spmv(rp, col, val, x) - b) < 1e-12 * np.linalg.norm(b)
+
+
+def test_cpu_path_equals_pipeline():
+    m = L.tp_triag_mesh(12)
+    cd, nd = m.dofs(1)
+    rng = np.random.default_rng(5)
+    a = np.exp(rng.standard_normal(m.ne))
+    fl = m.dirichlet_flags(1)
+    r1 = L.cpu_path_assemble(m, 1, cd, nd, a, 1.0, L.F_SINSIN, 0.0, fl)
+    r2 = L.galerkin_system(m, 1, a, 1.0, L.F_SINSIN)
+    for u, v in zip(r1, r2):
+        assert np.array_equal(u, v)
+
+
+def test_adjoint_gradient_matches_finite_differences():
+    n = 6
+    m = L.tp_triag_mesh(n)
+    rng = np.random.default_rng(7)
+    theta = rng.standard_normal(m.ne)
+    fl = m.dirichlet_flags(1)
+    Mrp, Mcol, Mval, _ = L.galerkin_system(m, 1, 0.0, 1.0)
+    uobs = rng.standard_normal(m.nv) * (1 - fl)
+
+    def solve(th):
+        rp, col, val, b = L.galerkin_system(m, 1, np.exp(th), 0.0, L.F_CONST, 1.0)
+        return rp, col, val, L.band_cholesky_solve(rp, col, val, b)
+
+    def J(th):
+        u = solve(th)[3]
+        d = u - uobs
+        return 0.5 * d @ L.csr_spmv(Mrp, Mcol, Mval, d)
+
+    rp, col, val, u = solve(theta)
+    rhs = L.csr_spmv(Mrp, Mcol, Mval, u - uobs) * (1 - fl)
+    lam = L.band_cholesky_solve(rp, col, val, rhs)
+    g = L.adjoint_gradient_p1(m, theta, u, lam)
+    for _ in range(4):
+        d = rng.standard_normal(m.ne)
+        eps = 1e-6
+        fd = (J(theta + eps * d) - J(theta - eps * d)) / (2 * eps)
+        assert abs(fd - g @ d) < 1e-6 * max(1.0, abs(fd))
