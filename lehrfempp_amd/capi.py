@@ -33,7 +33,8 @@ class Coeff(C.Structure):
 
 class PcgOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("maxit", C.c_int), ("mode", C.c_int),
-                ("check_every", C.c_int)]
+                ("check_every", C.c_int), ("refine_delta", C.c_double),
+                ("refine_max_interval", C.c_int)]
 
 
 class PcgInfo(C.Structure):
@@ -333,9 +334,10 @@ def spmv_timed(ctx, A: Csr, x: Vec, y: Vec, reps: int):
 
 
 def pcg(ctx, A: Csr, b: Vec, x: Vec, rtol=1e-10, maxit=None, mode=PCG_FP64, check_every=64,
-        raise_on_noconv=True):
+        raise_on_noconv=True, refine_delta=0.0, refine_max_interval=0):
     n = A.info()[0]
-    opts = PcgOpts(rtol, int(2 * n if maxit is None else maxit), mode, check_every)
+    opts = PcgOpts(rtol, int(2 * n if maxit is None else maxit), mode, check_every,
+                   float(refine_delta), int(refine_max_interval))
     info = PcgInfo()
     rc = lib().lfgpu_pcg(ctx.h, A.h, b.h, x.h, C.byref(opts), C.byref(info))
     if rc != OK and (rc != ENOCONV or raise_on_noconv):

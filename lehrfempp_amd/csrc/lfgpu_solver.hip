@@ -27,6 +27,8 @@ namespace {
 
 constexpr int kSlice = 64;
 constexpr int kRedBlocks = 1024;  // fixed grid of the deterministic reductions
+constexpr int kMaxGrid = 2048;   // workgroups of the solver sweeps
+constexpr int kPartSlots = 6;    // partial-sum arrays of kMaxGrid doubles
 
 inline unsigned grid_for(int64_t n, int block) {
   int64_t g = (n + block - 1) / block;
@@ -50,31 +52,6 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     for (int q = 0; q < BLOCK / 64; ++q) t += sh[q];
   }
   return t;  // valid in thread 0
-}
-
-// Sum of n partials by one block, fixed order.
-template <int BLOCK>
-__device__ __forceinline__ double reduce_partials(const double *part, int n, double *sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += BLOCK)
-    v += __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return block_sum<BLOCK>(v, sh);
-}
-
-// Thread 0 publishes the block's partial; returns true in every thread of
-// the block that arrived last (all partials visible to it).
-__device__ __forceinline__ bool publish_partial(double *part, int slot, double v,
-                                                unsigned *counter, int *sh_flag) {
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(part + slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned prev =
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    *sh_flag = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  const bool last = *sh_flag != 0;
-  if (last) __threadfence();
-  return last;
 }
 
 __global__ void k_dot_partial(const double *x, const double *y, int64_t n, double *part) {
@@ -104,20 +81,22 @@ __global__ void k_axpby(double a, const double *x, double b, double *y, int64_t 
 // ===========================================================================
 // solver cache (sliced ELL mirror + work vectors)
 // ===========================================================================
+// Solver scalars.  Every kernel recomputes the scalars it needs from the
+// previous kernel's per-workgroup partial sums (identical order in every
+// workgroup, so all workgroups agree bit for bit); workgroup 0 alone writes
+// this state, and values that are read and written within one iteration are
+// double buffered by iteration parity.
 struct PcgState {
-  double rho;      // (r, z) of the current iterate
-  double rho_prev; // (r, z) of the previous iterate (mixed reliable update)
-  double alpha, beta;
-  double rr;       // ||r||^2 of the recurrence residual
-  double thr;      // tol^2 ||b||^2
-  double bb;
+  double rho[2];   // (r, D^-1 r), indexed by iteration parity
+  double thr;      // tol^2 ||b||^2 (on entry: tol^2)
+  double bb;       // ||b||^2
+  double rr;       // last ||r||^2 (recurrence in FP64 mode, true in mixed)
   double rr_ref;   // mixed: ||r||^2 at the last reliable update
-  double rr_true;  // mixed: last FP64 true residual norm^2
-  double pq;
   int iter, done, converged, maxit;
-  int refine_req;  // mixed: reliable update requested
+  int refine[2];   // mixed: reliable update decided in this iteration (parity)
+  int iterp[2];    // mixed: iteration count, indexed by parity
   int refinements;
-  unsigned counter[4];
+  int last_refine; // mixed: iteration of the last reliable update
 };
 
 struct lfgpu_solver_cache {
@@ -129,12 +108,15 @@ struct lfgpu_solver_cache {
   float *s_val32 = nullptr;
   double *dinv = nullptr;
   float *dinv32 = nullptr;
+  double *rowsum = nullptr;
+  float *rowsum32 = nullptr;
   uint64_t built_version = ~0ull;
   bool have32 = false;
   // work vectors
   double *r = nullptr, *p = nullptr, *q = nullptr;
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *xa32 = nullptr;
-  double *part = nullptr;  // [4 x kMaxBlocks]
+  double *xlo = nullptr;   // mixed: low part of the double-double solution
+  double *part = nullptr;  // [kPartSlots x grid] partial sums
   int part_len = 0;
   PcgState *st = nullptr;
 };
@@ -149,6 +131,8 @@ void solver_cache_free(lfgpu_csr *A) {
   dfree(c->s_val32);
   dfree(c->dinv);
   dfree(c->dinv32);
+  dfree(c->rowsum);
+  dfree(c->rowsum32);
   dfree(c->r);
   dfree(c->p);
   dfree(c->q);
@@ -156,6 +140,7 @@ void solver_cache_free(lfgpu_csr *A) {
   dfree(c->p32);
   dfree(c->q32);
   dfree(c->xa32);
+  dfree(c->xlo);
   dfree(c->part);
   dfree(c->st);
   delete c;
@@ -165,43 +150,68 @@ void solver_cache_free(lfgpu_csr *A) {
 
 namespace {
 
-__global__ void k_slice_width(const int32_t *row_ptr, int32_t n, int32_t nslices, int32_t *w) {
+__global__ void k_slice_width(const int32_t *row_ptr, const int32_t *col, int32_t n,
+                              int32_t nslices, int32_t *w) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslices) return;
   int32_t m = 0;
   const int32_t r0 = s * kSlice, r1 = min(r0 + kSlice, n);
-  for (int32_t r = r0; r < r1; ++r) m = max(m, row_ptr[r + 1] - row_ptr[r]);
+  for (int32_t r = r0; r < r1; ++r) {
+    int32_t k = 0;
+    for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) k += (col[p] != r);
+    m = max(m, k);
+  }
   w[s] = m * kSlice;
 }
 
+// Off-diagonal columns in CSR order; padding points at the row itself.
 __global__ void k_fill_ell_cols(const int32_t *row_ptr, const int32_t *col, int32_t n,
                                 const int32_t *slice_off, int32_t *s_col) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int32_t s = r / kSlice, l = r % kSlice;
   const int32_t off = slice_off[s], w = (slice_off[s + 1] - off) / kSlice;
-  const int32_t b = row_ptr[r], len = row_ptr[r + 1] - b;
-  for (int32_t k = 0; k < w; ++k) s_col[off + k * kSlice + l] = k < len ? col[b + k] : r;
+  int32_t k = 0;
+  for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
+    if (col[p] != r) s_col[off + (k++) * kSlice + l] = col[p];
+  for (; k < w; ++k) s_col[off + k * kSlice + l] = r;
 }
 
-// Values of the ELL mirror and the Jacobi inverse diagonal
-// (Eigen DiagonalPreconditioner: 1/a_ii, or 1 when a_ii is absent or 0).
+// Off-diagonal values, the row sums s_i = sum_j a_ij (Neumaier-compensated,
+// so that s_i is accurate to a few ulp of itself even when the row nearly
+// cancels) and the Jacobi inverse diagonal (Eigen DiagonalPreconditioner:
+// 1/a_ii, or 1 when a_ii is absent or 0).
 __global__ void k_fill_ell_vals(const int32_t *row_ptr, const int32_t *col, const double *val,
                                 int32_t n, const int32_t *slice_off, double *s_val,
-                                float *s_val32, double *dinv, float *dinv32) {
+                                float *s_val32, double *rowsum, float *rowsum32, double *dinv,
+                                float *dinv32) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int32_t s = r / kSlice, l = r % kSlice;
   const int32_t off = slice_off[s], w = (slice_off[s + 1] - off) / kSlice;
-  const int32_t b = row_ptr[r], len = row_ptr[r + 1] - b;
-  double d = 1.0;
-  for (int32_t k = 0; k < w; ++k) {
-    const double v = k < len ? val[b + k] : 0.0;
-    if (k < len && col[b + k] == r && v != 0.0) d = 1.0 / v;
-    s_val[off + k * kSlice + l] = v;
-    if (s_val32) s_val32[off + k * kSlice + l] = (float)v;
+  double d = 1.0, sum = 0.0, comp = 0.0;
+  int32_t k = 0;
+  for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
+    const double v = val[p];
+    const double t = sum + v;
+    comp += fabs(sum) >= fabs(v) ? (sum - t) + v : (v - t) + sum;
+    sum = t;
+    if (col[p] == r) {
+      if (v != 0.0) d = 1.0 / v;
+    } else {
+      s_val[off + k * kSlice + l] = v;
+      if (s_val32) s_val32[off + k * kSlice + l] = (float)v;
+      ++k;
+    }
   }
+  for (; k < w; ++k) {
+    s_val[off + k * kSlice + l] = 0.0;
+    if (s_val32) s_val32[off + k * kSlice + l] = 0.0f;
+  }
+  const double rs = sum + comp;
+  rowsum[r] = rs;
   dinv[r] = d;
+  if (rowsum32) rowsum32[r] = (float)rs;
   if (dinv32) dinv32[r] = (float)d;
 }
 
@@ -217,15 +227,16 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32) {
     LFG_TRY(dalloc(&w, (size_t)c->nslices, "slice widths"));
     LFG_TRY(dalloc(&c->slice_off, (size_t)c->nslices + 1, "slice offsets"));
     hipLaunchKernelGGL(k_slice_width, dim3(grid_for(c->nslices, 256)), dim3(256), 0, s,
-                       A->row_ptr, n, c->nslices, w);
+                       A->row_ptr, A->col, n, c->nslices, w);
     LFG_HIP(hipGetLastError());
     int64_t len = 0;
     int rc = exclusive_scan_i32(s, w, c->slice_off, c->nslices, &len);
     dfree(w);
     LFG_TRY(rc);
     c->ell_len = len;
-    LFG_TRY(dalloc(&c->s_col, (size_t)len, "ELL columns"));
-    LFG_TRY(dalloc(&c->s_val, (size_t)len, "ELL values"));
+    LFG_TRY(dalloc(&c->s_col, (size_t)std::max<int64_t>(len, 1), "ELL columns"));
+    LFG_TRY(dalloc(&c->s_val, (size_t)std::max<int64_t>(len, 1), "ELL values"));
+    LFG_TRY(dalloc(&c->rowsum, (size_t)n, "row sums"));
     LFG_TRY(dalloc(&c->dinv, (size_t)n, "inverse diagonal"));
     hipLaunchKernelGGL(k_fill_ell_cols, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr,
                        A->col, n, c->slice_off, c->s_col);
@@ -233,25 +244,27 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32) {
     LFG_TRY(dalloc(&c->r, (size_t)n, "PCG r"));
     LFG_TRY(dalloc(&c->p, (size_t)n, "PCG p"));
     LFG_TRY(dalloc(&c->q, (size_t)n, "PCG q"));
-    // one slot per workgroup for up to four simultaneous partial sums
-    c->part_len = 4 * (int)grid_for(n, 256) + 64;
+    c->part_len = kPartSlots * kMaxGrid;
     LFG_TRY(dalloc(&c->part, (size_t)c->part_len, "reduction partials"));
     LFG_TRY(dalloc(&c->st, 1, "PCG state"));
   }
   if (need32 && !c->have32) {
-    LFG_TRY(dalloc(&c->s_val32, (size_t)c->ell_len, "ELL values FP32"));
+    LFG_TRY(dalloc(&c->s_val32, (size_t)std::max<int64_t>(c->ell_len, 1), "ELL values FP32"));
+    LFG_TRY(dalloc(&c->rowsum32, (size_t)n, "row sums FP32"));
     LFG_TRY(dalloc(&c->dinv32, (size_t)n, "inverse diagonal FP32"));
     LFG_TRY(dalloc(&c->r32, (size_t)n, "PCG r32"));
     LFG_TRY(dalloc(&c->p32, (size_t)n, "PCG p32"));
     LFG_TRY(dalloc(&c->q32, (size_t)n, "PCG q32"));
     LFG_TRY(dalloc(&c->xa32, (size_t)n, "PCG x32"));
+    LFG_TRY(dalloc(&c->xlo, (size_t)n, "PCG x low part"));
     c->have32 = true;
     c->built_version = ~0ull;
   }
   if (c->built_version != A->val_version) {
     hipLaunchKernelGGL(k_fill_ell_vals, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr,
                        A->col, A->val, n, c->slice_off, c->s_val, c->have32 ? c->s_val32 : nullptr,
-                       c->dinv, c->have32 ? c->dinv32 : nullptr);
+                       c->rowsum, c->have32 ? c->rowsum32 : nullptr, c->dinv,
+                       c->have32 ? c->dinv32 : nullptr);
     LFG_HIP(hipGetLastError());
     c->built_version = A->val_version;
   }
@@ -259,166 +272,145 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32) {
 }
 
 // ---------------------------------------------------------------------------
-// SpMV kernels on the sliced ELL mirror
+// SpMV on the sliced ELL mirror, in difference form:
+//   (A x)_i = sum_{j != i} a_ij (x_j - x_i) + s_i x_i,   s_i = sum_j a_ij.
+// Algebraically identical to sum_j a_ij x_j; for stiffness matrices, whose
+// rows nearly cancel (s_i = O(h^2) against entries O(1)), it avoids the
+// catastrophic cancellation of the plain form on smooth x, which would
+// otherwise put the attainable residual of a 2048^2 FP64 solve near 1e-10
+// and make an FP32 product useless for the smooth modes.
 // ---------------------------------------------------------------------------
-template <class V, class X>
+template <class V, class X, class S>
 __device__ __forceinline__ X ell_row(const int32_t *slice_off, const int32_t *s_col, const V *s_val,
-                                     const X *x, int32_t r) {
+                                     const S *rowsum, const X *x, int32_t r, X xr) {
   const int32_t s = r >> 6, l = r & 63;
   const int32_t off = slice_off[s];
   const int32_t w = (slice_off[s + 1] - off) >> 6;
   const V *vp = s_val + off + l;
   const int32_t *cp = s_col + off + l;
   X acc = 0;
-  for (int32_t k = 0; k < w; ++k) acc += (X)vp[64 * k] * x[cp[64 * k]];
-  return acc;
+  for (int32_t k = 0; k < w; ++k) acc += (X)vp[64 * k] * (x[cp[64 * k]] - xr);
+  return acc + (X)rowsum[r] * xr;
+}
+
+// FP32 storage, FP64 arithmetic: products of two FP32 numbers are exact in
+// FP64 and the row is summed in FP64, so the only FP32 rounding left in q is
+// the final conversion.  Summing in FP32 instead leaves a relative error of
+// ~eps32 / h on smooth vectors (7e-4 at h = 1/2048), enough to destroy the
+// conjugacy of the CG recurrences.
+__device__ __forceinline__ float ell_row32_acc64(const int32_t *slice_off, const int32_t *s_col,
+                                                 const float *s_val, const float *rowsum,
+                                                 const float *x, int32_t r, float xr) {
+  const int32_t s = r >> 6, l = r & 63;
+  const int32_t off = slice_off[s];
+  const int32_t w = (slice_off[s + 1] - off) >> 6;
+  const float *vp = s_val + off + l;
+  const int32_t *cp = s_col + off + l;
+  const double xd = (double)xr;
+  double acc = 0.0;
+  for (int32_t k = 0; k < w; ++k) acc += (double)vp[64 * k] * ((double)x[cp[64 * k]] - xd);
+  return (float)(acc + (double)rowsum[r] * xd);
 }
 
 __global__ __launch_bounds__(256) void k_spmv64(const int32_t *slice_off, const int32_t *s_col,
-                                                const double *s_val, const double *x, double *y,
-                                                int32_t n) {
+                                                const double *s_val, const double *rowsum,
+                                                const double *x, double *y, int32_t n) {
   const int32_t r = blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
-  y[r] = ell_row<double, double>(slice_off, s_col, s_val, x, r);
+  y[r] = ell_row<double, double, double>(slice_off, s_col, s_val, rowsum, x, r, x[r]);
 }
 
 // ---------------------------------------------------------------------------
-// FP64 PCG kernels (Eigen ConjugateGradient iteration, one sweep each)
+// PCG sweeps.  A fixed grid of G workgroups; workgroup g owns the contiguous
+// rows [g R, (g+1) R) (R a multiple of 256, so every wavefront covers one
+// 64-row ELL slice) and writes one partial sum per reduced quantity.
 // ---------------------------------------------------------------------------
+struct Sweep {
+  int32_t n;
+  int32_t rows_per_group;
+  int ngroups;
+};
+
+// Sum of ngroups partials, identical in every workgroup, broadcast to all
+// threads of the calling workgroup.
+__device__ __forceinline__ double all_reduce_partials(const double *part, int ngroups,
+                                                      double *sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < ngroups; i += 256) v += part[i];
+  v = block_sum<256>(v, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) sh[4] = v;
+  __syncthreads();
+  return sh[4];
+}
+
+__device__ __forceinline__ void store_partial(double *part, double v, double *sh) {
+  v = block_sum<256>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
 struct PcgArgs {
   const int32_t *slice_off;
   const int32_t *s_col;
   const double *s_val;
+  const double *rowsum;
   const double *dinv;
   double *x, *r, *p, *q;
-  double *part;
+  double *part;  // slots: 0 pq, 1 rr, 2 rz, 3 bb
   PcgState *st;
-  int32_t n;
+  Sweep sw;
 };
 
-// q = A p ; alpha = rho / (p . q)
-__global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a) {
-  __shared__ double sh[4];
-  __shared__ int flag;
-  if (a.st->done) return;
-  const int32_t r = blockIdx.x * 256 + threadIdx.x;
-  double v = 0.0;
-  if (r < a.n) {
-    const double qr = ell_row<double, double>(a.slice_off, a.s_col, a.s_val, a.p, r);
-    a.q[r] = qr;
-    v = a.p[r] * qr;
-  }
-  v = block_sum<256>(v, sh);
-  if (publish_partial(a.part, blockIdx.x, v, &a.st->counter[0], &flag)) {
-    const double pq = reduce_partials<256>(a.part, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      a.st->pq = pq;
-      a.st->alpha = a.st->rho / pq;
-      a.st->counter[0] = 0;
-    }
-  }
-}
-
-// x += alpha p ; r -= alpha q ; rr = r.r ; rho' = r.(D^-1 r) ; beta
-__global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a) {
-  __shared__ double sh[4];
-  __shared__ int flag;
-  if (a.st->done) return;
-  const double alpha = a.st->alpha;
-  const int32_t i = blockIdx.x * 256 + threadIdx.x;
-  double rr = 0.0, rz = 0.0;
-  if (i < a.n) {
-    const double pi = a.p[i];
-    a.x[i] += alpha * pi;
-    const double ri = a.r[i] - alpha * a.q[i];
-    a.r[i] = ri;
-    rr = ri * ri;
-    rz = ri * (a.dinv[i] * ri);
-  }
-  rr = block_sum<256>(rr, sh);
-  rz = block_sum<256>(rz, sh);
-  const int nb = gridDim.x;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(a.part + nb + blockIdx.x, rz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (publish_partial(a.part, blockIdx.x, rr, &a.st->counter[1], &flag)) {
-    const double rrs = reduce_partials<256>(a.part, nb, sh);
-    const double rzs = reduce_partials<256>(a.part + nb, nb, sh);
-    if (threadIdx.x == 0) {
-      PcgState *st = a.st;
-      st->rr = rrs;
-      st->counter[1] = 0;
-      if (rrs < st->thr) {
-        st->done = 1;
-        st->converged = 1;
-      } else {
-        st->beta = rzs / st->rho;
-        st->rho = rzs;
-        st->iter += 1;
-        if (st->iter >= st->maxit) st->done = 1;
-      }
-    }
-  }
-}
-
-// p = D^-1 r + beta p
-__global__ __launch_bounds__(256) void k_pcg_pupdate(PcgArgs a) {
-  if (a.st->done) return;
-  const double beta = a.st->beta;
-  const int32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i < a.n) a.p[i] = a.dinv[i] * a.r[i] + beta * a.p[i];
-}
-
-// setup: r = b - A x, partials of b.b and r.r
+// setup: r = b - A x, p = D^-1 r; partials of b.b, r.r, r.D^-1 r
 __global__ __launch_bounds__(256) void k_pcg_init(PcgArgs a, const double *b) {
-  __shared__ double sh[4];
-  __shared__ int flag;
-  const int32_t i = blockIdx.x * 256 + threadIdx.x;
+  __shared__ double sh[8];
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double bb = 0.0, rr = 0.0, rz = 0.0;
-  if (i < a.n) {
-    const double ax = ell_row<double, double>(a.slice_off, a.s_col, a.s_val, a.x, i);
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const double xi = a.x[i];
+    const double ax =
+        ell_row<double, double, double>(a.slice_off, a.s_col, a.s_val, a.rowsum, a.x, i, xi);
     const double bi = b[i];
     const double ri = bi - ax;
     a.r[i] = ri;
     const double zi = a.dinv[i] * ri;
     a.p[i] = zi;
-    bb = bi * bi;
-    rr = ri * ri;
-    rz = ri * zi;
+    bb += bi * bi;
+    rr += ri * ri;
+    rz += ri * zi;
   }
-  bb = block_sum<256>(bb, sh);
-  rr = block_sum<256>(rr, sh);
-  rz = block_sum<256>(rz, sh);
-  const int nb = gridDim.x;
+  store_partial(a.part + 3 * kMaxGrid, bb, sh);
+  store_partial(a.part + 1 * kMaxGrid, rr, sh);
+  store_partial(a.part + 2 * kMaxGrid, rz, sh);
+}
+
+// one workgroup: finish the setup reductions (Eigen's early exits)
+__global__ __launch_bounds__(256) void k_pcg_init_finish(PcgArgs a) {
+  __shared__ double sh[8];
+  const double bb = all_reduce_partials(a.part + 3 * kMaxGrid, a.sw.ngroups, sh);
+  const double rr = all_reduce_partials(a.part + 1 * kMaxGrid, a.sw.ngroups, sh);
+  const double rz = all_reduce_partials(a.part + 2 * kMaxGrid, a.sw.ngroups, sh);
   if (threadIdx.x == 0) {
-    __hip_atomic_store(a.part + nb + blockIdx.x, rr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(a.part + 2 * nb + blockIdx.x, rz, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (publish_partial(a.part, blockIdx.x, bb, &a.st->counter[2], &flag)) {
-    const double bbs = reduce_partials<256>(a.part, nb, sh);
-    const double rrs = reduce_partials<256>(a.part + nb, nb, sh);
-    const double rzs = reduce_partials<256>(a.part + 2 * nb, nb, sh);
-    if (threadIdx.x == 0) {
-      PcgState *st = a.st;
-      st->counter[2] = 0;
-      st->bb = bbs;
-      st->rr = rrs;
-      st->rho = rzs;
-      st->iter = 0;
-      st->converged = 0;
-      st->done = 0;
-      const double thr = st->thr * bbs;  // thr holds tol^2 on entry
-      st->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
-      if (bbs == 0.0) {
-        st->done = 1;
-        st->converged = 1;
-      } else if (rrs < st->thr) {
-        st->done = 1;
-        st->converged = 1;
-      } else if (st->maxit <= 0) {
-        st->done = 1;
-      }
+    PcgState *st = a.st;
+    st->bb = bb;
+    st->rr = rr;
+    st->rr_ref = rr;
+    st->rho[0] = rz;
+    st->iter = 0;
+    st->converged = 0;
+    st->done = 0;
+    st->refinements = 0;
+    st->last_refine = 0;
+    st->refine[0] = st->refine[1] = 0;
+    st->iterp[0] = st->iterp[1] = 0;
+    const double thr = st->thr * bb;  // thr holds tol^2 on entry
+    st->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+    if (bb == 0.0 || rr < st->thr) {
+      st->done = 1;
+      st->converged = 1;
+    } else if (st->maxit <= 0) {
+      st->done = 1;
     }
   }
 }
@@ -428,176 +420,248 @@ __global__ void k_zero_if_bb0(double *x, int32_t n, const PcgState *st) {
   if (i < n && st->bb == 0.0) x[i] = 0.0;
 }
 
+// q = A p ; partial p.q
+__global__ __launch_bounds__(256) void k_pcg_spmv(PcgArgs a) {
+  __shared__ double sh[8];
+  if (a.st->done) return;
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  double v = 0.0;
+  for (int32_t r = r0 + threadIdx.x; r < r1; r += 256) {
+    const double pr = a.p[r];
+    const double qr =
+        ell_row<double, double, double>(a.slice_off, a.s_col, a.s_val, a.rowsum, a.p, r, pr);
+    a.q[r] = qr;
+    v += pr * qr;
+  }
+  store_partial(a.part, v, sh);
+}
+
+// alpha = rho / (p.q) ; x += alpha p ; r -= alpha q ; partials r.r, r.D^-1 r
+__global__ __launch_bounds__(256) void k_pcg_update(PcgArgs a, int par) {
+  __shared__ double sh[8];
+  if (a.st->done) return;
+  const double alpha = a.st->rho[par] / all_reduce_partials(a.part, a.sw.ngroups, sh);
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  double rr = 0.0, rz = 0.0;
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    a.x[i] += alpha * a.p[i];
+    const double ri = a.r[i] - alpha * a.q[i];
+    a.r[i] = ri;
+    rr += ri * ri;
+    rz += ri * (a.dinv[i] * ri);
+  }
+  store_partial(a.part + 1 * kMaxGrid, rr, sh);
+  store_partial(a.part + 2 * kMaxGrid, rz, sh);
+}
+
+// Eigen's convergence test on the recurrence residual, then
+// beta = rho' / rho ; p = D^-1 r + beta p
+__global__ __launch_bounds__(256) void k_pcg_pupdate(PcgArgs a, int par) {
+  __shared__ double sh[8];
+  PcgState *st = a.st;
+  if (st->done) return;
+  const double rr = all_reduce_partials(a.part + 1 * kMaxGrid, a.sw.ngroups, sh);
+  const double rz = all_reduce_partials(a.part + 2 * kMaxGrid, a.sw.ngroups, sh);
+  if (rr < st->thr) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rr;
+      st->done = 1;
+      st->converged = 1;
+    }
+    return;
+  }
+  const double beta = rz / st->rho[par];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rr = rr;
+    st->rho[par ^ 1] = rz;
+    st->iter += 1;
+    if (st->iter >= st->maxit) st->done = 1;
+  }
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) a.p[i] = a.dinv[i] * a.r[i] + beta * a.p[i];
+}
+
 // ---------------------------------------------------------------------------
-// mixed precision: FP32 recurrences with FP64 reliable updates
+// mixed precision: FP32 matrix, vectors and recurrences (FP64 dot products),
+// solution kept as a double-double pair (x, xlo) so that its true residual is
+// not limited by rounding x to FP64; reliable updates recompute the true
+// residual b - A x - A xlo in FP64 whenever the recurrence residual has
+// dropped by delta (or after a caller-set number of iterations).
 // ---------------------------------------------------------------------------
 struct MixArgs {
   const int32_t *slice_off;
   const int32_t *s_col;
   const float *s_val32;
   const double *s_val;
+  const float *rowsum32;
+  const double *rowsum;
   const float *dinv32;
   const double *dinv;
-  double *x;          // FP64 solution
+  double *x;          // high part of the solution (the caller's vector)
+  double *xlo;        // low part
   const double *b;
-  double *r64;        // FP64 true residual scratch
+  double *r64;        // FP64 true residual
   float *r, *p, *q, *xa;
-  double *part;
+  double *part;       // slots: 0 pq, 1 rr, 2 rz, 4 rr64, 5 rz64
   PcgState *st;
-  int32_t n;
-  double delta2;      // reliable-update trigger: rr < delta2 * rr_ref
+  Sweep sw;
+  double delta2;      // reliable update when rr < delta2 * rr_ref
+  int max_interval;   // ... or after this many iterations
 };
 
-__global__ __launch_bounds__(256) void k_mix_spmv(MixArgs a) {
-  __shared__ double sh[4];
-  __shared__ int flag;
-  if (a.st->done) return;
-  const int32_t r = blockIdx.x * 256 + threadIdx.x;
-  double v = 0.0;
-  if (r < a.n) {
-    const float qr = ell_row<float, float>(a.slice_off, a.s_col, a.s_val32, a.p, r);
-    a.q[r] = qr;
-    v = (double)a.p[r] * (double)qr;
-  }
-  v = block_sum<256>(v, sh);
-  if (publish_partial(a.part, blockIdx.x, v, &a.st->counter[0], &flag)) {
-    const double pq = reduce_partials<256>(a.part, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      a.st->pq = pq;
-      a.st->alpha = a.st->rho / pq;
-      a.st->counter[0] = 0;
-    }
-  }
-}
-
-// xa += alpha p ; r -= alpha q ; rr, rz ; decide on a reliable update
-__global__ __launch_bounds__(256) void k_mix_update(MixArgs a) {
-  __shared__ double sh[4];
-  __shared__ int flag;
-  if (a.st->done) return;
-  const float alpha = (float)a.st->alpha;
-  const int32_t i = blockIdx.x * 256 + threadIdx.x;
-  double rr = 0.0, rz = 0.0;
-  if (i < a.n) {
-    a.xa[i] += alpha * a.p[i];
-    const float ri = a.r[i] - alpha * a.q[i];
-    a.r[i] = ri;
-    rr = (double)ri * (double)ri;
-    rz = (double)ri * (double)(a.dinv32[i] * ri);
-  }
-  rr = block_sum<256>(rr, sh);
-  rz = block_sum<256>(rz, sh);
-  const int nb = gridDim.x;
-  if (threadIdx.x == 0)
-    __hip_atomic_store(a.part + nb + blockIdx.x, rz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (publish_partial(a.part, blockIdx.x, rr, &a.st->counter[1], &flag)) {
-    const double rrs = reduce_partials<256>(a.part, nb, sh);
-    const double rzs = reduce_partials<256>(a.part + nb, nb, sh);
-    if (threadIdx.x == 0) {
-      PcgState *st = a.st;
-      st->counter[1] = 0;
-      st->rr = rrs;
-      st->iter += 1;
-      st->rho_prev = st->rho;
-      st->beta = rzs / st->rho;
-      st->rho = rzs;
-      st->refine_req = (rrs < st->thr || rrs < a.delta2 * st->rr_ref) ? 1 : 0;
-      if (st->iter >= st->maxit) st->refine_req = 1;
-    }
-  }
-}
-
-// reliable update, part 1: x += xa (FP64), xa = 0, r64 = b - A x (FP64),
-// partial sums of ||r64||^2 and (r64, D^-1 r64)
-__global__ __launch_bounds__(256) void k_mix_refine(MixArgs a) {
-  __shared__ double sh[4];
-  __shared__ int flag;
-  if (a.st->done || !a.st->refine_req) return;
-  // The FP64 matrix product needs the updated x of neighbouring rows, so the
-  // solution fold-in happens in k_mix_fold beforehand.
-  const int32_t i = blockIdx.x * 256 + threadIdx.x;
-  double rr = 0.0, rz = 0.0;
-  if (i < a.n) {
-    const double ax = ell_row<double, double>(a.slice_off, a.s_col, a.s_val, a.x, i);
-    const double ri = a.b[i] - ax;
-    a.r64[i] = ri;
-    rr = ri * ri;
-    rz = ri * (a.dinv[i] * ri);
-  }
-  rr = block_sum<256>(rr, sh);
-  rz = block_sum<256>(rz, sh);
-  const int nb = gridDim.x;
-  if (threadIdx.x == 0)
-    __hip_atomic_store(a.part + nb + blockIdx.x, rz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (publish_partial(a.part, blockIdx.x, rr, &a.st->counter[3], &flag)) {
-    const double rrs = reduce_partials<256>(a.part, nb, sh);
-    const double rzs = reduce_partials<256>(a.part + nb, nb, sh);
-    if (threadIdx.x == 0) {
-      PcgState *st = a.st;
-      st->counter[3] = 0;
-      st->rr_true = rrs;
-      st->refinements += 1;
-      if (rrs < st->thr) {
-        st->done = 1;
-        st->converged = 1;
-      } else if (st->iter >= st->maxit) {
-        st->done = 1;
-      } else {
-        // restart the recurrence residual from the true one; keep the search
-        // direction and rescale beta with the new (r, z)
-        st->rr_ref = rrs;
-        st->rr = rrs;
-        st->beta = rzs / st->rho_prev;
-        st->rho = rzs;
-      }
-    }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_mix_fold(MixArgs a) {
-  if (a.st->done || !a.st->refine_req) return;
-  const int32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i < a.n) {
-    a.x[i] += (double)a.xa[i];
-    a.xa[i] = 0.0f;
-  }
-}
-
-// p = D^-1 r + beta p in FP32; after a reliable update r comes from r64.
-__global__ __launch_bounds__(256) void k_mix_pupdate(MixArgs a) {
-  const PcgState *st = a.st;
-  if (st->done) return;
-  const int32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.n) return;
-  float ri = a.r[i];
-  if (st->refine_req) {
-    ri = (float)a.r64[i];
-    a.r[i] = ri;
-  }
-  a.p[i] = a.dinv32[i] * ri + (float)st->beta * a.p[i];
-}
-
-__global__ void k_mix_clear_req(PcgState *st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st->refine_req = 0;
-}
-
-// mixed setup after the FP64 init: r32 = r64, p32 = D^-1 r32, xa = 0
-__global__ void k_mix_setup(MixArgs a) {
+__global__ __launch_bounds__(256) void k_mix_setup(MixArgs a) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  if (i >= a.sw.n) return;
   const float ri = (float)a.r64[i];
   a.r[i] = ri;
   a.p[i] = a.dinv32[i] * ri;
   a.xa[i] = 0.0f;
+  a.xlo[i] = 0.0;
 }
 
-__global__ void k_mix_rho(PcgState *st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->rr_ref = st->rr;
-    st->rr_true = st->rr;
-    st->refinements = 0;
-    st->refine_req = 0;
+__global__ __launch_bounds__(256) void k_mix_spmv(MixArgs a) {
+  __shared__ double sh[8];
+  if (a.st->done) return;
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  double v = 0.0;
+  for (int32_t r = r0 + threadIdx.x; r < r1; r += 256) {
+    const float pr = a.p[r];
+    const float qr = ell_row32_acc64(a.slice_off, a.s_col, a.s_val32, a.rowsum32, a.p, r, pr);
+    a.q[r] = qr;
+    v += (double)pr * (double)qr;
   }
+  store_partial(a.part, v, sh);
+}
+
+__global__ __launch_bounds__(256) void k_mix_update(MixArgs a, int par) {
+  __shared__ double sh[8];
+  if (a.st->done) return;
+  const float alpha = (float)(a.st->rho[par] / all_reduce_partials(a.part, a.sw.ngroups, sh));
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  double rr = 0.0, rz = 0.0;
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    a.xa[i] += alpha * a.p[i];
+    const float ri = a.r[i] - alpha * a.q[i];
+    a.r[i] = ri;
+    rr += (double)ri * (double)ri;
+    rz += (double)ri * (double)(a.dinv32[i] * ri);
+  }
+  store_partial(a.part + 1 * kMaxGrid, rr, sh);
+  store_partial(a.part + 2 * kMaxGrid, rz, sh);
+}
+
+// Decide on a reliable update.  Without one: beta, p update.  With one: fold
+// the FP32 increment into the double-double solution (TwoSum).
+__global__ __launch_bounds__(256) void k_mix_decide(MixArgs a, int par) {
+  __shared__ double sh[8];
+  PcgState *st = a.st;
+  if (st->done) return;
+  const double rr = all_reduce_partials(a.part + 1 * kMaxGrid, a.sw.ngroups, sh);
+  const double rz = all_reduce_partials(a.part + 2 * kMaxGrid, a.sw.ngroups, sh);
+  const int it = st->iterp[par] + 1;
+  const bool refine = rr < st->thr || rr < a.delta2 * st->rr_ref ||
+                      it - st->last_refine >= a.max_interval || it >= st->maxit;
+  const double beta = rz / st->rho[par];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iterp[par ^ 1] = it;
+    st->iter = it;
+    st->refine[par] = refine ? 1 : 0;
+    if (!refine) st->rho[par ^ 1] = rz;
+  }
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  if (!refine) {
+    const float fb = (float)beta;
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256)
+      a.p[i] = a.dinv32[i] * a.r[i] + fb * a.p[i];
+  } else {
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+      const double hi = a.x[i], d = (double)a.xa[i];
+      const double s = hi + d;
+      const double bp = s - hi;
+      const double err = (hi - (s - bp)) + (d - bp);
+      a.x[i] = s;
+      a.xlo[i] += err;
+      a.xa[i] = 0.0f;
+    }
+  }
+}
+
+// reliable update, true residual r64 = b - A x - A xlo (FP64, difference form)
+__global__ __launch_bounds__(256) void k_mix_resid(MixArgs a, int par) {
+  __shared__ double sh[8];
+  if (a.st->done || !a.st->refine[par]) return;
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  double rr = 0.0, rz = 0.0;
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const double ah = ell_row<double, double, double>(a.slice_off, a.s_col, a.s_val, a.rowsum,
+                                                      a.x, i, a.x[i]);
+    const double al = ell_row<double, double, double>(a.slice_off, a.s_col, a.s_val, a.rowsum,
+                                                      a.xlo, i, a.xlo[i]);
+    const double ri = (a.b[i] - ah) - al;
+    a.r64[i] = ri;
+    rr += ri * ri;
+    rz += ri * (a.dinv[i] * ri);
+  }
+  store_partial(a.part + 4 * kMaxGrid, rr, sh);
+  store_partial(a.part + 5 * kMaxGrid, rz, sh);
+}
+
+// reliable update, finish: convergence on the true residual, or restart the
+// FP32 recurrence from it (beta rescaled with the new (r, D^-1 r)).
+__global__ __launch_bounds__(256) void k_mix_finish(MixArgs a, int par) {
+  __shared__ double sh[8];
+  PcgState *st = a.st;
+  if (st->done || !st->refine[par]) return;
+  const double rt = all_reduce_partials(a.part + 4 * kMaxGrid, a.sw.ngroups, sh);
+  const double rz = all_reduce_partials(a.part + 5 * kMaxGrid, a.sw.ngroups, sh);
+  const bool conv = rt < st->thr;
+  const bool stop = conv || st->iterp[par ^ 1] >= st->maxit;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rr = rt;
+    st->refinements += 1;
+    st->last_refine = st->iterp[par ^ 1];
+    if (stop) {
+      st->done = 1;
+      st->converged = conv ? 1 : 0;
+    } else {
+      st->rr_ref = rt;
+      st->rho[par ^ 1] = rz;
+    }
+  }
+  if (stop) return;
+  const float fb = (float)(rz / st->rho[par]);
+  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const float ri = (float)a.r64[i];
+    a.r[i] = ri;
+    a.p[i] = a.dinv32[i] * ri + fb * a.p[i];
+  }
+}
+
+// x = x + xlo (the solution handed back in FP64)
+__global__ __launch_bounds__(256) void k_mix_final(MixArgs a) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.sw.n) a.x[i] += a.xlo[i];
+}
+
+Sweep make_sweep(int32_t n) {
+  int64_t per = (n + kMaxGrid - 1) / kMaxGrid;
+  per = ((per + 255) / 256) * 256;
+  if (per < 256) per = 256;
+  Sweep s;
+  s.n = n;
+  s.rows_per_group = (int32_t)per;
+  s.ngroups = (int)((n + per - 1) / per);
+  if (s.ngroups < 1) s.ngroups = 1;
+  return s;
 }
 
 }  // namespace
@@ -773,7 +837,7 @@ extern "C" int lfgpu_spmv(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *
   lfgpu_solver_cache *c = A->cache;
   LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   hipLaunchKernelGGL(k_spmv64, dim3(grid_for(A->nrows, 256)), dim3(256), 0, ctx->stream,
-                     c->slice_off, c->s_col, c->s_val, x->d, y->d, A->nrows);
+                     c->slice_off, c->s_col, c->s_val, c->rowsum, x->d, y->d, A->nrows);
   LFG_HIP(hipGetLastError());
   LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
@@ -796,7 +860,7 @@ extern "C" int lfgpu_spmv_timed(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu
   LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   for (int k = 0; k < reps; ++k)
     hipLaunchKernelGGL(k_spmv64, dim3(grid_for(A->nrows, 256)), dim3(256), 0, ctx->stream,
-                       c->slice_off, c->s_col, c->s_val, x->d, y->d, A->nrows);
+                       c->slice_off, c->s_col, c->s_val, c->rowsum, x->d, y->d, A->nrows);
   LFG_HIP(hipGetLastError());
   LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
@@ -813,6 +877,7 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   lfgpu_csr *A = const_cast<lfgpu_csr *>(Ac);
   LFG_REQUIRE(A->nrows == A->ncols && b->n == A->nrows && x->n == A->nrows, LFGPU_EINVAL,
               "PCG size mismatch");
+  LFG_REQUIRE(b->d != x->d, LFGPU_EINVAL, "PCG right-hand side and solution must differ");
   LFG_REQUIRE(opts->rtol >= 0.0 && opts->maxit >= 0, LFGPU_EINVAL, "bad PCG options");
   LFG_REQUIRE(opts->mode == LFGPU_PCG_FP64 || opts->mode == LFGPU_PCG_MIXED, LFGPU_EINVAL,
               "unknown PCG mode");
@@ -821,10 +886,13 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   lfgpu_solver_cache *c = A->cache;
   hipStream_t s = ctx->stream;
   const int32_t n = A->nrows;
-  const unsigned grid = grid_for(n, 256);
-  LFG_REQUIRE((int64_t)grid * 3 <= c->part_len, LFGPU_ERANGE,
-              "reduction buffer smaller than three partials per workgroup");
-  const int check = opts->check_every > 0 ? opts->check_every : 64;
+  const Sweep sw = make_sweep(n);
+  const dim3 G(sw.ngroups), B(256);
+  const unsigned gflat = grid_for(n, 256);
+  // an even number of iterations per check keeps the parity of the launch
+  // index aligned with the iteration count
+  int check = opts->check_every > 0 ? opts->check_every : 64;
+  check += check & 1;
 
   PcgState h{};
   h.thr = opts->rtol * opts->rtol;
@@ -832,53 +900,53 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   LFG_HIP(hipMemcpyAsync(c->st, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
   LFG_HIP(hipEventRecord(ctx->ev0, s));
 
-  PcgArgs a{c->slice_off, c->s_col, c->s_val, c->dinv, x->d, c->r, c->p, c->q, c->part, c->st, n};
-  hipLaunchKernelGGL(k_pcg_init, dim3(grid), dim3(256), 0, s, a, b->d);
-  LFG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_zero_if_bb0, dim3(grid), dim3(256), 0, s, x->d, n, c->st);
+  PcgArgs a{c->slice_off, c->s_col, c->s_val, c->rowsum, c->dinv, x->d, c->r, c->p, c->q,
+            c->part, c->st, sw};
+  hipLaunchKernelGGL(k_pcg_init, G, B, 0, s, a, b->d);
+  hipLaunchKernelGGL(k_pcg_init_finish, dim3(1), B, 0, s, a);
+  hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), B, 0, s, x->d, n, c->st);
   LFG_HIP(hipGetLastError());
 
   PcgState *hs = reinterpret_cast<PcgState *>(ctx->pinned);
-  if (!mixed) {
-    for (;;) {
-      for (int k = 0; k < check; ++k) {
-        hipLaunchKernelGGL(k_pcg_spmv, dim3(grid), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_pcg_update, dim3(grid), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_pcg_pupdate, dim3(grid), dim3(256), 0, s, a);
-      }
-      LFG_HIP(hipGetLastError());
-      LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-      LFG_HIP(hipStreamSynchronize(s));
-      if (hs->done) break;
-    }
-  } else {
-    MixArgs m{c->slice_off, c->s_col, c->s_val32, c->s_val, c->dinv32, c->dinv, x->d, b->d,
-              c->r, c->r32, c->p32, c->q32, c->xa32, c->part, c->st, n, 1e-2};
-    hipLaunchKernelGGL(k_mix_setup, dim3(grid), dim3(256), 0, s, m);
-    hipLaunchKernelGGL(k_mix_rho, dim3(1), dim3(64), 0, s, c->st);
+  MixArgs m{c->slice_off, c->s_col, c->s_val32, c->s_val, c->rowsum32, c->rowsum, c->dinv32,
+            c->dinv, x->d, c->xlo, b->d, c->r, c->r32, c->p32, c->q32, c->xa32, c->part, c->st,
+            sw, 0.09, 2147483647};
+  if (opts->refine_delta > 0.0) m.delta2 = opts->refine_delta * opts->refine_delta;
+  if (opts->refine_max_interval > 0) m.max_interval = opts->refine_max_interval;
+  if (mixed) {
+    hipLaunchKernelGGL(k_mix_setup, dim3(gflat), B, 0, s, m);
     LFG_HIP(hipGetLastError());
-    for (;;) {
-      for (int k = 0; k < check; ++k) {
-        hipLaunchKernelGGL(k_mix_spmv, dim3(grid), dim3(256), 0, s, m);
-        hipLaunchKernelGGL(k_mix_update, dim3(grid), dim3(256), 0, s, m);
-        hipLaunchKernelGGL(k_mix_fold, dim3(grid), dim3(256), 0, s, m);
-        hipLaunchKernelGGL(k_mix_refine, dim3(grid), dim3(256), 0, s, m);
-        hipLaunchKernelGGL(k_mix_pupdate, dim3(grid), dim3(256), 0, s, m);
-        hipLaunchKernelGGL(k_mix_clear_req, dim3(1), dim3(64), 0, s, c->st);
+  }
+  for (;;) {
+    for (int k = 0; k < check; ++k) {
+      const int par = k & 1;
+      if (!mixed) {
+        hipLaunchKernelGGL(k_pcg_spmv, G, B, 0, s, a);
+        hipLaunchKernelGGL(k_pcg_update, G, B, 0, s, a, par);
+        hipLaunchKernelGGL(k_pcg_pupdate, G, B, 0, s, a, par);
+      } else {
+        hipLaunchKernelGGL(k_mix_spmv, G, B, 0, s, m);
+        hipLaunchKernelGGL(k_mix_update, G, B, 0, s, m, par);
+        hipLaunchKernelGGL(k_mix_decide, G, B, 0, s, m, par);
+        hipLaunchKernelGGL(k_mix_resid, G, B, 0, s, m, par);
+        hipLaunchKernelGGL(k_mix_finish, G, B, 0, s, m, par);
       }
-      LFG_HIP(hipGetLastError());
-      LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-      LFG_HIP(hipStreamSynchronize(s));
-      if (hs->done) break;
     }
+    LFG_HIP(hipGetLastError());
+    LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    LFG_HIP(hipStreamSynchronize(s));
+    if (hs->done) break;
+  }
+  if (mixed) {
+    hipLaunchKernelGGL(k_mix_final, dim3(gflat), B, 0, s, m);
+    LFG_HIP(hipGetLastError());
   }
   LFG_HIP(hipEventRecord(ctx->ev1, s));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0.f;
   LFG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   ctx->last_ms = ms;
-  const double relres =
-      hs->bb > 0.0 ? std::sqrt((mixed ? hs->rr_true : hs->rr) / hs->bb) : 0.0;
+  const double relres = hs->bb > 0.0 ? std::sqrt(hs->rr / hs->bb) : 0.0;
   if (info) {
     info->iters = hs->iter;
     info->converged = hs->converged;
