@@ -1,19 +1,25 @@
-"""Benchmark of the MI355X-native LehrFEM++ hot path.
+"""Benchmark of the MI355X-native LehrFEM++ hot path (one GPU).
 
-Workload (one GPU): reaction-diffusion -div(a grad u) + u = f on the unit
-square, 2048 x 2048 structured triangular mesh (8,388,608 cells, 4,198,401
-DOFs), P1 Lagrange, FP64 geometry and assembly, a = exp(g) per cell with
-g ~ N(0, 1) (numpy default_rng(seed)), f = sin(pi x) sin(pi y), zero
-Dirichlet data, Jacobi-PCG to ||r|| / ||b|| <= 1e-10.
+Default workload (--config p1): reaction-diffusion -div(a grad u) + u = f on
+the unit square, 2048 x 2048 structured triangular mesh (8,388,608 cells,
+4,198,401 DOFs), P1 Lagrange, FP64 geometry and assembly, a = exp(g) per cell
+with g ~ N(0, 1) (numpy default_rng(seed)), f = sin(pi x) sin(pi y), zero
+Dirichlet data, Jacobi-PCG to ||r|| / ||b|| <= 1e-10 (Eigen ConjugateGradient
+semantics).  One step = assemble the Galerkin matrix and load vector into
+the cached pattern + solve from x = 0.  Mesh, coefficient field and assembly
+plan are device resident before the timed region.  value = cells assembled
+and solved per second over the timed steps (Melem/s).
 
-One step = assemble the Galerkin matrix and load vector into the cached
-pattern (AssembleMatrixLocally + AssembleVectorLocally +
-FixFlaggedSolutionComponents + makeSparse) + solve with PCG from x = 0.
-The mesh, coefficient field and assembly plan are device resident before the
-timed region starts.  value = cells assembled-and-solved per second over the
-timed steps (Melem/s).
+Other configurations (each prints its own JSON line):
+  --config p3       P3 on a 512 x 512 mesh with interior vertices jittered by
+                    U(-0.2h, 0.2h), degree-6 quadrature, PCG in FP64 and mixed
+  --config batched  256 independent P1 problems, sides log-uniform in
+                    [32, 512], a = exp(N(0,1)), f = 1, one assembly launch and
+                    one graph-replayed batched PCG
+  --config adjoint  1024 x 1024 P1, J = 1/2 (u - u_obs)^T M (u - u_obs), dJ/dtheta
+                    by an adjoint solve, checked against central differences
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048] [--mode mixed|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config p1]
 """
 from __future__ import annotations
 
@@ -30,7 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "P1 assembly Melem/s + PCG time to rtol 1e-10 at 4.2M DOF; grad cost / solve"
 HBM_PEAK_GBS = 8000.0
-MAXIT = 40000  # a few seconds at the largest size; the solves converge far earlier
+MAXIT = 60000  # a few seconds at the largest size; the solves converge far earlier
 
 
 def log(msg):
@@ -48,6 +54,7 @@ class Problem:
     def __init__(self, capi, ctx, n, seed, gamma=1.0, f_sinsin=True, log_alpha=None):
         self.capi, self.ctx, self.n = capi, ctx, n
         xy, cells, _ = capi.tp_triag_mesh(n)
+        self.xy, self.cells = xy, cells
         self.ne, self.nv = cells.shape[0], xy.shape[0]
         g = (np.random.default_rng(seed).standard_normal(self.ne)
              if log_alpha is None else log_alpha)
@@ -82,30 +89,28 @@ class Problem:
 
 
 def spmv_bytes(nnz, n, val_bytes=8, vec_bytes=8):
+    """Algorithmic bytes of one CSR SpMV: values + int32 columns + row
+    pointers + x read once + y written once."""
     return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * vec_bytes * n
 
 
-def run_steps(P, mode, steps):
-    asm_ms, pcg_ms, iters = [], [], []
-    P.ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        asm_ms.append(P.assemble())
-        info = P.solve(mode)
-        pcg_ms.append(info.ms)
-        iters.append(info.iters)
-        if not info.converged:
-            raise RuntimeError("PCG did not converge")
-    P.ctx.sync()
-    wall = time.perf_counter() - t0
-    return wall, asm_ms, pcg_ms, iters, info
+def true_relres(capi, ctx, A, x, b):
+    y = capi.Vec(ctx, b.n)
+    capi.spmv(ctx, A, x, y)
+    r = b.numpy() - y.numpy()
+    y.close()
+    return float(np.linalg.norm(r) / np.linalg.norm(b.numpy()))
 
 
-def cpu_baseline(P, gpu_iters, capi):
-    """Reference-faithful CPU path (oracle, one thread) on a bounded sample."""
+def rel_err(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# ---------------------------------------------------------------- configs
+def cpu_baseline_p1(P, gpu_iters):
+    """Reference-faithful CPU path (the oracle, one thread) on a bounded sample."""
     from oracle import lfref as L  # the checker; timed here as the CPU baseline
-    ns = 512
-    m = L.tp_triag_mesh(ns)
+    m = L.tp_triag_mesh(512)
     cd, nd = m.dofs(1)
     a = np.exp(np.random.default_rng(0).standard_normal(m.ne))
     fl = m.dirichlet_flags(1)
@@ -128,101 +133,140 @@ def cpu_baseline(P, gpu_iters, capi):
         "unit": "Melem/s",
         "cores": 1,
         "kind": "port",
-        "sample": ("assembly timed on a 512x512 mesh (524,288 cells) and scaled per cell to "
-                   "%d cells; Jacobi-PCG timed for 10 iterations on the full %d-DOF system "
-                   "and scaled to the GPU's %d iterations (extrapolated)"
-                   % (P.ne, P.nv, gpu_iters)),
+        "sample": ("assembly (Eval per cell, triplets, Dirichlet fix, setFromTriplets) timed on "
+                   "a 512x512 mesh (524,288 cells) and scaled per cell to %d cells; Jacobi-PCG "
+                   "timed for 10 iterations on the full %d-DOF system and scaled to the GPU's "
+                   "%d iterations (extrapolated)" % (P.ne, P.nv, gpu_iters)),
         "assembly_us_per_cell": 1e6 * t_cell,
         "pcg_ms_per_iter": 1e3 * t_iter,
     }
 
 
-def adjoint_cost(capi, ctx, n=1024, seed=1):
-    """cost of dJ/dtheta (adjoint PCG + gradient kernel) / forward solve."""
-    P = Problem(capi, ctx, n, seed, gamma=0.0, f_sinsin=False)
+def adjoint_run(capi, ctx, n=1024, fd_dirs=0, seed_theta=1, seed_star=2):
+    """adjoint gradient of J on an n x n P1 mesh; returns the cost
+    ratio (adjoint solve + gradient kernel) / forward solve and FD errors."""
+    rng = np.random.default_rng(seed_theta)
+    P = Problem(capi, ctx, n, seed_theta, gamma=0.0, f_sinsin=False)
+    theta = P.log_alpha.copy()
+    theta_star = np.random.default_rng(seed_star).standard_normal(P.ne)
+    M = capi.Plan(ctx, P.mesh, P.flags)
+    M.assemble(P.fe, capi.coeff(capi.COEFF_CONST, 0.0), capi.coeff(capi.COEFF_CONST, 1.0),
+               None, None)
+    # observations: u(theta*) + 1 % Gaussian noise (relative to max|u|) on interior DOFs
+    P.theta.upload(theta_star)
     P.assemble()
-    uobs = capi.Vec(ctx, P.nv)
-    info_f = P.solve(capi.PCG_FP64)
-    # adjoint rhs M (u - u_obs) with u_obs = 0 here: the cost does not depend on it
-    Mplan_b = capi.Vec(ctx, P.nv)
-    lam = capi.Vec(ctx, P.nv)
-    capi.axpby(ctx, 1.0, P.x, 0.0, Mplan_b)
-    grad = capi.Vec(ctx, P.ne)
-    t_solve = []
-    for _ in range(2):
-        info_f = P.solve(capi.PCG_FP64)
-        t_solve.append(info_f.ms)
-    t_adj = []
-    for _ in range(2):
+    P.solve(capi.PCG_FP64, rtol=1e-12)
+    ustar = P.x.numpy()
+    noise = np.random.default_rng(seed_star + 1).standard_normal(P.nv) * (1 - P.flags)
+    uobs = capi.Vec(ctx, ustar + 0.01 * np.abs(ustar).max() * noise)
+    d, rhs, lam, grad = (capi.Vec(ctx, P.nv), capi.Vec(ctx, P.nv), capi.Vec(ctx, P.nv),
+                         capi.Vec(ctx, P.ne))
+
+    def J(th, rtol=1e-10):
+        P.theta.upload(th)
+        P.assemble()
+        info = P.solve(capi.PCG_FP64, rtol=rtol)
+        capi.axpby(ctx, 1.0, P.x, 0.0, d)
+        capi.axpby(ctx, -1.0, uobs, 1.0, d)
+        capi.spmv(ctx, M.csr, d, rhs)
+        return 0.5 * capi.dot(ctx, d, rhs), info
+
+    t_fwd, t_adj = [], []
+    for _ in range(3):
+        _, info = J(theta)
+        t_fwd.append(info.ms)
         lam.fill(0.0)
-        ia = capi.pcg(ctx, P.plan.csr, Mplan_b, lam, rtol=1e-10, check_every=128, maxit=MAXIT)
+        ia = capi.pcg(ctx, P.plan.csr, rhs, lam, rtol=1e-10, check_every=128, maxit=MAXIT)
         tg = capi.adjoint_gradient(ctx, P.mesh, P.fe, P.theta, P.x, lam, grad)
         t_adj.append(ia.ms + tg)
-    for h in (uobs, Mplan_b, lam, grad):
+    ratio = float(np.median(t_adj) / np.median(t_fwd))
+    out = {"grad_cost_per_solve": ratio, "forward_solve_ms": float(np.median(t_fwd)),
+           "adjoint_ms": float(np.median(t_adj)), "forward_iters": info.iters,
+           "adjoint_iters": ia.iters}
+    if fd_dirs:
+        # gradient with tight solves, then central differences with tight solves
+        J0, _ = J(theta, 1e-13)
+        lam.fill(0.0)
+        capi.pcg(ctx, P.plan.csr, rhs, lam, rtol=1e-13, check_every=128, maxit=MAXIT)
+        capi.adjoint_gradient(ctx, P.mesh, P.fe, P.theta, P.x, lam, grad)
+        gr = grad.numpy()
+        errs = []
+        eps = 1e-5
+        for k in range(fd_dirs):
+            dv = rng.standard_normal(P.ne)
+            dv /= np.linalg.norm(dv)
+            fd = (J(theta + eps * dv, 1e-13)[0] - J(theta - eps * dv, 1e-13)[0]) / (2 * eps)
+            errs.append(abs(fd - gr @ dv) / max(abs(fd), 1e-300))
+            if (k + 1) % 8 == 0:
+                log("fd direction %d: max rel err so far %.2e" % (k + 1, max(errs)))
+        out.update({"J": J0, "fd_directions": fd_dirs, "fd_max_rel_err": float(max(errs)),
+                    "fd_median_rel_err": float(np.median(errs))})
+    for h in (uobs, d, rhs, lam, grad, M):
         h.close()
     P.close()
-    return min(t_adj) / min(t_solve)
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--mode", choices=["mixed", "fp64"], default="fp64")
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--spmv-reps", type=int, default=50)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the path would shard); run with --gpus 1",
-              file=sys.stderr)
-        sys.exit(2)
-
-    from lehrfempp_amd import capi
-    ctx = capi.Context(0)
+def run_p1(args, capi, ctx):
     log("setup n=%d" % args.n)
     P = Problem(capi, ctx, args.n, args.seed)
     log("plan built in %.3f s: N=%d nnz=%d" % (P.plan_s, P.nv, P.nnz))
-    mode = capi.PCG_MIXED if args.mode == "mixed" else capi.PCG_FP64
-
+    modes = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
+             "mixed_matrix": capi.PCG_MIXED_MATRIX}
+    mode = modes[args.mode]
     for _ in range(args.warmup):
         P.assemble()
         P.solve(mode)
     log("warmup done")
-    wall, asm_ms, pcg_ms, iters, info = run_steps(P, mode, args.steps)
+    asm_ms, pcg_ms, iters = [], [], []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        asm_ms.append(P.assemble())
+        info = P.solve(mode)
+        pcg_ms.append(info.ms)
+        iters.append(info.iters)
+        if not info.converged:
+            raise RuntimeError("PCG did not converge")
+    ctx.sync()
+    wall = time.perf_counter() - t0
     log("timed %d steps: %.3f s (assembly %.3f ms, pcg %.1f ms, %d iters, relres %.2e)"
         % (args.steps, wall, np.median(asm_ms), np.median(pcg_ms), iters[-1], info.relres))
 
-    # the other precision, for the mixed-vs-FP64 throughput ratio
-    other = capi.PCG_FP64 if mode == capi.PCG_MIXED else capi.PCG_MIXED
-    info_o = P.solve(other)
-    log("other mode: %.1f ms, %d iters, relres %.2e" % (info_o.ms, info_o.iters, info_o.relres))
-    fp64_ms = info_o.ms if mode == capi.PCG_MIXED else float(np.median(pcg_ms))
-    mixed_ms = float(np.median(pcg_ms)) if mode == capi.PCG_MIXED else info_o.ms
+    # every precision mode, and a tight reference solution to measure errors
+    per_mode = {}
+    for name, md in modes.items():
+        inf = P.solve(md)
+        per_mode[name] = {"s": inf.ms / 1e3, "iters": inf.iters, "converged": inf.converged,
+                          "relres": inf.relres,
+                          "true_relres": true_relres(capi, ctx, P.plan.csr, P.x, P.b),
+                          "x": P.x.numpy()}
+        log("%s: %.1f ms, %d iters, relres %.2e, true relres %.2e"
+            % (name, inf.ms, inf.iters, inf.relres, per_mode[name]["true_relres"]))
+    P.solve(capi.PCG_MIXED_MATRIX, rtol=1e-13)
+    x_ref = P.x.numpy()
+    for name in modes:
+        per_mode[name]["err_vs_ref"] = rel_err(per_mode[name].pop("x"), x_ref)
 
-    # roofline of the dominant kernel: the FP64 sliced-ELL SpMV
+    # roofline of the dominant kernel: the FP64 SpMV sweep
     xs, ys = capi.Vec(ctx, np.random.default_rng(1).standard_normal(P.nv)), capi.Vec(ctx, P.nv)
     spmv_ms = capi.spmv_timed(ctx, P.plan.csr, xs, ys, args.spmv_reps)
     sb = spmv_bytes(P.nnz, P.nv)
     achieved = sb / (spmv_ms * 1e-3) / 1e9
-    log("spmv: %.1f us, %.0f GB/s" % (1e3 * spmv_ms, achieved))
+    log("spmv: %.1f us, %.0f GB/s (algorithmic)" % (1e3 * spmv_ms, achieved))
     xs.close()
     ys.close()
 
-    grad_ratio = None
+    adj = None
     if not args.no_adjoint:
-        grad_ratio = adjoint_cost(capi, ctx)
-        log("adjoint gradient cost / forward solve = %.2f" % grad_ratio)
-
+        adj = adjoint_run(capi, ctx)
+        log("adjoint gradient cost / forward solve = %.2f" % adj["grad_cost_per_solve"])
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(P, iters[-1], capi)
+        cpu = cpu_baseline_p1(P, iters[-1])
 
     asm_bytes = 12 * P.ne + 8 * P.ne + 16 * P.nv + 8 * P.nnz + 8 * P.nv
+    fp64_s = per_mode["fp64"]["s"]
     out = {
         "metric": METRIC,
         "value": P.ne * args.steps / wall / 1e6,
@@ -234,7 +278,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if mode == capi.PCG_FP64 else "f64 assembly, f32 inner PCG + f64 refinement",
+        "dtype": {"fp64": "f64", "mixed": "f32 Krylov + f64 refinement",
+                  "mixed_matrix": "f32 matrix + f64 Krylov"}[args.mode],
         "data": "synthetic",
         "config": {"workload": "P1 reaction-diffusion, %dx%d structured triangles, "
                                "a=exp(N(0,1)) per cell, f=sin(pi x)sin(pi y), zero Dirichlet, "
@@ -246,19 +291,158 @@ def main():
         "pcg_s": float(np.median(pcg_ms)) / 1e3,
         "pcg_iters": int(iters[-1]),
         "pcg_relres": info.relres,
-        "pcg_fp64_s": fp64_ms / 1e3,
-        "pcg_mixed_s": mixed_ms / 1e3,
-        "mixed_speedup_vs_fp64": fp64_ms / mixed_ms,
+        "pcg_modes": per_mode,
+        "mixed_speedup_vs_fp64": {k: fp64_s / v["s"] for k, v in per_mode.items()},
         "dof_per_s": P.nv / (float(np.median(pcg_ms)) * 1e-3),
-        "grad_cost_per_solve": grad_ratio,
+        "grad_cost_per_solve": None if adj is None else adj["grad_cost_per_solve"],
+        "adjoint": adj,
         "plan_build_s": P.plan_s,
-        "roofline": {"bound": "hbm", "kernel": "k_spmv64 (FP64 sliced-ELL SpMV)",
+        "roofline": {"bound": "hbm", "kernel": "k_spmv_f64 (FP64 SpMV, DIA storage)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes": sb, "avg_us": 1e3 * spmv_ms},
         "cpu_baseline": cpu,
     }
     P.close()
+    return out
+
+
+def run_p3(args, capi, ctx, n=512, seed=0):
+    xy, cells, edges = capi.tp_triag_mesh(n)
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    inner = (xy[:, 0] > 0) & (xy[:, 0] < 1) & (xy[:, 1] > 0) & (xy[:, 1] < 1)
+    xy = xy.copy()
+    xy[inner] += rng.uniform(-0.2 * h, 0.2 * h, size=(int(inner.sum()), 2))
+    ev, ce, co = capi.mesh_edges(xy.shape[0], cells, edges)
+    cd, nd = capi.dof_table(xy.shape[0], cells, len(ev), ce, co, 3)
+    # boundary DOFs: boundary vertices and the interior DOFs of boundary edges
+    vb = ((xy[:, 0] == 0) | (xy[:, 0] == 1) | (xy[:, 1] == 0) | (xy[:, 1] == 1))
+    eb = vb[ev[:, 0]] & vb[ev[:, 1]] & (
+        (xy[ev[:, 0], 0] == xy[ev[:, 1], 0]) | (xy[ev[:, 0], 1] == xy[ev[:, 1], 1]))
+    flags = np.zeros(nd, np.uint8)
+    flags[: xy.shape[0]] = vb
+    be = np.nonzero(eb)[0]
+    flags[xy.shape[0] + 2 * be] = 1
+    flags[xy.shape[0] + 2 * be + 1] = 1
+    mesh = capi.Mesh(ctx, xy, cells, cd, nd)
+    fe = capi.FE(ctx, 3)
+    plan = capi.Plan(ctx, mesh, flags)
+    nnz = plan.csr.info()[2]
+    theta = capi.Vec(ctx, np.random.default_rng(seed + 1).standard_normal(cells.shape[0]))
+    b, x = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
+    args_c = (fe, capi.coeff(capi.COEFF_EXPCELL, 1.0, theta), capi.coeff(capi.COEFF_CONST, 1.0),
+              capi.coeff(capi.COEFF_SINSIN, 1.0), b)
+    plan.assemble(*args_c)
+    asm = [plan.assemble(*args_c) for _ in range(3)]
+    log("p3: N=%d nnz=%d assembly %.3f ms" % (nd, nnz, np.median(asm)))
+    res = {}
+    for name, md in (("fp64", capi.PCG_FP64), ("mixed_matrix", capi.PCG_MIXED_MATRIX),
+                     ("mixed", capi.PCG_MIXED)):
+        x.fill(0.0)
+        inf = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, mode=md, check_every=128, maxit=MAXIT,
+                       raise_on_noconv=False)
+        res[name] = {"s": inf.ms / 1e3, "iters": inf.iters, "converged": inf.converged,
+                     "relres": inf.relres, "x": x.numpy(),
+                     "true_relres": true_relres(capi, ctx, plan.csr, x, b)}
+        log("p3 %s: %.1f ms, %d iters, true relres %.2e" % (name, inf.ms, inf.iters,
+                                                            res[name]["true_relres"]))
+    x.fill(0.0)
+    capi.pcg(ctx, plan.csr, b, x, rtol=1e-13, mode=capi.PCG_MIXED_MATRIX, check_every=128,
+             maxit=MAXIT)
+    xr = x.numpy()
+    for v in res.values():
+        v["err_vs_ref"] = rel_err(v.pop("x"), xr)
+    out = {"metric": "P3 assembly Melem/s + PCG time to rtol 1e-10", "config": {
+        "workload": "P3 reaction-diffusion, %dx%d jittered (0.2h) triangles, degree-6 "
+                    "quadrature, a=exp(N(0,1)), f=sin sin" % (n, n),
+        "cells": int(cells.shape[0]), "dofs": int(nd), "nnz": int(nnz)},
+        "assembly_ms": float(np.median(asm)),
+        "assembly_melem_s": cells.shape[0] / (np.median(asm) * 1e-3) / 1e6,
+        "pcg": res,
+        "mixed_speedup_vs_fp64": {k: res["fp64"]["s"] / v["s"] for k, v in res.items()}}
+    for hh in (b, x, theta, plan, fe, mesh):
+        hh.close()
+    return out
+
+
+def run_batched(args, capi, ctx, nprob=256, seed=4):
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), nprob))).astype(int)
+    xy, cells, flags, voff, coff = capi.batched_tp_mesh(sizes)
+    ne, nv = cells.shape[0], xy.shape[0]
+    log("batched: %d problems, %d cells, %d DOFs" % (nprob, ne, nv))
+    g = np.concatenate([np.random.default_rng(seed + 1 + k).standard_normal(int(coff[k + 1] - coff[k]))
+                        for k in range(nprob)])
+    mesh = capi.Mesh(ctx, xy, cells)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, mesh, flags)
+    theta = capi.Vec(ctx, g)
+    b, x = capi.Vec(ctx, nv), capi.Vec(ctx, nv)
+    acoef = (fe, capi.coeff(capi.COEFF_EXPCELL, 1.0, theta), capi.coeff(capi.COEFF_CONST, 0.0),
+             capi.coeff(capi.COEFF_CONST, 1.0), b)
+    plan.assemble(*acoef)
+    batch = capi.Batch(ctx, plan.csr, voff)
+    x.fill(0.0)
+    batch.pcg(b, x, rtol=1e-10, maxit=MAXIT)  # tunes and captures the graph
+    times = []
+    for _ in range(max(1, args.steps)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        asm_ms = plan.assemble(*acoef)
+        x.fill(0.0)
+        summ, infos = batch.pcg(b, x, rtol=1e-10, maxit=MAXIT)
+        ctx.sync()
+        times.append((time.perf_counter() - t0, asm_ms, summ.ms))
+    wall = float(np.median([t[0] for t in times]))
+    it = [i.iters for i in infos]
+    out = {"metric": "batched P1 assemble + PCG, Melem/s", "value": ne / wall / 1e6,
+           "unit": "Melem/s", "config": {"workload": "%d P1 problems, sides log-uniform in "
+                                                     "[32,512]" % nprob,
+                                         "cells": int(ne), "dofs": int(nv),
+                                         "nnz": int(plan.csr.info()[2])},
+           "ms_per_step": 1e3 * wall, "assembly_ms": float(np.median([t[1] for t in times])),
+           "pcg_ms": float(np.median([t[2] for t in times])), "iters_max": int(max(it)),
+           "iters_min": int(min(it)), "converged": bool(summ.converged),
+           "worst_relres": summ.relres, "tuned_rows_per_workgroup": summ.refinements}
+    for hh in (batch, b, x, theta, plan, fe, mesh):
+        hh.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint"], default="p1")
+    ap.add_argument("--mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--fd-dirs", type=int, default=64)
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(DESIGN.md describes how the path would shard); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from lehrfempp_amd import capi
+    ctx = capi.Context(0)
+    if args.config == "p1":
+        out = run_p1(args, capi, ctx)
+    elif args.config == "p3":
+        out = run_p3(args, capi, ctx)
+    elif args.config == "batched":
+        out = run_batched(args, capi, ctx)
+    else:
+        out = adjoint_run(capi, ctx, n=1024, fd_dirs=args.fd_dirs)
+        out = {"metric": "adjoint gradient cost / forward solve", "config": {
+            "workload": "P1 1024x1024, theta ~ N(0,1) per cell, u_obs = u(theta*) + 1% noise"},
+            **out}
     ctx.close()
     print(json.dumps(out))
 

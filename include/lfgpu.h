@@ -204,12 +204,17 @@ int lfgpu_spmv_timed(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *x,
  * ConjugateGradient<.., DiagonalPreconditioner>: stop when
  * ||b - A x||_2 < rtol ||b||_2.
  * mode LFGPU_PCG_FP64: everything in FP64.
- * mode LFGPU_PCG_MIXED: CG recurrences, matrix and SpMV in FP32 with
- *   reliable updates: the true residual b - A x is recomputed in FP64 and the
- *   solution accumulated in FP64; convergence is always judged on the FP64
- *   true residual. */
+ * mode LFGPU_PCG_MIXED: matrix, vectors and CG recurrences in FP32 (dot
+ *   products and SpMV row sums accumulated in FP64) with reliable updates:
+ *   the true residual b - A x is recomputed in FP64 whenever the recurrence
+ *   residual has fallen by refine_delta, the solution is accumulated as a
+ *   double-double pair, and convergence is judged on the FP64 true residual.
+ * mode LFGPU_PCG_MIXED_MATRIX: as LFGPU_PCG_MIXED, but only the matrix values
+ *   (and the Jacobi diagonal) are stored in FP32; the Krylov vectors stay
+ *   FP64. */
 #define LFGPU_PCG_FP64 0
 #define LFGPU_PCG_MIXED 1
+#define LFGPU_PCG_MIXED_MATRIX 2
 typedef struct lfgpu_pcg_opts {
   double rtol;     /* relative residual target                    */
   int maxit;       /* iteration cap (Eigen: 2*n by default)       */
@@ -230,6 +235,27 @@ typedef struct lfgpu_pcg_info {
 } lfgpu_pcg_info;
 int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *b,
               lfgpu_vec *x, const lfgpu_pcg_opts *opts, lfgpu_pcg_info *info);
+
+/* ---- batched solves --------------------------------------------------
+ * Many independent SPD systems stored as one block-diagonal matrix (e.g. the
+ * plan of a mesh that is the disjoint union of the problems' meshes, so one
+ * assembly launch assembles all of them).  row_offsets [nprob+1] (host)
+ * partitions the rows; problem p owns rows [row_offsets[p], row_offsets[p+1])
+ * and gets its own Jacobi-PCG (Eigen semantics per problem).  All problems
+ * advance together, one pair of sweeps per iteration; chunks of iterations
+ * are captured into a hipGraph once and replayed, and the workgroup tile is
+ * chosen by timing candidates on the first solve.  No reference
+ * counterpart: a batch replaces nprob separate solve calls. */
+typedef struct lfgpu_batch lfgpu_batch;
+int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
+                       const int32_t *row_offsets, lfgpu_batch **out);
+int lfgpu_batch_destroy(lfgpu_batch *bt);
+/* infos: nprob entries (or NULL); summary: iters = max over problems,
+ * converged = all converged, relres = worst, refinements = the tuned rows
+ * per workgroup, ms = device time.  Only LFGPU_PCG_FP64 is supported. */
+int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec *b,
+                    lfgpu_vec *x, const lfgpu_pcg_opts *opts,
+                    lfgpu_pcg_info *infos, lfgpu_pcg_info *summary);
 
 /* ---- adjoint gradient ----------------------------------------------------
  * For the P1 problem A(theta) u = f with A(theta) = sum_e exp(theta_e) K_e

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "liblfgpu.so")
 
 OK, EINVAL, EDEGENERATE, ENOCONV, EOOM, EHIP, ENORULE, ERANGE = range(8)
 COEFF_CONST, COEFF_CELL, COEFF_QP, COEFF_SINSIN, COEFF_EXPCELL = range(5)
-PCG_FP64, PCG_MIXED = 0, 1
+PCG_FP64, PCG_MIXED, PCG_MIXED_MATRIX = 0, 1, 2
 
 _P = C.c_void_p
 _d = C.POINTER(C.c_double)
@@ -83,6 +83,10 @@ SIGNATURES = {
     "lfgpu_spmv": (C.c_int, [_P, _P, _P, _P]),
     "lfgpu_spmv_timed": (C.c_int, [_P, _P, _P, _P, C.c_int, _d]),
     "lfgpu_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo)]),
+    "lfgpu_batch_create": (C.c_int, [_P, _P, C.c_int, _i32, C.POINTER(_P)]),
+    "lfgpu_batch_destroy": (C.c_int, [_P]),
+    "lfgpu_batch_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo),
+                                  C.POINTER(PcgInfo)]),
     "lfgpu_adjoint_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "lfgpu_axpby": (C.c_int, [_P, C.c_double, _P, C.c_double, _P]),
     "lfgpu_dot": (C.c_int, [_P, _P, _P, _d]),
@@ -345,6 +349,34 @@ def pcg(ctx, A: Csr, b: Vec, x: Vec, rtol=1e-10, maxit=None, mode=PCG_FP64, chec
     return info
 
 
+class Batch:
+    """Batched Jacobi-PCG over a block-diagonal matrix (lfgpu_batch_*)."""
+
+    def __init__(self, ctx, A: Csr, row_offsets):
+        self.ctx = ctx
+        self.off = np.ascontiguousarray(row_offsets, dtype=np.int32)
+        self.nprob = self.off.size - 1
+        self.h = _P()
+        check(lib().lfgpu_batch_create(ctx.h, A.h, self.nprob, _ptr(self.off, C.c_int32),
+                                       C.byref(self.h)))
+
+    def pcg(self, b: Vec, x: Vec, rtol=1e-10, maxit=100000, check_every=64,
+            raise_on_noconv=True):
+        opts = PcgOpts(rtol, int(maxit), PCG_FP64, check_every, 0.0, 0)
+        infos = (PcgInfo * self.nprob)()
+        summ = PcgInfo()
+        rc = lib().lfgpu_batch_pcg(self.ctx.h, self.h, b.h, x.h, C.byref(opts), infos,
+                                   C.byref(summ))
+        if rc != OK and (rc != ENOCONV or raise_on_noconv):
+            check(rc)
+        return summ, list(infos)
+
+    def close(self):
+        if self.h:
+            lib().lfgpu_batch_destroy(self.h)
+            self.h = _P()
+
+
 def adjoint_gradient(ctx, mesh: Mesh, fe: FE, theta: Vec, u: Vec, lam: Vec, grad: Vec):
     check(lib().lfgpu_adjoint_gradient(ctx.h, mesh.h, fe.h, theta.h, u.h, lam.h, grad.h))
     return ctx.last_ms()
@@ -373,6 +405,22 @@ def tp_triag_mesh(nx, ny=None, box=(0.0, 0.0, 1.0, 1.0)):
                                     _ptr(xy, C.c_double), _ptr(cells, C.c_int32),
                                     _ptr(edges, C.c_int32)))
     return xy, cells, edges
+
+
+def batched_tp_mesh(sizes, box=(0.0, 0.0, 1.0, 1.0)):
+    """Disjoint union of n x n TPTriagMeshBuilder meshes, in order: (xy, cells,
+    boundary flags, vertex offsets [len(sizes)+1], cell offsets)."""
+    xs, cs, fl, voff, coff = [], [], [], [0], [0]
+    for n in sizes:
+        xy, cells, _ = tp_triag_mesh(int(n), int(n), box)
+        j, i = np.divmod(np.arange(xy.shape[0]), n + 1)
+        fl.append(((i == 0) | (i == n) | (j == 0) | (j == n)).astype(np.uint8))
+        xs.append(xy)
+        cs.append(cells + voff[-1])
+        voff.append(voff[-1] + xy.shape[0])
+        coff.append(coff[-1] + cells.shape[0])
+    return (np.concatenate(xs), np.ascontiguousarray(np.concatenate(cs), np.int32),
+            np.concatenate(fl), np.array(voff, np.int64), np.array(coff, np.int64))
 
 
 def mesh_edges(nv, cells, pre_edges=None):
