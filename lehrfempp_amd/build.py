@@ -43,6 +43,22 @@ def _compile(src: str) -> str:
     return obj
 
 
+DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "homdir_demo.cc")
+DEMO_BIN = os.path.join(ROOT, "tests", "cpp", "homdir_demo")
+
+
+def build_cpp_demo() -> str:
+    """Compile the C++ facade driver (tests/cpp/homdir_demo.cc) with g++
+    against liblfgpu.so."""
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-I", ROOT, "-I", os.path.join(ROOT, "include"),
+           DEMO_SRC, "-L", HERE, "-llfgpu", "-Wl,-rpath,$ORIGIN/../../lehrfempp_amd",
+           "-o", DEMO_BIN]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed for the C++ facade driver:\n%s" % res.stderr[-8000:])
+    return DEMO_BIN
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -53,8 +69,9 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), res.stderr[-8000:]))
     os.replace(tmp, LIB)
+    build_cpp_demo()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DEMO_BIN)
     return LIB
 
 

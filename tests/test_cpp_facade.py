@@ -1,0 +1,27 @@
+"""The C++ facade (lehrfempp_amd/include/lf_amd/lf_amd.h) over the C ABI:
+it compiles with the host toolchain, and its driver in the shape of
+homDir_linfe_demo.cc reproduces the fused device path through the
+reference calling pattern (run on the GPU)."""
+import os
+import subprocess
+
+import pytest
+
+from lehrfempp_amd import build as B
+
+
+def test_facade_driver_compiles(tmp_path):
+    out = str(tmp_path / "demo")
+    cmd = ["g++", "-std=c++20", "-O0", "-Wall", "-Werror", "-I", B.ROOT, "-I",
+           os.path.join(B.ROOT, "include"), B.DEMO_SRC, "-L", B.HERE, "-llfgpu", "-o", out]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-4000:]
+
+
+@pytest.mark.gpu
+def test_facade_driver_runs_on_gpu():
+    assert os.path.exists(B.DEMO_BIN), "build the driver first (python -m lehrfempp_amd.build)"
+    res = subprocess.run([B.DEMO_BIN], capture_output=True, text=True, timeout=120)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "OK" in res.stdout
