@@ -476,6 +476,49 @@ __device__ __forceinline__ void store_partial(double *part, double v, double *sh
   if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
+// Several sums at once over a 256-thread workgroup; every thread ends with
+// the totals (summed in the same order in every workgroup).  sh needs 4*K
+// doubles.
+template <int K>
+__device__ __forceinline__ void block_sum_k(double (&v)[K], double *sh) {
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[4 * k + w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = ((sh[4 * k] + sh[4 * k + 1]) + sh[4 * k + 2]) + sh[4 * k + 3];
+}
+
+// Interleaved partials of the FP64 iteration: group g's slots at part[8 g + k].
+constexpr int kIlv = 8;
+
+template <int K>
+__device__ __forceinline__ void store_partials_ilv(double *part, double (&v)[K], const int (&slots)[K],
+                                                   double *sh) {
+  block_sum_k<K>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[kIlv * blockIdx.x + slots[k]] = v[k];
+}
+
+template <int K>
+__device__ __forceinline__ void reduce_partials_ilv(const double *part, int ngroups,
+                                                    const int (&slots)[K], double (&out)[K],
+                                                    double *sh) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = 0.0;
+  for (int g = threadIdx.x; g < ngroups; g += 256)
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] += part[kIlv * g + slots[k]];
+  block_sum_k<K>(out, sh);
+}
+
 // partial slots (FP64 mode); P_RR and P_RZ are followed by their parity twin
 constexpr int P_PQ = 0, P_QDR = 1, P_QDQ = 2, P_RR = 3, P_RZ = 5, P_BB = 7;
 // mixed mode (after the shared setup)
@@ -499,7 +542,7 @@ struct CgArgs {
 // 1 slots, which iteration 0 reads)
 template <int FMT>
 __global__ __launch_bounds__(256) void k_cg_init(CgArgs a, const double *__restrict__ b) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double bb = 0.0, rr = 0.0, rz = 0.0;
@@ -514,17 +557,18 @@ __global__ __launch_bounds__(256) void k_cg_init(CgArgs a, const double *__restr
     rr += ri * ri;
     rz += ri * zi;
   }
-  store_partial(slot(a.part, P_BB), bb, sh);
-  store_partial(slot(a.part, P_RR + 1), rr, sh);
-  store_partial(slot(a.part, P_RZ + 1), rz, sh);
+  double v[3] = {bb, rr, rz};
+  const int sl[3] = {P_BB, P_RR + 1, P_RZ + 1};
+  store_partials_ilv<3>(a.part, v, sl, sh);
 }
 
 // one workgroup: Eigen's early exits (zero rhs, converged start, maxit 0)
 __global__ __launch_bounds__(256) void k_cg_init_finish(CgArgs a) {
-  __shared__ double sh[8];
-  const double bb = all_reduce(slot(a.part, P_BB), a.sw.ngroups, sh);
-  const double rr = all_reduce(slot(a.part, P_RR + 1), a.sw.ngroups, sh);
-  const double rz = all_reduce(slot(a.part, P_RZ + 1), a.sw.ngroups, sh);
+  __shared__ double sh[16];
+  double t[3];
+  const int sl[3] = {P_BB, P_RR + 1, P_RZ + 1};
+  reduce_partials_ilv<3>(a.part, a.sw.ngroups, sl, t, sh);
+  const double bb = t[0], rr = t[1], rz = t[2];
   if (threadIdx.x == 0) {
     PcgState *st = a.st;
     st->bb = bb;
@@ -562,10 +606,13 @@ __global__ void k_zero_if_bb0(double *x, int32_t n, const PcgState *st) {
 // p.q, q.D^-1 r and q.D^-1 q.
 template <int FMT>
 __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   PcgState *st = a.st;
   if (st->done) return;
-  const double rr = all_reduce(slot(a.part, P_RR + (par ^ 1)), a.sw.ngroups, sh);
+  double t[1];
+  const int sl_in[1] = {P_RR + (par ^ 1)};
+  reduce_partials_ilv<1>(a.part, a.sw.ngroups, sl_in, t, sh);
+  const double rr = t[0];
   if (rr < st->thr) {  // converged: the iteration count stays as it is
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->rr = rr;
@@ -583,6 +630,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
   const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double pq = 0.0, qdr = 0.0, qdq = 0.0;
+#pragma unroll 2
   for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
     const double pi = a.p[i];
     const double qi = apply64<FMT>(a.m, a.p, i, pi);
@@ -592,9 +640,9 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
     qdr += dq * a.r[i];
     qdq += dq * qi;
   }
-  store_partial(slot(a.part, P_PQ), pq, sh);
-  store_partial(slot(a.part, P_QDR), qdr, sh);
-  store_partial(slot(a.part, P_QDQ), qdq, sh);
+  double v[3] = {pq, qdr, qdq};
+  const int sl[3] = {P_PQ, P_QDR, P_QDQ};
+  store_partials_ilv<3>(a.part, v, sl, sh);
 }
 
 // FP64 iteration k, sweep 2: alpha = rho_k / (p.q);
@@ -603,19 +651,20 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
 //   x += alpha p ; r -= alpha q ; p = D^-1 r + beta p ;
 //   partials ||r||^2 and (r, D^-1 r) of the new residual (parity par).
 __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   PcgState *st = a.st;
   if (st->done) return;
-  const double rho = all_reduce(slot(a.part, P_RZ + (par ^ 1)), a.sw.ngroups, sh);
-  const double pq = all_reduce(slot(a.part, P_PQ), a.sw.ngroups, sh);
-  const double qdr = all_reduce(slot(a.part, P_QDR), a.sw.ngroups, sh);
-  const double qdq = all_reduce(slot(a.part, P_QDQ), a.sw.ngroups, sh);
+  double t[4];
+  const int sl_in[4] = {P_RZ + (par ^ 1), P_PQ, P_QDR, P_QDQ};
+  reduce_partials_ilv<4>(a.part, a.sw.ngroups, sl_in, t, sh);
+  const double rho = t[0], pq = t[1], qdr = t[2], qdq = t[3];
   const double alpha = rho / pq;
   const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
   const double beta = rho_next / rho;
   const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
+#pragma unroll 2
   for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
     const double pi = a.p[i];
     a.x[i] += alpha * pi;
@@ -626,8 +675,9 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
     rr += ri * ri;
     rz += ri * zi;
   }
-  store_partial(slot(a.part, P_RR + par), rr, sh);
-  store_partial(slot(a.part, P_RZ + par), rz, sh);
+  double v[2] = {rr, rz};
+  const int sl[2] = {P_RR + par, P_RZ + par};
+  store_partials_ilv<2>(a.part, v, sl, sh);
 }
 
 // ---------------------------------------------------------------------------

@@ -483,3 +483,94 @@ def test_adjoint_gradient_against_finite_differences(ctx):
         assert abs(fd - gr @ dv) <= 1e-6 * max(abs(fd), 1e-3 * abs(J0)), (fd, gr @ dv)
     for h in (vobs, d, b, u, lam, rhs, grad, vt, M, A, fe, g):
         h.close()
+
+
+# ------------------------------------------------- unstructured / edge cases
+def delaunay_mesh(npts, seed):
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    g = np.linspace(0.0, 1.0, 9)
+    border = np.array([(x, y) for x in g for y in (0.0, 1.0)] + [(x, y) for y in g[1:-1] for x in (0.0, 1.0)])
+    pts = np.vstack([border, rng.uniform(0.02, 0.98, size=(npts, 2))])
+    tri = Delaunay(pts).simplices.astype(np.int32)
+    # counter-clockwise orientation is not required; keep Delaunay's order
+    return L.TriMesh(pts, tri)
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_unstructured_mesh_assembly_and_solve(ctx, degree):
+    m = delaunay_mesh(400, 5 + degree)
+    rng = np.random.default_rng(degree)
+    alpha = np.exp(rng.standard_normal(m.ne))
+    (rp, col, val, b), (rpo, colo, valo, bo) = _assemble_both(ctx, m, degree, alpha, 1.0,
+                                                              L.F_SINSIN)
+    assert np.array_equal(rp, rpo) and np.array_equal(col, colo)
+    assert np.array_equal(val, valo)
+    assert np.abs(b - bo).max() <= 1e-15 * np.abs(bo).max() * 8
+    # solve on the device (sliced-ELL storage) vs the direct solve
+    A = capi.coo_to_csr(ctx, np.repeat(np.arange(len(rp) - 1), np.diff(rp)), col, val,
+                        len(rp) - 1, len(rp) - 1)
+    vb, x = capi.Vec(ctx, b), capi.Vec(ctx, len(b))
+    info = capi.pcg(ctx, A, vb, x, rtol=1e-12)
+    xd = L.band_cholesky_solve(rpo, colo, valo, bo)
+    assert info.converged and np.linalg.norm(x.numpy() - xd) <= 1e-9 * np.linalg.norm(xd)
+    for h in (vb, x, A):
+        h.close()
+
+
+def test_plan_without_fixed_dofs_is_the_plain_pattern(ctx):
+    # no Dirichlet selector: every triplet stays (Neumann matrix, rows sum to 0)
+    m = L.tp_triag_mesh(12)
+    g = gpu_mesh(ctx, m, 1)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, None)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CONST, 1.0), capi.coeff(capi.COEFF_CONST, 0.0),
+                  None, None)
+    rp, col, val = plan.csr.download()
+    cd, nd = m.dofs(1)
+    ro = L.set_from_triplets(*L.assemble_coo(cd, L.element_matrices(m, 1, 1.0, 0.0)), nd)
+    for u, v in zip((rp, col, val), ro):
+        assert np.array_equal(u, v)
+    rs = np.add.reduceat(val, rp[:-1])
+    assert np.abs(rs).max() < 1e-12  # bvp_fe_tests.cc:57-60
+    for h in (plan, fe, g):
+        h.close()
+
+
+def test_all_dofs_fixed_gives_identity(ctx):
+    m = L.tp_triag_mesh(3)
+    g = gpu_mesh(ctx, m, 1)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, np.ones(m.nv, np.uint8))
+    vals = capi.Vec(ctx, np.arange(m.nv, dtype=float))
+    b = capi.Vec(ctx, m.nv)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CONST, 1.0), capi.coeff(capi.COEFF_CONST, 0.0),
+                  capi.coeff(capi.COEFF_CONST, 1.0), b, vals)
+    rp, col, val = plan.csr.download()
+    assert np.array_equal(col, np.arange(m.nv)) and np.all(val == 1.0)
+    assert np.array_equal(b.numpy(), np.arange(m.nv, dtype=float))
+    for h in (vals, b, plan, fe, g):
+        h.close()
+
+
+def test_invalid_inputs_are_rejected(ctx):
+    with pytest.raises(capi.LfgpuError) as e:
+        capi.Mesh(ctx, np.zeros((0, 2)), np.zeros((0, 3), np.int32))
+    assert e.value.code == capi.EINVAL
+    with pytest.raises(capi.LfgpuError) as e:  # DOF index outside [0, ndofs)
+        capi.Mesh(ctx, np.array([[0, 0], [1, 0], [0, 1.0]]), np.array([[0, 1, 2]]),
+                  np.array([[0, 1, 2, 3, 4, 5, 6, 7, 8, 99]]), 10)
+    assert e.value.code == capi.ERANGE
+    m = L.tp_triag_mesh(4)
+    g = gpu_mesh(ctx, m, 1)
+    fe3 = capi.FE(ctx, 3)
+    with pytest.raises(capi.LfgpuError) as e:  # P3 element on a P1 DOF layout
+        capi.element_matrices(ctx, g, fe3, capi.coeff(), capi.coeff())
+    assert e.value.code == capi.EINVAL
+    fe1 = capi.FE(ctx, 1)
+    short = capi.Vec(ctx, 5)
+    with pytest.raises(capi.LfgpuError) as e:  # per-cell data of the wrong length
+        capi.element_matrices(ctx, g, fe1, capi.coeff(capi.COEFF_CELL, 1.0, short), capi.coeff())
+    assert e.value.code == capi.EINVAL
+    for h in (short, fe1, fe3, g):
+        h.close()
