@@ -106,6 +106,13 @@ def rel_err(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def row_scaled_err(rp, a, b):
+    """max over rows of max |a - b| / max |b| within the row (CSR values)."""
+    starts = rp[:-1][np.diff(rp) > 0]
+    scale = np.maximum(np.maximum.reduceat(np.abs(b), starts), 1e-300)
+    return float((np.maximum.reduceat(np.abs(a - b), starts) / scale).max())
+
+
 # ---------------------------------------------------------------- configs
 def cpu_baseline_p1(P, gpu_iters):
     """Reference-faithful CPU path (the oracle, one thread) on a bounded sample."""
@@ -248,6 +255,13 @@ def run_p1(args, capi, ctx):
     for name in modes:
         per_mode[name]["err_vs_ref"] = rel_err(per_mode[name].pop("x"), x_ref)
 
+    # the two-phase assembly (matrix-core element matrices, then the gather)
+    P.plan.assemble(P.fe, P.alpha, P.gamma, P.f, P.b, mfma=True)
+    asm_mfma = [P.plan.assemble(P.fe, P.alpha, P.gamma, P.f, P.b, mfma=True) for _ in range(3)]
+    P.assemble()
+    log("assembly: row-owner %.3f ms, MFMA two-phase %.3f ms"
+        % (np.median(asm_ms), np.median(asm_mfma)))
+
     # roofline of the dominant kernel: the FP64 SpMV sweep
     xs, ys = capi.Vec(ctx, np.random.default_rng(1).standard_normal(P.nv)), capi.Vec(ctx, P.nv)
     spmv_ms = capi.spmv_timed(ctx, P.plan.csr, xs, ys, args.spmv_reps)
@@ -288,6 +302,7 @@ def run_p1(args, capi, ctx):
         "assembly_melem_s": P.ne / (np.median(asm_ms) * 1e-3) / 1e6,
         "assembly_ms": float(np.median(asm_ms)),
         "assembly_gbs_algorithmic": asm_bytes / (np.median(asm_ms) * 1e-3) / 1e9,
+        "assembly_mfma_two_phase_ms": float(np.median(asm_mfma)),
         "pcg_s": float(np.median(pcg_ms)) / 1e3,
         "pcg_iters": int(iters[-1]),
         "pcg_relres": info.relres,
@@ -335,7 +350,23 @@ def run_p3(args, capi, ctx, n=512, seed=0):
               capi.coeff(capi.COEFF_SINSIN, 1.0), b)
     plan.assemble(*args_c)
     asm = [plan.assemble(*args_c) for _ in range(3)]
-    log("p3: N=%d nnz=%d assembly %.3f ms" % (nd, nnz, np.median(asm)))
+    log("p3: N=%d nnz=%d assembly (exact element order) %.3f ms" % (nd, nnz, np.median(asm)))
+    rp, _, v_exact = plan.csr.download()
+    b_exact = b.numpy()
+    # reference solution on the matrix that equals the oracle's bit for bit
+    x.fill(0.0)
+    capi.pcg(ctx, plan.csr, b, x, rtol=1e-13, mode=capi.PCG_MIXED_MATRIX, check_every=128,
+             maxit=MAXIT)
+    xr = x.numpy()
+    # element matrices on the matrix cores; the solves below use this matrix
+    plan.assemble(*args_c, mfma=True)
+    asm_mfma = [plan.assemble(*args_c, mfma=True) for _ in range(3)]
+    _, _, v_mfma = plan.csr.download()
+    mat_err = row_scaled_err(rp, v_mfma, v_exact)
+    rhs_equal = bool(np.array_equal(b.numpy(), b_exact))
+    del v_exact, v_mfma
+    log("p3: assembly (MFMA contraction) %.3f ms, matrix row-scaled diff %.2e, rhs equal %s"
+        % (np.median(asm_mfma), mat_err, rhs_equal))
     res = {}
     for name, md in (("fp64", capi.PCG_FP64), ("mixed_matrix", capi.PCG_MIXED_MATRIX),
                      ("mixed", capi.PCG_MIXED)):
@@ -347,10 +378,6 @@ def run_p3(args, capi, ctx, n=512, seed=0):
                      "true_relres": true_relres(capi, ctx, plan.csr, x, b)}
         log("p3 %s: %.1f ms, %d iters, true relres %.2e" % (name, inf.ms, inf.iters,
                                                             res[name]["true_relres"]))
-    x.fill(0.0)
-    capi.pcg(ctx, plan.csr, b, x, rtol=1e-13, mode=capi.PCG_MIXED_MATRIX, check_every=128,
-             maxit=MAXIT)
-    xr = x.numpy()
     for v in res.values():
         v["err_vs_ref"] = rel_err(v.pop("x"), xr)
     out = {"metric": "P3 assembly Melem/s + PCG time to rtol 1e-10", "config": {
@@ -359,6 +386,10 @@ def run_p3(args, capi, ctx, n=512, seed=0):
         "cells": int(cells.shape[0]), "dofs": int(nd), "nnz": int(nnz)},
         "assembly_ms": float(np.median(asm)),
         "assembly_melem_s": cells.shape[0] / (np.median(asm) * 1e-3) / 1e6,
+        "assembly_mfma_ms": float(np.median(asm_mfma)),
+        "assembly_mfma_melem_s": cells.shape[0] / (np.median(asm_mfma) * 1e-3) / 1e6,
+        "mfma_matrix_row_scaled_diff": mat_err, "mfma_rhs_bitwise_equal": rhs_equal,
+        "pcg_matrix": "MFMA-contracted; err_vs_ref against the exact-order matrix's solution",
         "pcg": res,
         "mixed_speedup_vs_fp64": {k: res["fp64"]["s"] / v["s"] for k, v in res.items()}}
     for hh in (b, x, theta, plan, fe, mesh):
@@ -383,28 +414,51 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
              capi.coeff(capi.COEFF_CONST, 1.0), b)
     plan.assemble(*acoef)
     batch = capi.Batch(ctx, plan.csr, voff)
+    # reference: every problem to rtol 1e-13 (FP32 matrix, FP64 vectors and
+    # true-residual stopping test)
     x.fill(0.0)
-    batch.pcg(b, x, rtol=1e-10, maxit=MAXIT)  # tunes and captures the graph
-    times = []
-    for _ in range(max(1, args.steps)):
-        ctx.sync()
-        t0 = time.perf_counter()
-        asm_ms = plan.assemble(*acoef)
+    batch.pcg(b, x, rtol=1e-13, maxit=MAXIT, mode=capi.PCG_MIXED_MATRIX)
+    x_ref = x.numpy()
+    modes = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
+             "mixed_matrix": capi.PCG_MIXED_MATRIX}
+    per_mode = {}
+    for name, md in modes.items():
         x.fill(0.0)
-        summ, infos = batch.pcg(b, x, rtol=1e-10, maxit=MAXIT)
-        ctx.sync()
-        times.append((time.perf_counter() - t0, asm_ms, summ.ms))
-    wall = float(np.median([t[0] for t in times]))
-    it = [i.iters for i in infos]
-    out = {"metric": "batched P1 assemble + PCG, Melem/s", "value": ne / wall / 1e6,
-           "unit": "Melem/s", "config": {"workload": "%d P1 problems, sides log-uniform in "
-                                                     "[32,512]" % nprob,
-                                         "cells": int(ne), "dofs": int(nv),
-                                         "nnz": int(plan.csr.info()[2])},
-           "ms_per_step": 1e3 * wall, "assembly_ms": float(np.median([t[1] for t in times])),
-           "pcg_ms": float(np.median([t[2] for t in times])), "iters_max": int(max(it)),
-           "iters_min": int(min(it)), "converged": bool(summ.converged),
-           "worst_relres": summ.relres, "tuned_rows_per_workgroup": summ.refinements}
+        batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md)  # tunes / captures the graph
+        times = []
+        for _ in range(max(1, args.steps)):
+            ctx.sync()
+            t0 = time.perf_counter()
+            asm_ms = plan.assemble(*acoef)
+            x.fill(0.0)
+            summ, infos = batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md)
+            ctx.sync()
+            times.append((time.perf_counter() - t0, asm_ms, summ.ms))
+        xs = x.numpy()
+        errs = [rel_err(xs[voff[k]:voff[k + 1]], x_ref[voff[k]:voff[k + 1]])
+                for k in range(nprob)]
+        it = [i.iters for i in infos]
+        wall = float(np.median([t[0] for t in times]))
+        per_mode[name] = {
+            "melem_s": ne / wall / 1e6, "ms_per_step": 1e3 * wall,
+            "assembly_ms": float(np.median([t[1] for t in times])),
+            "pcg_ms": float(np.median([t[2] for t in times])), "iters_max": int(max(it)),
+            "iters_min": int(min(it)), "iters_sum": int(sum(it)),
+            "refinements_sum": int(sum(i.refinements for i in infos)),
+            "converged": bool(summ.converged), "worst_relres": summ.relres,
+            "worst_err_vs_ref": float(max(errs))}
+        log("batched %s: %.1f ms/step (pcg %.1f ms), iters %d..%d, worst err %.2e"
+            % (name, 1e3 * wall, per_mode[name]["pcg_ms"], min(it), max(it), max(errs)))
+    sel = per_mode[args.batch_mode]
+    out = {"metric": "batched P1 assemble + PCG, Melem/s", "value": sel["melem_s"],
+           "unit": "Melem/s", "pcg_mode": args.batch_mode,
+           "config": {"workload": "%d P1 problems, sides log-uniform in [32,512], "
+                                  "a=exp(N(0,1)), f=1" % nprob,
+                      "cells": int(ne), "dofs": int(nv), "nnz": int(plan.csr.info()[2])},
+           "ms_per_step": sel["ms_per_step"], "modes": per_mode,
+           "speedup_vs_fp64": {k: per_mode["fp64"]["ms_per_step"] / v["ms_per_step"]
+                               for k, v in per_mode.items()},
+           "tuned_rows_per_workgroup": summ.refinements}
     for hh in (batch, b, x, theta, plan, fe, mesh):
         hh.close()
     return out
@@ -418,6 +472,7 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint"], default="p1")
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
+    ap.add_argument("--batch-mode", choices=["fp64", "mixed", "mixed_matrix"], default="mixed")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")

@@ -142,6 +142,14 @@ int lfgpu_geometry(lfgpu_ctx *ctx, const lfgpu_mesh *m, lfgpu_vec *J,
 int lfgpu_element_matrices(lfgpu_ctx *ctx, const lfgpu_mesh *m,
                            const lfgpu_fe *fe, const lfgpu_coeff *alpha,
                            const lfgpu_coeff *gamma, lfgpu_vec *out);
+/* The same element matrices as lfgpu_element_matrices, computed as a
+ * contraction of per-cell coefficients with fixed reference matrices on the
+ * FP64 matrix cores (the B^T D B products of loc_comp_ellbvp.h:324-337
+ * regrouped).  Entries agree with lfgpu_element_matrices up to rounding,
+ * within 1e-12 of the row's largest entry, not bit for bit. */
+int lfgpu_element_matrices_mfma(lfgpu_ctx *ctx, const lfgpu_mesh *m,
+                                const lfgpu_fe *fe, const lfgpu_coeff *alpha,
+                                const lfgpu_coeff *gamma, lfgpu_vec *out);
 /* ScalarLoadElementVectorProvider::Eval for all cells
  * (uscalfe/loc_comp_ellbvp.h:691-746): out [ncells x nloc]. */
 int lfgpu_element_vectors(lfgpu_ctx *ctx, const lfgpu_mesh *m,
@@ -174,6 +182,17 @@ int lfgpu_assemble_galerkin(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
                             const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,
                             const lfgpu_coeff *f, const lfgpu_vec *fixed_vals,
                             lfgpu_vec *rhs);
+
+/* lfgpu_assemble_galerkin with the element matrices taken from
+ * lfgpu_element_matrices_mfma (kept in a plan-owned buffer of
+ * ncells x nloc^2 doubles, allocated on the first call) and gathered in the
+ * same triplet order.  Pattern and rhs (without prescribed nonzero values)
+ * are identical to lfgpu_assemble_galerkin; matrix values agree up to the
+ * element matrices' rounding. */
+int lfgpu_assemble_galerkin_mfma(lfgpu_ctx *ctx, lfgpu_plan *p,
+                                 const lfgpu_fe *fe, const lfgpu_coeff *alpha,
+                                 const lfgpu_coeff *gamma, const lfgpu_coeff *f,
+                                 const lfgpu_vec *fixed_vals, lfgpu_vec *rhs);
 
 /* ---- CSR matrices ------------------------------------------------------ */
 int lfgpu_csr_info(const lfgpu_csr *A, int32_t *nrows, int32_t *ncols,
@@ -252,7 +271,9 @@ int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
 int lfgpu_batch_destroy(lfgpu_batch *bt);
 /* infos: nprob entries (or NULL); summary: iters = max over problems,
  * converged = all converged, relres = worst, refinements = the tuned rows
- * per workgroup, ms = device time.  Only LFGPU_PCG_FP64 is supported. */
+ * per workgroup, ms = device time.  All three modes of lfgpu_pcg: in the
+ * mixed modes every problem runs its own reliable updates and stops on its
+ * FP64 true residual (infos[p].refinements counts them). */
 int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec *b,
                     lfgpu_vec *x, const lfgpu_pcg_opts *opts,
                     lfgpu_pcg_info *infos, lfgpu_pcg_info *summary);

@@ -69,12 +69,16 @@ SIGNATURES = {
     "lfgpu_fe_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lfgpu_geometry": (C.c_int, [_P, _P, _P, _P, _P]),
     "lfgpu_element_matrices": (C.c_int, [_P, _P, _P, C.POINTER(Coeff), C.POINTER(Coeff), _P]),
+    "lfgpu_element_matrices_mfma": (C.c_int, [_P, _P, _P, C.POINTER(Coeff), C.POINTER(Coeff),
+                                              _P]),
     "lfgpu_element_vectors": (C.c_int, [_P, _P, _P, C.POINTER(Coeff), _P]),
     "lfgpu_plan_create": (C.c_int, [_P, _P, _u8, C.POINTER(_P)]),
     "lfgpu_plan_destroy": (C.c_int, [_P]),
     "lfgpu_plan_csr": (C.c_int, [_P, C.POINTER(_P)]),
     "lfgpu_assemble_galerkin": (C.c_int, [_P, _P, _P, C.POINTER(Coeff), C.POINTER(Coeff),
                                           C.POINTER(Coeff), _P, _P]),
+    "lfgpu_assemble_galerkin_mfma": (C.c_int, [_P, _P, _P, C.POINTER(Coeff), C.POINTER(Coeff),
+                                               C.POINTER(Coeff), _P, _P]),
     "lfgpu_csr_info": (C.c_int, [_P, _i32, _i32, _i64]),
     "lfgpu_csr_download": (C.c_int, [_P, _i32, _i32, _d]),
     "lfgpu_coo_to_csr": (C.c_int, [_P, C.c_int64, _i32, _i32, _d, C.c_int32, C.c_int32,
@@ -276,12 +280,14 @@ class Plan:
         self.csr = Csr(ctx, ch, owned=False)
 
     def assemble(self, fe: FE, alpha: Coeff, gamma: Coeff, f: Coeff | None, rhs: Vec | None,
-                 fixed_vals: Vec | None = None):
-        check(lib().lfgpu_assemble_galerkin(self.ctx.h, self.h, fe.h, C.byref(alpha),
-                                            C.byref(gamma),
-                                            None if f is None else C.byref(f),
-                                            None if fixed_vals is None else fixed_vals.h,
-                                            None if rhs is None else rhs.h))
+                 fixed_vals: Vec | None = None, mfma: bool = False):
+        """lfgpu_assemble_galerkin, or lfgpu_assemble_galerkin_mfma with mfma=True.
+        Returns the device time in ms."""
+        fn = lib().lfgpu_assemble_galerkin_mfma if mfma else lib().lfgpu_assemble_galerkin
+        check(fn(self.ctx.h, self.h, fe.h, C.byref(alpha), C.byref(gamma),
+                 None if f is None else C.byref(f),
+                 None if fixed_vals is None else fixed_vals.h,
+                 None if rhs is None else rhs.h))
         return self.ctx.last_ms()
 
     def close(self):
@@ -290,10 +296,10 @@ class Plan:
             self.h = _P()
 
 
-def element_matrices(ctx, mesh: Mesh, fe: FE, alpha: Coeff, gamma: Coeff):
+def element_matrices(ctx, mesh: Mesh, fe: FE, alpha: Coeff, gamma: Coeff, mfma: bool = False):
     out = Vec(ctx, mesh.ne * mesh.nloc * mesh.nloc)
-    check(lib().lfgpu_element_matrices(ctx.h, mesh.h, fe.h, C.byref(alpha), C.byref(gamma),
-                                       out.h))
+    fn = lib().lfgpu_element_matrices_mfma if mfma else lib().lfgpu_element_matrices
+    check(fn(ctx.h, mesh.h, fe.h, C.byref(alpha), C.byref(gamma), out.h))
     a = out.numpy().reshape(mesh.ne, mesh.nloc, mesh.nloc)
     out.close()
     return a
@@ -361,8 +367,8 @@ class Batch:
                                        C.byref(self.h)))
 
     def pcg(self, b: Vec, x: Vec, rtol=1e-10, maxit=100000, check_every=64,
-            raise_on_noconv=True):
-        opts = PcgOpts(rtol, int(maxit), PCG_FP64, check_every, 0.0, 0)
+            raise_on_noconv=True, mode=PCG_FP64, refine_delta=0.0, refine_max_interval=0):
+        opts = PcgOpts(rtol, int(maxit), mode, check_every, refine_delta, refine_max_interval)
         infos = (PcgInfo * self.nprob)()
         summ = PcgInfo()
         rc = lib().lfgpu_batch_pcg(self.ctx.h, self.h, b.h, x.h, C.byref(opts), infos,

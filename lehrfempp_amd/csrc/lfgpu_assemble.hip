@@ -160,6 +160,8 @@ extern "C" int lfgpu_plan_destroy(lfgpu_plan *p) {
   dfree(p->rec_key);
   dfree(p->rec_meta);
   dfree(p->fixed);
+  dfree(p->ke);
+  dfree(p->fe);
   delete p;
   return LFGPU_OK;
 }
@@ -431,7 +433,142 @@ int launch_assemble(hipStream_t s, const lfgpu_plan *p, const AsmArgs &a) {
   return LFGPU_OK;
 }
 
+// Row-owner gather of precomputed element matrices and vectors: the same
+// record walk and summation order as k_assemble_rows, reading row i of cell
+// c from ke[(c nloc + i) nloc ...] instead of evaluating it.
+struct KeArgs {
+  const int32_t *cell_dofs;
+  const int32_t *rec_ptr;
+  const int32_t *rec_key;
+  const uint8_t *rec_meta;
+  const uint8_t *fixed;
+  const int32_t *row_ptr;
+  const double *ke;
+  const double *fvec;        // may be null
+  double *val;
+  double *rhs;               // may be null
+  const double *fixed_vals;  // may be null
+  int64_t nrows;
+};
+
+template <int NLOC, int MS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_assemble_rows_ke(KeArgs a) {
+  extern __shared__ double s_acc[];
+  const int64_t r0 = (int64_t)blockIdx.x * BLOCK;
+  const int64_t r1 = r0 + BLOCK < a.nrows ? r0 + BLOCK : a.nrows;
+  const int32_t base = a.row_ptr[r0];
+  const int32_t cnt = a.row_ptr[r1] - base;
+  for (int t = threadIdx.x; t < cnt; t += BLOCK) s_acc[t] = 0.0;
+  __syncthreads();
+  const int64_t r = r0 + threadIdx.x;
+  if (r < r1) {
+    const int32_t off = a.row_ptr[r] - base;
+    if (a.fixed[r]) {
+      s_acc[off] = 1.0;
+      if (a.rhs) a.rhs[r] = a.fixed_vals ? a.fixed_vals[r] : 0.0;
+    } else {
+      double b = 0.0;
+      const int32_t pb = a.rec_ptr[r], pe = a.rec_ptr[r + 1];
+      for (int32_t p = pb; p < pe; ++p) {
+        const int32_t key = a.rec_key[p];
+        uint8_t mt[MS];
+        load_meta<NLOC, MS>(a.rec_meta, p, mt);
+        const double *row = a.ke + (int64_t)key * NLOC;
+#pragma unroll
+        for (int j = 0; j < NLOC; ++j) {
+          const uint8_t sl = mt[1 + j];
+          if (sl != kDrop) s_acc[off + sl] += row[j];
+        }
+        if (a.fvec) b += a.fvec[key];
+      }
+      if (a.fixed_vals) {
+        for (int32_t p = pb; p < pe; ++p) {
+          const int32_t key = a.rec_key[p];
+          const int64_t cell = key / NLOC;
+          uint8_t mt[MS];
+          load_meta<NLOC, MS>(a.rec_meta, p, mt);
+          const double *row = a.ke + (int64_t)key * NLOC;
+#pragma unroll
+          for (int j = 0; j < NLOC; ++j)
+            if (mt[1 + j] == kDrop)
+              b += row[j] * (-1.0 * a.fixed_vals[a.cell_dofs[cell * NLOC + j]]);
+        }
+      }
+      if (a.rhs) a.rhs[r] = b;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < cnt; t += BLOCK) a.val[base + t] = s_acc[t];
+}
+
+template <int NLOC, int MS>
+int launch_assemble_ke(hipStream_t s, const lfgpu_plan *p, const KeArgs &a) {
+  const size_t lds = (size_t)std::max(p->max_block_nnz, 1) * sizeof(double);
+  const unsigned grid = grid_for(a.nrows, p->block);
+  switch (p->block) {
+    case 256:
+      hipLaunchKernelGGL((k_assemble_rows_ke<NLOC, MS, 256>), dim3(grid), dim3(256), lds, s, a);
+      break;
+    case 128:
+      hipLaunchKernelGGL((k_assemble_rows_ke<NLOC, MS, 128>), dim3(grid), dim3(128), lds, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_assemble_rows_ke<NLOC, MS, 64>), dim3(grid), dim3(64), lds, s, a);
+      break;
+  }
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
+}
+
 }  // namespace
+
+extern "C" int lfgpu_assemble_galerkin_mfma(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
+                                            const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,
+                                            const lfgpu_coeff *f, const lfgpu_vec *fixed_vals,
+                                            lfgpu_vec *rhs) {
+  LFG_REQUIRE(ctx && p && fe, LFGPU_EINVAL, "NULL argument to lfgpu_assemble_galerkin_mfma");
+  const lfgpu_mesh *m = p->mesh;
+  LFG_REQUIRE(fe->nloc == m->nloc, LFGPU_EINVAL,
+              "finite element and DOF layout disagree on the number of local DOFs");
+  LFG_REQUIRE(!rhs || rhs->n == m->ndofs, LFGPU_EINVAL, "rhs must have ndofs entries");
+  LFG_REQUIRE(!fixed_vals || fixed_vals->n == m->ndofs, LFGPU_EINVAL,
+              "fixed_vals must have ndofs entries");
+  CoeffD ca, cg, cf;
+  LFG_TRY(resolve_coeff(alpha, m->ncells, fe->nqp, false, &ca, "alpha"));
+  LFG_TRY(resolve_coeff(gamma, m->ncells, fe->nqp, false, &cg, "gamma"));
+  LFG_TRY(resolve_coeff(f, m->ncells, fe->nqp, true, &cf, "f"));
+  const int nloc = m->nloc;
+  if (!p->ke) LFG_TRY(dalloc(&p->ke, (size_t)std::max<int64_t>(1, m->ncells * nloc * nloc),
+                             "element matrices"));
+  if (f && !p->fe) LFG_TRY(dalloc(&p->fe, (size_t)std::max<int64_t>(1, m->ncells * nloc),
+                                  "element vectors"));
+  KeArgs a;
+  a.cell_dofs = m->cell_dofs;
+  a.rec_ptr = p->rec_ptr;
+  a.rec_key = p->rec_key;
+  a.rec_meta = p->rec_meta;
+  a.fixed = p->fixed;
+  a.row_ptr = p->csr.row_ptr;
+  a.ke = p->ke;
+  a.fvec = f ? p->fe : nullptr;
+  a.val = p->csr.val;
+  a.rhs = rhs ? rhs->d : nullptr;
+  a.fixed_vals = (fixed_vals && p->has_fixed) ? fixed_vals->d : nullptr;
+  a.nrows = m->ndofs;
+  LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+  LFG_TRY(element_contract(ctx->stream, m, fe, ca, cg, f ? &cf : nullptr, p->ke, p->fe));
+  if (nloc == 3)
+    LFG_TRY((launch_assemble_ke<3, 4>(ctx->stream, p, a)));
+  else
+    LFG_TRY((launch_assemble_ke<10, 12>(ctx->stream, p, a)));
+  LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  LFG_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  LFG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ms;
+  csr_values_changed(&p->csr);
+  return LFGPU_OK;
+}
 
 extern "C" int lfgpu_assemble_galerkin(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
                                        const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,

@@ -608,6 +608,7 @@ extern "C" int lfgpu_fe_destroy(lfgpu_fe *fe) {
   if (!fe) return LFGPU_OK;
   if (fe->ctx) (void)hipStreamSynchronize(fe->ctx->stream);
   dfree(fe->tab);
+  dfree(fe->rten);
   delete fe;
   return LFGPU_OK;
 }
@@ -654,19 +655,46 @@ extern "C" int lfgpu_fe_create(lfgpu_ctx *ctx, int degree, int nqp, const double
       dN[i + nloc * (2 * k + 1)] = gy[i];
     }
   }
+  // Reference tensors for the contraction form of the element matrix.  With
+  // B = (J^T J)^{-1} = M^T M, M = J^{-T}, the integrand of
+  // loc_comp_ellbvp.h:324-337 is
+  //   alpha (B00 dx_i dx_j + B11 dy_i dy_j + B01 (dx_i dy_j + dy_i dx_j))
+  //   + gamma N_i N_j,
+  // so K_e = sum over (point, term) of a cell coefficient times a fixed
+  // reference matrix.
+  const int ne = nloc * nloc, nep = (ne + 15) / 16 * 16;
+  std::vector<double> rten((size_t)(4 + 4 * q) * nep, 0.0);
+  for (int k = 0; k < q; ++k)
+    for (int i = 0; i < nloc; ++i)
+      for (int j = 0; j < nloc; ++j) {
+        const double w = W[k];
+        const double dxi = dN[i + nloc * (2 * k)], dyi = dN[i + nloc * (2 * k + 1)];
+        const double dxj = dN[j + nloc * (2 * k)], dyj = dN[j + nloc * (2 * k + 1)];
+        const double t[4] = {w * (dxi * dxj), w * (dyi * dyj), w * (dxi * dyj + dyi * dxj),
+                             w * (Nt[i + nloc * k] * Nt[j + nloc * k])};
+        for (int c = 0; c < 4; ++c) {
+          rten[(size_t)(4 + 4 * k + c) * nep + i * nloc + j] = t[c];
+          rten[(size_t)c * nep + i * nloc + j] += t[c];
+        }
+      }
   lfgpu_fe *fe = new lfgpu_fe();
   fe->ctx = ctx;
   fe->degree = degree;
   fe->nloc = nloc;
   fe->nqp = q;
   fe->tab_len = (int)tab.size();
+  fe->nep = nep;
   int rc = dalloc(&fe->tab, tab.size(), "FE tables");
+  if (rc == LFGPU_OK) rc = dalloc(&fe->rten, rten.size(), "reference tensors");
   if (rc != LFGPU_OK) {
-    delete fe;
+    lfgpu_fe_destroy(fe);
     return rc;
   }
   hipError_t e = hipMemcpyAsync(fe->tab, tab.data(), tab.size() * sizeof(double),
                                 hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(fe->rten, rten.data(), rten.size() * sizeof(double),
+                       hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     lfgpu_fe_destroy(fe);

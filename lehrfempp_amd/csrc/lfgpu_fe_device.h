@@ -189,4 +189,18 @@ __device__ __forceinline__ void coeff_qp(const CoeffD &c, int64_t cell, const Ce
   }
 }
 
+// The value coeff_qp<NQP> stores at point k, for kernels that spread the
+// points of one cell over several lanes.
+template <int NQP>
+__device__ __forceinline__ double coeff_one(const CoeffD &c, int64_t cell, const CellXY &p,
+                                            const double *tab, int k) {
+  if (c.kind == LFGPU_COEFF_CONST) return c.value;
+  if (c.kind == LFGPU_COEFF_CELL) return c.value * c.data[cell];
+  if (c.kind == LFGPU_COEFF_EXPCELL) return c.value * exp(c.data[cell]);
+  double x = 0.0, y = 0.0;
+  if (c.kind == LFGPU_COEFF_SINSIN)
+    tria_global(p.x0, p.y0, p.x1, p.y1, p.x2, p.y2, tab[NQP + 2 * k], tab[NQP + 2 * k + 1], x, y);
+  return coeff_at(c, cell, k, NQP, x, y);
+}
+
 }  // namespace lfgpu

@@ -93,6 +93,12 @@ struct lfgpu_fe {
   // device table: w[nqp] | pts[2 nqp] | N[nloc nqp] | dN[nloc 2 nqp]
   double *tab = nullptr;
   int tab_len = 0;
+  // reference tensors of the element matrix, [4 + 4 nqp][nep] row-major:
+  // rows 0..3 are the quadrature sums of the four terms (dx dx, dy dy,
+  // dx dy + dy dx, N N), rows 4 + 4k + t the weighted term t at point k;
+  // nep = nloc^2 rounded up to a multiple of 16
+  double *rten = nullptr;
+  int nep = 0;
 };
 
 struct lfgpu_solver_cache;  // lfgpu_solver.hip
@@ -122,9 +128,15 @@ struct lfgpu_plan {
   bool has_fixed = false;
   int block = 256;             // rows per workgroup of the assembly kernel
   int max_block_nnz = 0;       // LDS doubles needed by one workgroup
+  double *ke = nullptr;        // [ncells x nloc^2] element matrices (contraction path)
+  double *fe = nullptr;        // [ncells x nloc] element vectors (contraction path)
 };
 
 namespace lfgpu {
 void solver_cache_free(lfgpu_csr *A);
+// Element matrices (and, if has_f, element vectors) of every cell through the
+// reference-tensor contraction on the matrix cores (lfgpu_mfma.hip).
+int element_contract(hipStream_t s, const lfgpu_mesh *m, const lfgpu_fe *fe, const CoeffD &alpha,
+                     const CoeffD &gamma, const CoeffD *f, double *ke, double *fvec);
 inline void csr_values_changed(lfgpu_csr *A) { A->val_version++; }
 }  // namespace lfgpu

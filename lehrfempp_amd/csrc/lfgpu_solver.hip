@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "lfgpu_internal.h"
@@ -459,6 +460,17 @@ Sweep make_sweep(int32_t n) {
   return s;
 }
 
+// Row chunk of workgroup b of a G-workgroup sweep.  Workgroups are handed to
+// the 8 XCDs round-robin (b -> XCD b % 8), so chunk (b % 8) G/8 + b / 8 gives
+// every XCD a contiguous band of rows: the off-diagonal neighbours of a chunk
+// (rows +-(n+1) on a structured mesh) then live in chunks that the same XCD,
+// and so the same L2, processes at about the same time.  A bijection on
+// [0, G); the last G % 8 workgroups keep their own index.
+__device__ __forceinline__ int32_t sweep_chunk(int b, int G) {
+  const int per = G >> 3;
+  return b < 8 * per ? (b & 7) * per + (b >> 3) : b;
+}
+
 // Sum of the ngroups partials, identical in every workgroup, broadcast to all
 // threads of the workgroup.
 __device__ __forceinline__ double all_reduce(const double *part, int ngroups, double *sh) {
@@ -543,7 +555,7 @@ struct CgArgs {
 template <int FMT>
 __global__ __launch_bounds__(256) void k_cg_init(CgArgs a, const double *__restrict__ b) {
   __shared__ double sh[16];
-  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double bb = 0.0, rr = 0.0, rz = 0.0;
   for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
@@ -627,7 +639,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
     st->iter += 1;
     if (st->iter >= st->maxit) st->done = 1;
   }
-  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double pq = 0.0, qdr = 0.0, qdq = 0.0;
 #pragma unroll 2
@@ -661,7 +673,7 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
   const double alpha = rho / pq;
   const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
   const double beta = rho_next / rho;
-  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
 #pragma unroll 2
@@ -752,7 +764,7 @@ template <int FMT, class VT>
 __global__ __launch_bounds__(256) void k_mix_spmv(MixArgs<VT> a) {
   __shared__ double sh[8];
   if (a.st->done) return;
-  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double v = 0.0;
   for (int32_t r = r0 + threadIdx.x; r < r1; r += 256) {
@@ -769,7 +781,7 @@ __global__ __launch_bounds__(256) void k_mix_update(MixArgs<VT> a, int par) {
   __shared__ double sh[8];
   if (a.st->done) return;
   const VT alpha = (VT)(a.st->rho[par] / all_reduce(slot(a.part, M_PQ), a.sw.ngroups, sh));
-  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
   for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
@@ -802,7 +814,7 @@ __global__ __launch_bounds__(256) void k_mix_decide(MixArgs<VT> a, int par) {
     st->refine[par] = refine ? 1 : 0;
     if (!refine) st->rho[par ^ 1] = rz;
   }
-  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   if (!refine) {
     const VT fb = (VT)beta;
@@ -826,7 +838,7 @@ template <int FMT, class VT>
 __global__ __launch_bounds__(256) void k_mix_resid(MixArgs<VT> a, int par) {
   __shared__ double sh[8];
   if (a.st->done || !a.st->refine[par]) return;
-  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
   for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
@@ -866,7 +878,7 @@ __global__ __launch_bounds__(256) void k_mix_finish(MixArgs<VT> a, int par) {
   }
   if (stop) return;
   const VT fb = (VT)(rz / st->rho[par]);
-  const int32_t r0 = blockIdx.x * a.sw.rows_per_group;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
     const VT ri = (VT)a.r64[i];
@@ -1102,6 +1114,258 @@ void launch_bcg_chunk(hipStream_t s, const BArgs &a, int count) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched mixed precision: the reliable-update scheme of the single solve
+// (k_mix_*), run per problem on the workgroup table.  VT = float: FP32
+// matrix, recurrence vectors and increment; VT = double: FP32 matrix only.
+// FP64 true residuals, dot products and a double-double solution throughout.
+// ---------------------------------------------------------------------------
+struct BMState {
+  double thr, bb, rr, rr_ref;
+  double rho[2];
+  int iter, done, converged, maxit;
+  int iterp[2], refine[2];
+  int last_refine, refinements;
+};
+
+constexpr int MB_BB = 0, MB_PQ = 1, MB_RR = 2, MB_RZ = 3, MB_RR64 = 4, MB_RZ64 = 5;
+
+template <class VT>
+struct BMArgs {
+  MatView m;
+  const float *__restrict__ dinv32;
+  const double *__restrict__ dinv;
+  double *__restrict__ x;
+  double *__restrict__ xlo;
+  const double *__restrict__ b;
+  double *__restrict__ r64;
+  VT *__restrict__ r;
+  VT *__restrict__ p;
+  VT *__restrict__ q;
+  VT *__restrict__ xa;
+  double *part;  // [kPartSlots x ngroups]
+  int ngroups;
+  const BGroup *groups;
+  const BProb *probs;
+  BMState *st;
+  double delta2;
+  int max_interval;
+};
+
+template <class VT>
+__device__ __forceinline__ double *mslot(const BMArgs<VT> &a, int k) {
+  return a.part + (int64_t)k * a.ngroups;
+}
+
+// r64 = b - A x in FP64, the FP32 recurrence started from it
+template <int FMT, class VT>
+__global__ __launch_bounds__(256) void k_bmix_init(BMArgs<VT> a) {
+  __shared__ double sh[8];
+  const BGroup G = a.groups[blockIdx.x];
+  double bb = 0.0, rr = 0.0, rz = 0.0;
+  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+    const double bi = a.b[i];
+    const double ri = bi - apply64<FMT>(a.m, a.x, i, a.x[i]);
+    a.r64[i] = ri;
+    const VT rv = (VT)ri;
+    a.r[i] = rv;
+    a.p[i] = (VT)a.dinv32[i] * rv;
+    a.xa[i] = (VT)0;
+    a.xlo[i] = 0.0;
+    bb += bi * bi;
+    rr += ri * ri;
+    rz += ri * (a.dinv[i] * ri);
+  }
+  store_partial(mslot(a, MB_BB), bb, sh);
+  store_partial(mslot(a, MB_RR64), rr, sh);
+  store_partial(mslot(a, MB_RZ64), rz, sh);
+}
+
+// one workgroup per problem: Eigen's early exits as in k_cg_init_finish
+template <class VT>
+__global__ __launch_bounds__(256) void k_bmix_init_finish(BMArgs<VT> a) {
+  __shared__ double sh[8];
+  const BProb P = a.probs[blockIdx.x];
+  const double bb = seg_reduce(mslot(a, MB_BB), P.gb, P.ge, sh);
+  const double rr = seg_reduce(mslot(a, MB_RR64), P.gb, P.ge, sh);
+  const double rz = seg_reduce(mslot(a, MB_RZ64), P.gb, P.ge, sh);
+  if (threadIdx.x == 0) {
+    BMState *st = a.st + blockIdx.x;
+    st->bb = bb;
+    st->rr = rr;
+    st->rr_ref = rr;
+    st->rho[0] = st->rho[1] = rz;
+    st->iter = -1;
+    st->iterp[0] = st->iterp[1] = 0;
+    st->refine[0] = st->refine[1] = 0;
+    st->converged = 0;
+    st->done = 0;
+    st->refinements = 0;
+    st->last_refine = 0;
+    const double thr = st->thr * bb;  // thr holds tol^2 on entry
+    st->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+    if (bb == 0.0 || rr < st->thr) {
+      st->done = 1;
+      st->converged = 1;
+      st->iter = 0;
+    } else if (st->maxit <= 0) {
+      st->done = 1;
+      st->iter = 0;
+    }
+  }
+}
+
+template <class VT>
+__global__ void k_bmix_zero_if_bb0(BMArgs<VT> a) {
+  const BGroup G = a.groups[blockIdx.x];
+  if (a.st[G.prob].bb != 0.0) return;
+  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) a.x[i] = 0.0;
+}
+
+template <int FMT, class VT>
+__global__ __launch_bounds__(256) void k_bmix_spmv(BMArgs<VT> a) {
+  __shared__ double sh[8];
+  const BGroup G = a.groups[blockIdx.x];
+  if (a.st[G.prob].done) return;
+  double v = 0.0;
+  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+    const VT pi = a.p[i];
+    const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
+    a.q[i] = qi;
+    v += (double)pi * (double)qi;
+  }
+  store_partial(mslot(a, MB_PQ), v, sh);
+}
+
+template <class VT>
+__global__ __launch_bounds__(256) void k_bmix_update(BMArgs<VT> a, int par) {
+  __shared__ double sh[8];
+  const BGroup G = a.groups[blockIdx.x];
+  const BMState *st = a.st + G.prob;
+  if (st->done) return;
+  const BProb P = a.probs[G.prob];
+  const VT alpha = (VT)(st->rho[par] / seg_reduce(mslot(a, MB_PQ), P.gb, P.ge, sh));
+  double rr = 0.0, rz = 0.0;
+  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+    a.xa[i] += alpha * a.p[i];
+    const VT ri = a.r[i] - alpha * a.q[i];
+    a.r[i] = ri;
+    rr += (double)ri * (double)ri;
+    rz += (double)ri * (double)((VT)a.dinv32[i] * ri);
+  }
+  store_partial(mslot(a, MB_RR), rr, sh);
+  store_partial(mslot(a, MB_RZ), rz, sh);
+}
+
+template <class VT>
+__global__ __launch_bounds__(256) void k_bmix_decide(BMArgs<VT> a, int par) {
+  __shared__ double sh[8];
+  const BGroup G = a.groups[blockIdx.x];
+  BMState *st = a.st + G.prob;
+  if (st->done) return;
+  const BProb P = a.probs[G.prob];
+  const bool writer = blockIdx.x == (unsigned)G.gb && threadIdx.x == 0;
+  const double rr = seg_reduce(mslot(a, MB_RR), P.gb, P.ge, sh);
+  const double rz = seg_reduce(mslot(a, MB_RZ), P.gb, P.ge, sh);
+  const int it = st->iterp[par] + 1;
+  const bool refine = rr < st->thr || rr < a.delta2 * st->rr_ref ||
+                      it - st->last_refine >= a.max_interval || it >= st->maxit;
+  const double beta = rz / st->rho[par];
+  if (writer) {
+    st->iterp[par ^ 1] = it;
+    st->iter = it;
+    st->refine[par] = refine ? 1 : 0;
+    if (!refine) st->rho[par ^ 1] = rz;
+  }
+  if (!refine) {
+    const VT fb = (VT)beta;
+    for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256)
+      a.p[i] = (VT)a.dinv32[i] * a.r[i] + fb * a.p[i];
+  } else {
+    for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+      const double hi = a.x[i], d = (double)a.xa[i];
+      const double s = hi + d;
+      const double bp = s - hi;
+      const double err = (hi - (s - bp)) + (d - bp);
+      a.x[i] = s;
+      a.xlo[i] += err;
+      a.xa[i] = (VT)0;
+    }
+  }
+}
+
+template <int FMT, class VT>
+__global__ __launch_bounds__(256) void k_bmix_resid(BMArgs<VT> a, int par) {
+  __shared__ double sh[8];
+  const BGroup G = a.groups[blockIdx.x];
+  const BMState *st = a.st + G.prob;
+  if (st->done || !st->refine[par]) return;
+  double rr = 0.0, rz = 0.0;
+  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+    const double ah = apply64<FMT>(a.m, a.x, i, a.x[i]);
+    const double al = apply64<FMT>(a.m, a.xlo, i, a.xlo[i]);
+    const double ri = (a.b[i] - ah) - al;
+    a.r64[i] = ri;
+    rr += ri * ri;
+    rz += ri * (a.dinv[i] * ri);
+  }
+  store_partial(mslot(a, MB_RR64), rr, sh);
+  store_partial(mslot(a, MB_RZ64), rz, sh);
+}
+
+template <class VT>
+__global__ __launch_bounds__(256) void k_bmix_finish(BMArgs<VT> a, int par) {
+  __shared__ double sh[8];
+  const BGroup G = a.groups[blockIdx.x];
+  BMState *st = a.st + G.prob;
+  if (st->done || !st->refine[par]) return;
+  const BProb P = a.probs[G.prob];
+  const bool writer = blockIdx.x == (unsigned)G.gb && threadIdx.x == 0;
+  const double rt = seg_reduce(mslot(a, MB_RR64), P.gb, P.ge, sh);
+  const double rz = seg_reduce(mslot(a, MB_RZ64), P.gb, P.ge, sh);
+  const bool conv = rt < st->thr;
+  const bool stop = conv || st->iterp[par ^ 1] >= st->maxit;
+  const double rho = st->rho[par];
+  if (writer) {
+    st->rr = rt;
+    st->refinements += 1;
+    st->last_refine = st->iterp[par ^ 1];
+    if (stop) {
+      st->done = 1;
+      st->converged = conv ? 1 : 0;
+    } else {
+      st->rr_ref = rt;
+      st->rho[par ^ 1] = rz;
+    }
+  }
+  if (stop) return;
+  const VT fb = (VT)(rz / rho);
+  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+    const VT ri = (VT)a.r64[i];
+    a.r[i] = ri;
+    a.p[i] = (VT)a.dinv32[i] * ri + fb * a.p[i];
+  }
+}
+
+template <class VT>
+__global__ void k_bmix_final(BMArgs<VT> a) {
+  const BGroup G = a.groups[blockIdx.x];
+  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) a.x[i] += a.xlo[i];
+}
+
+template <int FMT, class VT>
+void launch_bmix_chunk(hipStream_t s, const BMArgs<VT> &a, int count) {
+  const dim3 G(a.ngroups), B(256);
+  for (int k = 0; k < count; ++k) {
+    const int par = k & 1;
+    hipLaunchKernelGGL((k_bmix_spmv<FMT, VT>), G, B, 0, s, a);
+    hipLaunchKernelGGL(k_bmix_update<VT>, G, B, 0, s, a, par);
+    hipLaunchKernelGGL(k_bmix_decide<VT>, G, B, 0, s, a, par);
+    hipLaunchKernelGGL((k_bmix_resid<FMT, VT>), G, B, 0, s, a, par);
+    hipLaunchKernelGGL(k_bmix_finish<VT>, G, B, 0, s, a, par);
+  }
+}
+
 }  // namespace
 
 struct lfgpu_batch {
@@ -1116,10 +1380,16 @@ struct lfgpu_batch {
   BState *st = nullptr;
   double *part = nullptr;
   BState *h_st = nullptr;  // pinned
+  BMState *mst = nullptr;    // mixed-precision states (allocated on first use)
+  BMState *h_mst = nullptr;  // pinned
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_count = 0;
+  int graph_mode = -1;
   const double *graph_x = nullptr;
+  const double *graph_b = nullptr;
+  double graph_delta2 = 0.0;
+  int graph_max_interval = 0;
   double tune_ms[4] = {0, 0, 0, 0};
 };
 
@@ -1235,6 +1505,104 @@ int batch_tune(lfgpu_batch *bt, const double *b) {
   dfree(xs);
   LFG_TRY(rc);
   return batch_layout(bt, bestR);
+}
+
+template <class VT>
+BMArgs<VT> batch_mixed_args(lfgpu_batch *bt, const double *b, double *x, double delta2,
+                            int max_interval) {
+  lfgpu_solver_cache *c = bt->A->cache;
+  BMArgs<VT> a;
+  a.m = view_of(c, bt->A->nrows);
+  a.dinv32 = c->dinv32;
+  a.dinv = c->dinv;
+  a.x = x;
+  a.xlo = c->xlo;
+  a.b = b;
+  a.r64 = c->r64;
+  if constexpr (std::is_same<VT, float>::value) {
+    a.r = c->r32;
+    a.p = c->p32;
+    a.q = c->q32;
+    a.xa = c->xa32;
+  } else {
+    a.r = c->r;
+    a.p = c->p;
+    a.q = c->q;
+    a.xa = c->xa64;
+  }
+  a.part = bt->part;
+  a.ngroups = bt->ngroups;
+  a.groups = bt->groups;
+  a.probs = bt->probs;
+  a.st = bt->mst;
+  a.delta2 = delta2;
+  a.max_interval = max_interval;
+  return a;
+}
+
+template <class VT>
+void batch_mixed_chunk(lfgpu_batch *bt, const BMArgs<VT> &a, int count) {
+  if (bt->A->cache->fmt == FMT_DIA)
+    launch_bmix_chunk<FMT_DIA, VT>(bt->ctx->stream, a, count);
+  else
+    launch_bmix_chunk<FMT_ELL, VT>(bt->ctx->stream, a, count);
+}
+
+// Mixed-precision batch solve: set-up, graph-replayed chunks until every
+// problem has stopped, then x += xlo.
+template <class VT>
+int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg_opts *opts,
+                    int check, int mode) {
+  hipStream_t s = bt->ctx->stream;
+  const double delta2 =
+      opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
+  const int max_interval = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
+  for (int p = 0; p < bt->nprob; ++p) {
+    bt->h_mst[p] = BMState{};
+    bt->h_mst[p].thr = opts->rtol * opts->rtol;
+    bt->h_mst[p].maxit = opts->maxit;
+  }
+  LFG_HIP(hipMemcpyAsync(bt->mst, bt->h_mst, bt->nprob * sizeof(BMState), hipMemcpyHostToDevice,
+                         s));
+  const BMArgs<VT> a = batch_mixed_args<VT>(bt, b, x, delta2, max_interval);
+  const dim3 G(a.ngroups), B(256);
+  if (bt->A->cache->fmt == FMT_DIA)
+    hipLaunchKernelGGL((k_bmix_init<FMT_DIA, VT>), G, B, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_bmix_init<FMT_ELL, VT>), G, B, 0, s, a);
+  hipLaunchKernelGGL(k_bmix_init_finish<VT>, dim3(bt->nprob), B, 0, s, a);
+  hipLaunchKernelGGL(k_bmix_zero_if_bb0<VT>, G, B, 0, s, a);
+  LFG_HIP(hipGetLastError());
+  // the captured arguments depend on the layout, x, b and the refinement
+  // options; recapture when any of them changes
+  if (!bt->exec || bt->graph_count != check || bt->graph_x != x || bt->graph_b != b ||
+      bt->graph_mode != mode || bt->graph_delta2 != delta2 ||
+      bt->graph_max_interval != max_interval) {
+    batch_release_graph(bt);
+    LFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    batch_mixed_chunk<VT>(bt, a, check);
+    hipError_t e = hipStreamEndCapture(s, &bt->graph);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    LFG_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+    bt->graph_count = check;
+    bt->graph_x = x;
+    bt->graph_b = b;
+    bt->graph_mode = mode;
+    bt->graph_delta2 = delta2;
+    bt->graph_max_interval = max_interval;
+  }
+  for (;;) {
+    LFG_HIP(hipGraphLaunch(bt->exec, s));
+    LFG_HIP(hipMemcpyAsync(bt->h_mst, bt->mst, bt->nprob * sizeof(BMState),
+                           hipMemcpyDeviceToHost, s));
+    LFG_HIP(hipStreamSynchronize(s));
+    bool all = true;
+    for (int p = 0; p < bt->nprob && all; ++p) all = bt->h_mst[p].done != 0;
+    if (all) break;
+  }
+  hipLaunchKernelGGL(k_bmix_final<VT>, G, B, 0, s, a);
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
 }
 
 }  // namespace
@@ -1404,7 +1772,9 @@ extern "C" int lfgpu_batch_destroy(lfgpu_batch *bt) {
   dfree(bt->probs);
   dfree(bt->part);
   dfree(bt->st);
+  dfree(bt->mst);
   if (bt->h_st) (void)hipHostFree(bt->h_st);
+  if (bt->h_mst) (void)hipHostFree(bt->h_mst);
   delete bt;
   return LFGPU_OK;
 }
@@ -1415,36 +1785,52 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
   LFG_REQUIRE(ctx && bt && b && x && opts, LFGPU_EINVAL, "NULL argument to lfgpu_batch_pcg");
   LFG_REQUIRE(b->n == bt->A->nrows && x->n == bt->A->nrows && b->d != x->d, LFGPU_EINVAL,
               "batch PCG size mismatch");
-  LFG_REQUIRE(opts->mode == LFGPU_PCG_FP64, LFGPU_EINVAL, "the batched solver runs in FP64");
+  LFG_REQUIRE(opts->mode == LFGPU_PCG_FP64 || opts->mode == LFGPU_PCG_MIXED ||
+                  opts->mode == LFGPU_PCG_MIXED_MATRIX,
+              LFGPU_EINVAL, "unknown PCG mode");
   LFG_REQUIRE(opts->rtol >= 0.0 && opts->maxit >= 0, LFGPU_EINVAL, "bad PCG options");
-  LFG_TRY(ensure_cache(ctx, bt->A, false));
+  const bool mixed = opts->mode != LFGPU_PCG_FP64;
+  LFG_TRY(ensure_cache(ctx, bt->A, mixed));
   hipStream_t s = ctx->stream;
   if (bt->rows_per_group == 0) LFG_TRY(batch_tune(bt, b->d));
+  if (mixed && !bt->mst) {
+    LFG_TRY(dalloc(&bt->mst, (size_t)bt->nprob, "batch mixed states"));
+    LFG_HIP(hipHostMalloc(&bt->h_mst, bt->nprob * sizeof(BMState), hipHostMallocDefault));
+  }
   int check = opts->check_every > 0 ? opts->check_every : 64;
   check += check & 1;
   LFG_HIP(hipEventRecord(ctx->ev0, s));
-  LFG_TRY(batch_init(bt, b->d, x->d, opts->rtol, opts->maxit));
-  const BArgs a = batch_args(bt, x->d);
-  // One chunk of iterations is captured into a graph once and replayed until
-  // every problem has stopped; the captured kernel arguments depend on the
-  // batch layout and on x only.
-  if (!bt->exec || bt->graph_count != check || bt->graph_x != x->d) {
-    batch_release_graph(bt);
-    LFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    batch_chunk(bt, a, check);
-    hipError_t e = hipStreamEndCapture(s, &bt->graph);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    LFG_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
-    bt->graph_count = check;
-    bt->graph_x = x->d;
-  }
-  for (;;) {
-    LFG_HIP(hipGraphLaunch(bt->exec, s));
-    LFG_HIP(hipMemcpyAsync(bt->h_st, bt->st, bt->nprob * sizeof(BState), hipMemcpyDeviceToHost, s));
-    LFG_HIP(hipStreamSynchronize(s));
-    bool all = true;
-    for (int p = 0; p < bt->nprob && all; ++p) all = bt->h_st[p].done != 0;
-    if (all) break;
+  if (opts->mode == LFGPU_PCG_MIXED) {
+    LFG_TRY(batch_run_mixed<float>(bt, b->d, x->d, opts, check, opts->mode));
+  } else if (opts->mode == LFGPU_PCG_MIXED_MATRIX) {
+    LFG_TRY(batch_run_mixed<double>(bt, b->d, x->d, opts, check, opts->mode));
+  } else {
+    LFG_TRY(batch_init(bt, b->d, x->d, opts->rtol, opts->maxit));
+    const BArgs a = batch_args(bt, x->d);
+    // One chunk of iterations is captured into a graph once and replayed
+    // until every problem has stopped; the captured kernel arguments depend
+    // on the batch layout and on x only.
+    if (!bt->exec || bt->graph_count != check || bt->graph_x != x->d ||
+        bt->graph_mode != LFGPU_PCG_FP64) {
+      batch_release_graph(bt);
+      LFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      batch_chunk(bt, a, check);
+      hipError_t e = hipStreamEndCapture(s, &bt->graph);
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      LFG_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
+      bt->graph_count = check;
+      bt->graph_x = x->d;
+      bt->graph_mode = LFGPU_PCG_FP64;
+    }
+    for (;;) {
+      LFG_HIP(hipGraphLaunch(bt->exec, s));
+      LFG_HIP(hipMemcpyAsync(bt->h_st, bt->st, bt->nprob * sizeof(BState),
+                             hipMemcpyDeviceToHost, s));
+      LFG_HIP(hipStreamSynchronize(s));
+      bool all = true;
+      for (int p = 0; p < bt->nprob && all; ++p) all = bt->h_st[p].done != 0;
+      if (all) break;
+    }
   }
   LFG_HIP(hipEventRecord(ctx->ev1, s));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
@@ -1454,17 +1840,25 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
   int conv = 1, maxit_seen = 0;
   double worst = 0.0;
   for (int p = 0; p < bt->nprob; ++p) {
-    const BState &h = bt->h_st[p];
-    const double rel = h.bb > 0.0 ? std::sqrt(h.rr / h.bb) : 0.0;
+    double bb, rr;
+    int iter, converged, refinements = 0;
+    if (mixed) {
+      const BMState &h = bt->h_mst[p];
+      bb = h.bb, rr = h.rr, iter = h.iter, converged = h.converged, refinements = h.refinements;
+    } else {
+      const BState &h = bt->h_st[p];
+      bb = h.bb, rr = h.rr, iter = h.iter, converged = h.converged;
+    }
+    const double rel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
     if (infos) {
-      infos[p].iters = h.iter < 0 ? 0 : h.iter;
-      infos[p].converged = h.converged;
-      infos[p].refinements = 0;
+      infos[p].iters = iter < 0 ? 0 : iter;
+      infos[p].converged = converged;
+      infos[p].refinements = refinements;
       infos[p].relres = rel;
       infos[p].ms = ms;
     }
-    conv &= h.converged;
-    maxit_seen = std::max(maxit_seen, h.iter);
+    conv &= converged;
+    maxit_seen = std::max(maxit_seen, iter);
     worst = std::max(worst, rel);
   }
   if (summary) {

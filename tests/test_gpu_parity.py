@@ -434,6 +434,54 @@ def test_batched_pcg_matches_per_problem_solves(ctx):
         h.close()
 
 
+@pytest.mark.parametrize("mode", [capi.PCG_MIXED, capi.PCG_MIXED_MATRIX])
+def test_batched_mixed_pcg_matches_direct_solves(ctx, mode):
+    # per-problem reliable updates: each problem stops on its own FP64 true
+    # residual; FP64 mode on the same batch in between must still work
+    sizes = [9, 17, 5, 31, 12, 24, 40, 8, 3]
+    xy, cells, flags, voff, coff = capi.batched_tp_mesh(sizes)
+    rng = np.random.default_rng(22)
+    alpha = np.exp(rng.standard_normal(cells.shape[0]))
+    g = capi.Mesh(ctx, xy, cells)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, flags)
+    va = capi.Vec(ctx, alpha)
+    b = capi.Vec(ctx, g.ndofs)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 1.0),
+                  capi.coeff(capi.COEFF_CONST, 1.0), b)
+    bh = b.numpy()
+    # a zero right-hand side for one problem: Eigen returns x = 0 at once
+    bh[voff[2]:voff[3]] = 0.0
+    b.upload(bh)
+    batch = capi.Batch(ctx, plan.csr, voff)
+    x = capi.Vec(ctx, g.ndofs)
+    rp, col, val = plan.csr.download()
+    for rep, md in enumerate((mode, capi.PCG_FP64, mode)):
+        x.upload(rng.standard_normal(g.ndofs))  # nonzero initial guesses
+        summ, infos = batch.pcg(b, x, rtol=1e-10, mode=md)
+        assert summ.converged, rep
+        xg = x.numpy()
+        for k in range(len(sizes)):
+            lo, hi = int(voff[k]), int(voff[k + 1])
+            sub = slice(int(rp[lo]), int(rp[hi]))
+            rpk = np.ascontiguousarray(rp[lo:hi + 1] - rp[lo], np.int32)
+            colk = np.ascontiguousarray(col[sub] - lo, np.int32)
+            valk = np.ascontiguousarray(val[sub])
+            bk = bh[lo:hi]
+            if not bk.any():
+                assert not xg[lo:hi].any() and infos[k].iters == 0
+                continue
+            xd = L.band_cholesky_solve(rpk, colk, valk, bk)
+            assert np.linalg.norm(xg[lo:hi] - xd) <= 1e-8 * np.linalg.norm(xd), (rep, k)
+            # the stopping test is on the true residual in the mixed modes
+            r = bk - L.csr_spmv(rpk, colk, valk, xg[lo:hi])
+            if md != capi.PCG_FP64:
+                assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(bk), (rep, k)
+                assert infos[k].refinements >= 1
+    for h in (x, batch, b, va, plan, fe, g):
+        h.close()
+
+
 # ---------------------------------------------- adjoint gradient end to end
 def test_adjoint_gradient_against_finite_differences(ctx):
     # J(theta) = 1/2 (u - u_obs)^T M (u - u_obs), A(exp(theta)) u = f;
@@ -573,4 +621,97 @@ def test_invalid_inputs_are_rejected(ctx):
         capi.element_matrices(ctx, g, fe1, capi.coeff(capi.COEFF_CELL, 1.0, short), capi.coeff())
     assert e.value.code == capi.EINVAL
     for h in (short, fe1, fe3, g):
+        h.close()
+
+
+# ------------------------------------- matrix-core (MFMA) contraction path
+def _row_rel(K, Ko):
+    """max |K - Ko| / max |Ko row| over the rows of every element matrix."""
+    scale = np.maximum(np.abs(Ko).max(axis=-1, keepdims=True), 1e-300)
+    return float((np.abs(K - Ko) / scale).max())
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+@pytest.mark.parametrize("kind", ["const", "expcell", "qp", "sinsin"])
+def test_mfma_element_matrices_match_oracle(ctx, degree, kind):
+    # lfgpu_element_matrices_mfma regroups loc_comp_ellbvp.h:324-337 into
+    # coefficient x reference-matrix products; "const"/"expcell" take the
+    # folded four-term contraction, "qp"/"sinsin" the per-point one
+    m = jittered_mesh(12 if degree == 3 else 24, seed=5)
+    g = gpu_mesh(ctx, m, degree)
+    fe = capi.FE(ctx, degree)
+    rng = np.random.default_rng(21)
+    q = fe.nqp
+    vecs = []
+    if kind == "const":
+        a_h, c_h = 2.3, 0.7
+        ca, cg = capi.coeff(capi.COEFF_CONST, a_h), capi.coeff(capi.COEFF_CONST, c_h)
+    elif kind == "expcell":
+        th = rng.standard_normal(m.ne)
+        va = capi.Vec(ctx, th)
+        vecs.append(va)
+        ca, cg = capi.coeff(capi.COEFF_EXPCELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 1.0)
+        a_h, c_h = np.exp(th), 1.0
+    elif kind == "qp":
+        a_h = rng.uniform(0.5, 2, (m.ne, q))
+        c_h = rng.uniform(0, 1, (m.ne, q))
+        va, vc = capi.Vec(ctx, a_h), capi.Vec(ctx, c_h)
+        vecs += [va, vc]
+        ca, cg = capi.coeff(capi.COEFF_QP, 1.0, va), capi.coeff(capi.COEFF_QP, 1.0, vc)
+    else:
+        ca, cg = capi.coeff(capi.COEFF_SINSIN, 1.0), capi.coeff(capi.COEFF_CONST, 0.25)
+        P = L.qp_coords(m, degree)
+        a_h, c_h = np.sin(np.pi * P[..., 0]) * np.sin(np.pi * P[..., 1]), 0.25
+    Km = capi.element_matrices(ctx, g, fe, ca, cg, mfma=True)
+    Ke = capi.element_matrices(ctx, g, fe, ca, cg)
+    Ko = L.element_matrices(m, degree, a_h, c_h)
+    tol = 1e-12  # relative to each element-matrix row's largest entry
+    assert _row_rel(Km, Ke) <= tol, _row_rel(Km, Ke)
+    if kind != "sinsin":  # device sin vs glibc sin differ in the last bit
+        assert np.array_equal(Ke, Ko)
+    assert _row_rel(Km, Ko) <= tol
+    for v in vecs:
+        v.close()
+    fe.close()
+    g.close()
+
+
+@pytest.mark.parametrize("degree,n", [(1, 50), (3, 11)])
+def test_mfma_assembly_matches_exact_assembly(ctx, degree, n):
+    m = jittered_mesh(n, seed=17)
+    cd, nd = m.dofs(degree)
+    flags = m.dirichlet_flags(degree)
+    g = gpu_mesh(ctx, m, degree)
+    fe = capi.FE(ctx, degree)
+    plan = capi.Plan(ctx, g, flags)
+    th = capi.Vec(ctx, np.random.default_rng(n).standard_normal(m.ne))
+    args = (fe, capi.coeff(capi.COEFF_EXPCELL, 1.0, th), capi.coeff(capi.COEFF_CONST, 1.0),
+            capi.coeff(capi.COEFF_SINSIN, 1.0))
+    b1, b2 = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
+    plan.assemble(*args, b1)
+    rp, col, v1 = plan.csr.download()
+    plan.assemble(*args, b2, mfma=True)
+    rp2, col2, v2 = plan.csr.download()
+    assert np.array_equal(rp, rp2) and np.array_equal(col, col2)
+    assert row_scaled_err(rp, v2, v1) <= 1e-12
+    # the element vectors follow the reference order in both paths
+    assert np.array_equal(b1.numpy(), b2.numpy())
+    # and the oracle's matrix, pattern and rhs
+    alpha = np.exp(th.numpy())
+    rpo, colo, valo, bo = L.galerkin_system(m, degree, alpha, 1.0, L.F_SINSIN, 1.0, flags=flags)
+    assert np.array_equal(rp2, rpo) and np.array_equal(col2, colo)
+    assert row_scaled_err(rp, v2, valo) <= 1e-12
+    assert np.abs(b2.numpy() - bo).max() <= 8e-15 * np.abs(bo).max()
+    # prescribed boundary values go through the same element matrices
+    gv = capi.Vec(ctx, np.random.default_rng(2).standard_normal(nd) * flags)
+    plan.assemble(*args, b1, gv)
+    plan.assemble(*args, b2, gv, mfma=True)
+    assert np.abs(b1.numpy() - b2.numpy()).max() <= 1e-12 * np.abs(b1.numpy()).max()
+    # the solve on the contracted matrix agrees with the solve on the exact one
+    x = capi.Vec(ctx, nd)
+    info = capi.pcg(ctx, plan.csr, b2, x, rtol=1e-12)
+    assert info.converged
+    xo = L.band_cholesky_solve(rpo, colo, valo, b2.numpy())
+    assert np.abs(x.numpy() - xo).max() <= 1e-8 * np.abs(xo).max()
+    for h in (gv, x, b1, b2, th, plan, fe, g):
         h.close()
