@@ -354,10 +354,14 @@ def run_p3(args, capi, ctx, n=512, seed=0):
     rp, _, v_exact = plan.csr.download()
     b_exact = b.numpy()
     # reference solution on the matrix that equals the oracle's bit for bit
-    x.fill(0.0)
-    capi.pcg(ctx, plan.csr, b, x, rtol=1e-13, mode=capi.PCG_MIXED_MATRIX, check_every=128,
-             maxit=MAXIT)
-    xr = x.numpy()
+    xr = None
+    if not args.no_solve:
+        x.fill(0.0)
+        ref = capi.pcg(ctx, plan.csr, b, x, rtol=1e-12, mode=capi.PCG_MIXED_MATRIX,
+                       check_every=128, maxit=MAXIT, raise_on_noconv=False)
+        xr = x.numpy()
+        log("p3 reference solve: %d iters, true relres %.2e"
+            % (ref.iters, true_relres(capi, ctx, plan.csr, x, b)))
     # element matrices on the matrix cores; the solves below use this matrix
     plan.assemble(*args_c, mfma=True)
     asm_mfma = [plan.assemble(*args_c, mfma=True) for _ in range(3)]
@@ -368,8 +372,12 @@ def run_p3(args, capi, ctx, n=512, seed=0):
     log("p3: assembly (MFMA contraction) %.3f ms, matrix row-scaled diff %.2e, rhs equal %s"
         % (np.median(asm_mfma), mat_err, rhs_equal))
     res = {}
-    for name, md in (("fp64", capi.PCG_FP64), ("mixed_matrix", capi.PCG_MIXED_MATRIX),
-                     ("mixed", capi.PCG_MIXED)):
+    if args.no_solve:
+        modes_p3 = ()
+    else:
+        modes_p3 = (("fp64", capi.PCG_FP64), ("mixed_matrix", capi.PCG_MIXED_MATRIX),
+                    ("mixed", capi.PCG_MIXED))
+    for name, md in modes_p3:
         x.fill(0.0)
         inf = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, mode=md, check_every=128, maxit=MAXIT,
                        raise_on_noconv=False)
@@ -391,7 +399,8 @@ def run_p3(args, capi, ctx, n=512, seed=0):
         "mfma_matrix_row_scaled_diff": mat_err, "mfma_rhs_bitwise_equal": rhs_equal,
         "pcg_matrix": "MFMA-contracted; err_vs_ref against the exact-order matrix's solution",
         "pcg": res,
-        "mixed_speedup_vs_fp64": {k: res["fp64"]["s"] / v["s"] for k, v in res.items()}}
+        "mixed_speedup_vs_fp64": {k: res["fp64"]["s"] / v["s"] for k, v in res.items()}
+        if "fp64" in res else None}
     for hh in (b, x, theta, plan, fe, mesh):
         hh.close()
     return out
@@ -464,25 +473,52 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
     return out
 
 
+def run_traffic(args, capi, ctx):
+    """A short fixed sequence for HBM byte counters (rocprofv3 --pmc): the
+    FP64 SpMV of the 2048x2048 system, a few PCG iterations and a plain copy
+    kernel of known size.  Prints the dispatch list with algorithmic bytes."""
+    P = Problem(capi, ctx, args.n, args.seed)
+    P.assemble()
+    xs, ys = capi.Vec(ctx, np.random.default_rng(1).standard_normal(P.nv)), capi.Vec(ctx, P.nv)
+    capi.spmv_timed(ctx, P.plan.csr, xs, ys, 3)
+    P.x.fill(0.0)
+    capi.pcg(ctx, P.plan.csr, P.b, P.x, rtol=1e-10, maxit=4, check_every=2,
+             raise_on_noconv=False)
+    ncopy = 1 << 27  # 1 GiB of doubles
+    c0, c1 = capi.Vec(ctx, ncopy), capi.Vec(ctx, ncopy)
+    c0.fill(1.0)
+    for _ in range(3):
+        capi.copy(ctx, c0, c1)
+    ctx.sync()
+    out = {"config": "traffic", "n": args.n, "dofs": P.nv, "nnz": P.nnz,
+           "spmv_dispatches": 3, "spmv_algorithmic_bytes": spmv_bytes(P.nnz, P.nv),
+           "copy_dispatches": 3, "copy_bytes_read": 8 * ncopy, "copy_bytes_written": 8 * ncopy}
+    for h in (c0, c1, xs, ys):
+        h.close()
+    P.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint"], default="p1")
+    ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint", "traffic"],
+                    default="p1")
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
     ap.add_argument("--batch-mode", choices=["fp64", "mixed", "mixed_matrix"], default="mixed")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="p3: assembly only")
     ap.add_argument("--fd-dirs", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(DESIGN.md describes how the path would shard); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from lehrfempp_amd import capi
@@ -493,6 +529,8 @@ def main():
         out = run_p3(args, capi, ctx)
     elif args.config == "batched":
         out = run_batched(args, capi, ctx)
+    elif args.config == "traffic":
+        out = run_traffic(args, capi, ctx)
     else:
         out = adjoint_run(capi, ctx, n=1024, fd_dirs=args.fd_dirs)
         out = {"metric": "adjoint gradient cost / forward solve", "config": {

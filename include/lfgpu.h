@@ -69,6 +69,8 @@ int lfgpu_vec_size(const lfgpu_vec *v, int64_t *n);
 int lfgpu_vec_upload(lfgpu_vec *v, const double *host, int64_t n);
 int lfgpu_vec_download(const lfgpu_vec *v, double *host, int64_t n);
 int lfgpu_vec_fill(lfgpu_vec *v, double value);
+/* y = x on the context stream (both of the same size, different vectors). */
+int lfgpu_vec_copy(lfgpu_ctx *ctx, const lfgpu_vec *x, lfgpu_vec *y);
 
 /* ---- mesh + DOF handler ------------------------------------------------
  * Replaces the cell loop's view of the mesh and of UniformFEDofHandler
@@ -244,6 +246,12 @@ typedef struct lfgpu_pcg_opts {
                            /* factor since the last one (default 0.3)  */
   int refine_max_interval; /* ... or after this many iterations        */
                            /* (default: never forced)                  */
+  /* FP64 mode only: 1 = one fused sweep per iteration with the
+   * single-reduction (Chronopoulos-Gear) recurrences, which give Eigen's
+   * iterates in exact arithmetic and read the off-diagonal neighbours'
+   * preconditioned residual on the fly; 0 = two sweeps with Eigen's own
+   * recurrences (default). */
+  int single_reduction;
 } lfgpu_pcg_opts;
 typedef struct lfgpu_pcg_info {
   int iters;       /* Eigen's iteration count                     */

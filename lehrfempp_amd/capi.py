@@ -34,7 +34,7 @@ class Coeff(C.Structure):
 class PcgOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("maxit", C.c_int), ("mode", C.c_int),
                 ("check_every", C.c_int), ("refine_delta", C.c_double),
-                ("refine_max_interval", C.c_int)]
+                ("refine_max_interval", C.c_int), ("single_reduction", C.c_int)]
 
 
 class PcgInfo(C.Structure):
@@ -55,6 +55,7 @@ SIGNATURES = {
     "lfgpu_vec_upload": (C.c_int, [_P, _d, C.c_int64]),
     "lfgpu_vec_download": (C.c_int, [_P, _d, C.c_int64]),
     "lfgpu_vec_fill": (C.c_int, [_P, C.c_double]),
+    "lfgpu_vec_copy": (C.c_int, [_P, _P, _P]),
     "lfgpu_mesh_create": (C.c_int, [_P, C.c_int64, _d, C.c_int64, _i32, C.c_int, _i64,
                                     C.c_int64, C.POINTER(_P)]),
     "lfgpu_mesh_destroy": (C.c_int, [_P]),
@@ -344,10 +345,10 @@ def spmv_timed(ctx, A: Csr, x: Vec, y: Vec, reps: int):
 
 
 def pcg(ctx, A: Csr, b: Vec, x: Vec, rtol=1e-10, maxit=None, mode=PCG_FP64, check_every=64,
-        raise_on_noconv=True, refine_delta=0.0, refine_max_interval=0):
+        raise_on_noconv=True, refine_delta=0.0, refine_max_interval=0, single_reduction=False):
     n = A.info()[0]
     opts = PcgOpts(rtol, int(2 * n if maxit is None else maxit), mode, check_every,
-                   float(refine_delta), int(refine_max_interval))
+                   float(refine_delta), int(refine_max_interval), int(bool(single_reduction)))
     info = PcgInfo()
     rc = lib().lfgpu_pcg(ctx.h, A.h, b.h, x.h, C.byref(opts), C.byref(info))
     if rc != OK and (rc != ENOCONV or raise_on_noconv):
@@ -368,7 +369,7 @@ class Batch:
 
     def pcg(self, b: Vec, x: Vec, rtol=1e-10, maxit=100000, check_every=64,
             raise_on_noconv=True, mode=PCG_FP64, refine_delta=0.0, refine_max_interval=0):
-        opts = PcgOpts(rtol, int(maxit), mode, check_every, refine_delta, refine_max_interval)
+        opts = PcgOpts(rtol, int(maxit), mode, check_every, refine_delta, refine_max_interval, 0)
         infos = (PcgInfo * self.nprob)()
         summ = PcgInfo()
         rc = lib().lfgpu_batch_pcg(self.ctx.h, self.h, b.h, x.h, C.byref(opts), infos,
@@ -386,6 +387,10 @@ class Batch:
 def adjoint_gradient(ctx, mesh: Mesh, fe: FE, theta: Vec, u: Vec, lam: Vec, grad: Vec):
     check(lib().lfgpu_adjoint_gradient(ctx.h, mesh.h, fe.h, theta.h, u.h, lam.h, grad.h))
     return ctx.last_ms()
+
+
+def copy(ctx, x: Vec, y: Vec):
+    check(lib().lfgpu_vec_copy(ctx.h, x.h, y.h))
 
 
 def dot(ctx, x: Vec, y: Vec):

@@ -257,6 +257,29 @@ extern "C" int lfgpu_vec_fill(lfgpu_vec *v, double value) {
   return LFGPU_OK;
 }
 
+// y = x with 16-byte accesses: the reference point for HBM byte counters
+// (n doubles read, n written).
+__global__ void k_copy(const double2 *__restrict__ x, double2 *__restrict__ y, int64_t n2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) y[i] = x[i];
+}
+
+extern "C" int lfgpu_vec_copy(lfgpu_ctx *ctx, const lfgpu_vec *x, lfgpu_vec *y) {
+  LFG_REQUIRE(ctx && x && y && x->n == y->n, LFGPU_EINVAL, "bad lfgpu_vec_copy arguments");
+  LFG_REQUIRE(x->d != y->d, LFGPU_EINVAL, "lfgpu_vec_copy needs two different vectors");
+  if (x->n == 0) return LFGPU_OK;
+  const int64_t n2 = x->n / 2;
+  if (n2 > 0)
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n2, 256)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const double2 *>(x->d), reinterpret_cast<double2 *>(y->d),
+                       n2);
+  if (x->n & 1)
+    LFG_HIP(hipMemcpyAsync(y->d + x->n - 1, x->d + x->n - 1, sizeof(double),
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
+}
+
 // ===========================================================================
 // host topology helpers
 // ===========================================================================

@@ -40,7 +40,7 @@ constexpr unsigned kMaxGrid = 65536;
 // Operand layout of v_mfma_f64_16x16x4f64 (64 lanes, l = lane):
 //   A (16 x 4):  lane l holds A[l % 16][l / 16]
 //   B (4 x 16):  lane l holds B[l / 16][l % 16]
-//   D (16 x 16): lane l, component v holds D[4 (l / 16) + v][l % 16]
+//   D (16 x 16): lane l, component v holds D[l / 16 + 4 v][l % 16]
 // Here A rows are cells, A columns / B rows are terms and B / D columns are
 // element-matrix entries (row-major i * nloc + j).
 template <int NLOC, int NQP, int KT>
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_element_contract(
       if (e < NE) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int64_t rc = wbase + 4 * q + v;
+          const int64_t rc = wbase + q + 4 * v;
           if (rc < ncells) ke[rc * NE + e] = acc[v];
         }
       }

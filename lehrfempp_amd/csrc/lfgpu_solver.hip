@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -56,6 +57,7 @@ struct PcgState {
   int iterp[2];    // mixed: iteration count by parity
   int refinements;
   int last_refine; // mixed: iteration of the last reliable update
+  double alpha[2]; // single-reduction FP64: alpha by parity
 };
 
 struct lfgpu_solver_cache {
@@ -81,6 +83,7 @@ struct lfgpu_solver_cache {
   double *r64 = nullptr;   // mixed with FP64 vectors: true residual
   double *xa64 = nullptr;  // mixed with FP64 vectors: increment accumulator
   double *part = nullptr;  // [kPartSlots x kMaxGrid] partial sums
+  double *cg1 = nullptr;   // single-reduction PCG: r, w, s by parity [6 n]
   PcgState *st = nullptr;
 };
 
@@ -91,8 +94,8 @@ void solver_cache_free(lfgpu_csr *A) {
   for (void *p : {(void *)c->slice_off, (void *)c->s_col, (void *)c->val, (void *)c->val32,
                   (void *)c->dinv, (void *)c->dinv32, (void *)c->rowsum, (void *)c->rowsum32,
                   (void *)c->r, (void *)c->p, (void *)c->q, (void *)c->r32, (void *)c->p32,
-                  (void *)c->q32, (void *)c->xa32, (void *)c->xlo, (void *)c->r64, (void *)c->xa64, (void *)c->part,
-                  (void *)c->st})
+                  (void *)c->q32, (void *)c->xa32, (void *)c->xlo, (void *)c->r64, (void *)c->xa64,
+                  (void *)c->part, (void *)c->cg1, (void *)c->st})
     dfree(p);
   delete c;
   A->cache = nullptr;
@@ -508,26 +511,27 @@ __device__ __forceinline__ void block_sum_k(double (&v)[K], double *sh) {
 }
 
 // Interleaved partials of the FP64 iteration: group g's slots at part[8 g + k].
-constexpr int kIlv = 8;
-
+// Partial sums are stored slot-major (part[slot * kMaxGrid + group]): the
+// redundant re-reduction at the start of the next sweep reads, per slot, one
+// contiguous run of ngroups doubles in every workgroup.
 template <int K>
-__device__ __forceinline__ void store_partials_ilv(double *part, double (&v)[K], const int (&slots)[K],
-                                                   double *sh) {
+__device__ __forceinline__ void store_partials_k(double *part, double (&v)[K], const int (&slots)[K],
+                                                 double *sh) {
   block_sum_k<K>(v, sh);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < K; ++k) part[kIlv * blockIdx.x + slots[k]] = v[k];
+    for (int k = 0; k < K; ++k) part[slots[k] * kMaxGrid + blockIdx.x] = v[k];
 }
 
 template <int K>
-__device__ __forceinline__ void reduce_partials_ilv(const double *part, int ngroups,
-                                                    const int (&slots)[K], double (&out)[K],
-                                                    double *sh) {
+__device__ __forceinline__ void reduce_partials_k(const double *part, int ngroups,
+                                                  const int (&slots)[K], double (&out)[K],
+                                                  double *sh) {
 #pragma unroll
   for (int k = 0; k < K; ++k) out[k] = 0.0;
   for (int g = threadIdx.x; g < ngroups; g += 256)
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[k] += part[kIlv * g + slots[k]];
+    for (int k = 0; k < K; ++k) out[k] += part[slots[k] * kMaxGrid + g];
   block_sum_k<K>(out, sh);
 }
 
@@ -571,7 +575,7 @@ __global__ __launch_bounds__(256) void k_cg_init(CgArgs a, const double *__restr
   }
   double v[3] = {bb, rr, rz};
   const int sl[3] = {P_BB, P_RR + 1, P_RZ + 1};
-  store_partials_ilv<3>(a.part, v, sl, sh);
+  store_partials_k<3>(a.part, v, sl, sh);
 }
 
 // one workgroup: Eigen's early exits (zero rhs, converged start, maxit 0)
@@ -579,7 +583,7 @@ __global__ __launch_bounds__(256) void k_cg_init_finish(CgArgs a) {
   __shared__ double sh[16];
   double t[3];
   const int sl[3] = {P_BB, P_RR + 1, P_RZ + 1};
-  reduce_partials_ilv<3>(a.part, a.sw.ngroups, sl, t, sh);
+  reduce_partials_k<3>(a.part, a.sw.ngroups, sl, t, sh);
   const double bb = t[0], rr = t[1], rz = t[2];
   if (threadIdx.x == 0) {
     PcgState *st = a.st;
@@ -623,7 +627,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
   if (st->done) return;
   double t[1];
   const int sl_in[1] = {P_RR + (par ^ 1)};
-  reduce_partials_ilv<1>(a.part, a.sw.ngroups, sl_in, t, sh);
+  reduce_partials_k<1>(a.part, a.sw.ngroups, sl_in, t, sh);
   const double rr = t[0];
   if (rr < st->thr) {  // converged: the iteration count stays as it is
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -654,7 +658,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
   }
   double v[3] = {pq, qdr, qdq};
   const int sl[3] = {P_PQ, P_QDR, P_QDQ};
-  store_partials_ilv<3>(a.part, v, sl, sh);
+  store_partials_k<3>(a.part, v, sl, sh);
 }
 
 // FP64 iteration k, sweep 2: alpha = rho_k / (p.q);
@@ -668,7 +672,7 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
   if (st->done) return;
   double t[4];
   const int sl_in[4] = {P_RZ + (par ^ 1), P_PQ, P_QDR, P_QDQ};
-  reduce_partials_ilv<4>(a.part, a.sw.ngroups, sl_in, t, sh);
+  reduce_partials_k<4>(a.part, a.sw.ngroups, sl_in, t, sh);
   const double rho = t[0], pq = t[1], qdr = t[2], qdq = t[3];
   const double alpha = rho / pq;
   const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
@@ -689,7 +693,222 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
   }
   double v[2] = {rr, rz};
   const int sl[2] = {P_RR + par, P_RZ + par};
-  store_partials_ilv<2>(a.part, v, sl, sh);
+  store_partials_k<2>(a.part, v, sl, sh);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 single-reduction PCG (Chronopoulos-Gear): one sweep per iteration.
+//   u = D^-1 r, w = A u;  gamma = (r, u), delta = (w, u)
+//   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
+//   p = u + beta p, s = w + beta s (s = A p), x += alpha p, r -= alpha s,
+//   then the next w = A D^-1 r.
+// The product needs the new u at the off-diagonal neighbours, which other
+// workgroups write in the same sweep; every row therefore re-derives it from
+// the previous r, w, s of the neighbour with the same operations its owner
+// uses (bit-identical), and r, w, s are double-buffered by iteration parity.
+// p and x are only touched by their own row.
+// ---------------------------------------------------------------------------
+constexpr int C_D = 0, C_RR = 3, C_G = 5;  // partial slots (+parity)
+
+struct Cg1Args {
+  MatView m;
+  const double *__restrict__ dinv;
+  double *__restrict__ x;
+  double *__restrict__ p;
+  double *r[2], *w[2], *s[2];
+  double *part;
+  PcgState *st;
+  Sweep sw;
+};
+
+// The new preconditioned residual of row j, u_j = D^-1 (r_j - alpha (w_j +
+// beta s_j)), from the previous iteration's vectors.
+__device__ __forceinline__ double cg1_u(const double *__restrict__ dinv,
+                                        const double *__restrict__ r,
+                                        const double *__restrict__ w,
+                                        const double *__restrict__ s, int32_t j, double alpha,
+                                        double beta) {
+  const double sj = w[j] + beta * s[j];
+  const double rj = r[j] - alpha * sj;
+  return dinv[j] * rj;
+}
+
+// (A u)_i for u given row-wise by the functor U (difference form as apply64)
+template <int FMT, class U>
+__device__ __forceinline__ double apply64_fn(const MatView &m, int32_t r, double ur, const U &u) {
+  double acc = 0.0;
+  if constexpr (FMT == FMT_DIA) {
+#pragma unroll
+    for (int d = 0; d < kMaxDiag; ++d) {
+      if (d < m.ndiag) {
+        int32_t c = r + m.off[d];
+        c = c < 0 ? 0 : (c >= m.n ? m.n - 1 : c);
+        acc += m.val[(int64_t)d * m.n + r] * (u(c) - ur);
+      }
+    }
+  } else {
+    const int32_t sl = r >> 6, l = r & 63;
+    const int32_t o = m.slice_off[sl];
+    const int32_t wd = (m.slice_off[sl + 1] - o) >> 6;
+    const double *vp = m.val + o + l;
+    const int32_t *cp = m.s_col + o + l;
+    for (int32_t k = 0; k < wd; ++k) acc += vp[64 * k] * (u(cp[64 * k]) - ur);
+  }
+  return acc + m.rowsum[r] * ur;
+}
+
+// setup 1: r = b - A x into r[1]; p = 0, s[1] = 0; partials b.b
+template <int FMT>
+__global__ __launch_bounds__(256) void k_cg1_init_r(Cg1Args a, const double *__restrict__ b) {
+  __shared__ double sh[16];
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  double bb = 0.0;
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const double bi = b[i];
+    a.r[1][i] = bi - apply64<FMT>(a.m, a.x, i, a.x[i]);
+    a.p[i] = 0.0;
+    a.s[1][i] = 0.0;
+    bb += bi * bi;
+  }
+  double v[1] = {bb};
+  const int sl[1] = {P_BB};
+  store_partials_k<1>(a.part, v, sl, sh);
+}
+
+// setup 2: w = A D^-1 r into w[1]; partials r.r, r.u, w.u (parity 1)
+template <int FMT>
+__global__ __launch_bounds__(256) void k_cg1_init_w(Cg1Args a) {
+  __shared__ double sh[16];
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  const double *__restrict__ rv = a.r[1];
+  const double *__restrict__ dinv = a.dinv;
+  double rr = 0.0, gam = 0.0, dlt = 0.0;
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const double ri = rv[i];
+    const double ui = dinv[i] * ri;
+    const double wi =
+        apply64_fn<FMT>(a.m, i, ui, [&](int32_t j) { return dinv[j] * rv[j]; });
+    a.w[1][i] = wi;
+    rr += ri * ri;
+    gam += ri * ui;
+    dlt += wi * ui;
+  }
+  double v[3] = {rr, gam, dlt};
+  const int sl[3] = {C_RR + 1, C_G + 1, C_D + 1};
+  store_partials_k<3>(a.part, v, sl, sh);
+}
+
+// one workgroup: Eigen's early exits
+__global__ __launch_bounds__(256) void k_cg1_init_finish(Cg1Args a) {
+  __shared__ double sh[16];
+  double t[2];
+  const int sl[2] = {P_BB, C_RR + 1};
+  reduce_partials_k<2>(a.part, a.sw.ngroups, sl, t, sh);
+  const double bb = t[0], rr = t[1];
+  if (threadIdx.x == 0) {
+    PcgState *st = a.st;
+    st->bb = bb;
+    st->rr = rr;
+    st->iter = -1;
+    st->iterp[0] = st->iterp[1] = -1;
+    st->converged = 0;
+    st->done = 0;
+    st->refinements = 0;
+    const double thr = st->thr * bb;  // thr holds tol^2 on entry
+    st->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+    if (bb == 0.0 || rr < st->thr) {
+      st->done = 1;
+      st->converged = 1;
+      st->iter = 0;
+    } else if (st->maxit <= 0) {
+      st->done = 1;
+      st->iter = 0;
+    }
+  }
+}
+
+// iteration (par = iteration & 1): reads the parity par^1 vectors and
+// partials, writes parity par.  The stopping test and the iteration count
+// follow k_cg_spmv: a converged residual leaves the count as it is, and the
+// maxit-th update is not applied.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_cg1(Cg1Args a, int par) {
+  __shared__ double sh[16];
+  PcgState *st = a.st;
+  if (st->done) return;
+  double t[3];
+  const int sl_in[3] = {C_RR + (par ^ 1), C_G + (par ^ 1), C_D + (par ^ 1)};
+  reduce_partials_k<3>(a.part, a.sw.ngroups, sl_in, t, sh);
+  const double rr = t[0], gam = t[1], dlt = t[2];
+  const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+  const int itp = st->iterp[par ^ 1];
+  if (rr < st->thr) {
+    if (writer) {
+      st->rr = rr;
+      st->done = 1;
+      st->converged = 1;
+      st->iter = itp < 0 ? 0 : itp;
+    }
+    return;
+  }
+  const int it = itp + 1;
+  if (it >= st->maxit) {
+    if (writer) {
+      st->rr = rr;
+      st->iter = it;
+      st->done = 1;
+    }
+    return;
+  }
+  const double beta = it == 0 ? 0.0 : gam / st->rho[par ^ 1];
+  const double alpha = it == 0 ? gam / dlt : gam / (dlt - beta * gam / st->alpha[par ^ 1]);
+  if (writer) {
+    st->rr = rr;
+    st->iter = it;
+    st->iterp[par] = it;
+    st->rho[par] = gam;
+    st->alpha[par] = alpha;
+  }
+  const double *__restrict__ ro = a.r[par ^ 1];
+  const double *__restrict__ wo = a.w[par ^ 1];
+  const double *__restrict__ so = a.s[par ^ 1];
+  double *__restrict__ rn = a.r[par];
+  double *__restrict__ wn = a.w[par];
+  double *__restrict__ sn = a.s[par];
+  const double *__restrict__ dinv = a.dinv;
+  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
+  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
+  double rr2 = 0.0, gam2 = 0.0, dlt2 = 0.0;
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    const double di = dinv[i];
+    const double ri = ro[i];
+    const double pi = di * ri + beta * a.p[i];
+    const double si = wo[i] + beta * so[i];
+    const double rni = ri - alpha * si;
+    const double ui = di * rni;
+    const double wi = apply64_fn<FMT>(a.m, i, ui, [&](int32_t j) {
+      return cg1_u(dinv, ro, wo, so, j, alpha, beta);
+    });
+    a.x[i] += alpha * pi;
+    a.p[i] = pi;
+    sn[i] = si;
+    rn[i] = rni;
+    wn[i] = wi;
+    rr2 += rni * rni;
+    gam2 += rni * ui;
+    dlt2 += wi * ui;
+  }
+  double v[3] = {rr2, gam2, dlt2};
+  const int sl[3] = {C_RR + par, C_G + par, C_D + par};
+  store_partials_k<3>(a.part, v, sl, sh);
+}
+
+template <int FMT>
+void launch_cg1_chunk(hipStream_t s, const Sweep &sw, const Cg1Args &a, int count) {
+  const dim3 G(sw.ngroups), B(256);
+  for (int k = 0; k < count; ++k) hipLaunchKernelGGL(k_cg1<FMT>, G, B, 0, s, a, k & 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1647,6 +1866,32 @@ extern "C" int lfgpu_spmv_timed(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu
   return LFGPU_OK;
 }
 
+namespace {
+// end of a solve: device time, info, status
+int pcg_finish(lfgpu_ctx *ctx, const PcgState *hs, bool mixed, lfgpu_pcg_info *info) {
+  LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+  LFG_HIP(hipEventSynchronize(ctx->ev1));
+  float ms = 0.f;
+  LFG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  ctx->last_ms = ms;
+  const double relres = hs->bb > 0.0 ? std::sqrt(hs->rr / hs->bb) : 0.0;
+  if (info) {
+    info->iters = hs->iter < 0 ? 0 : hs->iter;
+    info->converged = hs->converged;
+    info->refinements = mixed ? hs->refinements : 0;
+    info->relres = relres;
+    info->ms = ms;
+  }
+  if (!hs->converged) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.3e", relres);
+    set_error(std::string("PCG did not reach rtol within maxit iterations (relres ") + buf + ")");
+    return LFGPU_ENOCONV;
+  }
+  return LFGPU_OK;
+}
+}  // namespace
+
 extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b, lfgpu_vec *x,
                          const lfgpu_pcg_opts *opts, lfgpu_pcg_info *info) {
   LFG_REQUIRE(ctx && Ac && b && x && opts, LFGPU_EINVAL, "NULL argument to lfgpu_pcg");
@@ -1671,6 +1916,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   int check = opts->check_every > 0 ? opts->check_every : 64;
   check += check & 1;
 
+  const bool single = opts->mode == LFGPU_PCG_FP64 && opts->single_reduction;
+  if (single && !c->cg1)
+    LFG_TRY(dalloc(&c->cg1, (size_t)6 * n, "single-reduction PCG vectors"));
   PcgState h{};
   h.thr = opts->rtol * opts->rtol;
   h.maxit = opts->maxit;
@@ -1678,6 +1926,43 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   LFG_HIP(hipEventRecord(ctx->ev0, s));
 
   const MatView mv = view_of(c, n);
+  PcgState *hs = reinterpret_cast<PcgState *>(ctx->pinned);
+  if (single) {
+    Cg1Args a1;
+    a1.m = mv;
+    a1.dinv = c->dinv;
+    a1.x = x->d;
+    a1.p = c->p;
+    for (int q = 0; q < 2; ++q) {
+      a1.r[q] = c->cg1 + (size_t)q * n;
+      a1.w[q] = c->cg1 + (size_t)(2 + q) * n;
+      a1.s[q] = c->cg1 + (size_t)(4 + q) * n;
+    }
+    a1.part = c->part;
+    a1.st = c->st;
+    a1.sw = sw;
+    if (c->fmt == FMT_DIA) {
+      hipLaunchKernelGGL(k_cg1_init_r<FMT_DIA>, G, B, 0, s, a1, b->d);
+      hipLaunchKernelGGL(k_cg1_init_w<FMT_DIA>, G, B, 0, s, a1);
+    } else {
+      hipLaunchKernelGGL(k_cg1_init_r<FMT_ELL>, G, B, 0, s, a1, b->d);
+      hipLaunchKernelGGL(k_cg1_init_w<FMT_ELL>, G, B, 0, s, a1);
+    }
+    hipLaunchKernelGGL(k_cg1_init_finish, dim3(1), B, 0, s, a1);
+    hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), B, 0, s, x->d, n, c->st);
+    LFG_HIP(hipGetLastError());
+    for (;;) {
+      if (c->fmt == FMT_DIA)
+        launch_cg1_chunk<FMT_DIA>(s, sw, a1, check);
+      else
+        launch_cg1_chunk<FMT_ELL>(s, sw, a1, check);
+      LFG_HIP(hipGetLastError());
+      LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+      LFG_HIP(hipStreamSynchronize(s));
+      if (hs->done) break;
+    }
+    return pcg_finish(ctx, hs, false, info);
+  }
   // the setup sweep leaves r = b - A x in the buffer the chosen mode reads
   // its true residual from
   double *r_init = opts->mode == LFGPU_PCG_MIXED_MATRIX ? c->r64 : c->r;
@@ -1690,7 +1975,6 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), B, 0, s, x->d, n, c->st);
   LFG_HIP(hipGetLastError());
 
-  PcgState *hs = reinterpret_cast<PcgState *>(ctx->pinned);
   const double delta2 =
       opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
   const int max_interval = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
@@ -1716,25 +2000,7 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
       if (hs->done) break;
     }
   }
-  LFG_HIP(hipEventRecord(ctx->ev1, s));
-  LFG_HIP(hipEventSynchronize(ctx->ev1));
-  float ms = 0.f;
-  LFG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  ctx->last_ms = ms;
-  const double relres = hs->bb > 0.0 ? std::sqrt(hs->rr / hs->bb) : 0.0;
-  if (info) {
-    info->iters = hs->iter < 0 ? 0 : hs->iter;
-    info->converged = hs->converged;
-    info->refinements = mixed ? hs->refinements : 0;
-    info->relres = relres;
-    info->ms = ms;
-  }
-  if (!hs->converged) {
-    set_error("PCG did not reach rtol within maxit iterations (relres " +
-              std::to_string(relres) + ")");
-    return LFGPU_ENOCONV;
-  }
-  return LFGPU_OK;
+  return pcg_finish(ctx, hs, mixed, info);
 }
 
 extern "C" int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
