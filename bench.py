@@ -26,6 +26,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -94,6 +95,19 @@ def spmv_bytes(nnz, n, val_bytes=8, vec_bytes=8):
     return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * vec_bytes * n
 
 
+def pmc_traffic(kernel, name="pmc_traffic_p1_2048.md"):
+    """HBM bytes per dispatch of `kernel` recorded in the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE of the same dispatches, each scaled
+    by the copy-kernel calibration written next to it), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_dispatch %s: (\d+)" % re.escape(kernel), text, re.M)
+    return int(m.group(1)) if m else None
+
+
 def true_relres(capi, ctx, A, x, b):
     y = capi.Vec(ctx, b.n)
     capi.spmv(ctx, A, x, y)
@@ -116,7 +130,7 @@ def row_scaled_err(rp, a, b):
 # ---------------------------------------------------------------- configs
 def cpu_baseline_p1(P, gpu_iters):
     """Reference-faithful CPU path (the oracle, one thread) on a bounded sample."""
-    from oracle import lfref as L  # the checker; timed here as the CPU baseline
+    from oracle import lfref as L  # CPU restatement, timed here as the CPU baseline
     m = L.tp_triag_mesh(512)
     cd, nd = m.dofs(1)
     a = np.exp(np.random.default_rng(0).standard_normal(m.ne))
@@ -150,7 +164,7 @@ def cpu_baseline_p1(P, gpu_iters):
 
 
 def adjoint_run(capi, ctx, n=1024, fd_dirs=0, seed_theta=1, seed_star=2):
-    """adjoint gradient of J on an n x n P1 mesh; returns the cost
+    """Adjoint gradient of J on an n x n P1 mesh; returns the cost
     ratio (adjoint solve + gradient kernel) / forward solve and FD errors."""
     rng = np.random.default_rng(seed_theta)
     P = Problem(capi, ctx, n, seed_theta, gamma=0.0, f_sinsin=False)
@@ -314,8 +328,13 @@ def run_p1(args, capi, ctx):
         "plan_build_s": P.plan_s,
         "roofline": {"bound": "hbm", "kernel": "k_spmv_f64 (FP64 SpMV, DIA storage)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("k_spmv_f64"),
                      "algorithmic_bytes": sb, "avg_us": 1e3 * spmv_ms},
+        "pcg_iteration": {
+            "algorithmic_bytes": sb + 80 * P.nv,
+            "us": 1e3 * float(np.median(pcg_ms)) / max(1, iters[-1]),
+            "achieved_gbs": (sb + 80 * P.nv) / (float(np.median(pcg_ms)) * 1e-3
+                                                / max(1, iters[-1])) / 1e9},
         "cpu_baseline": cpu,
     }
     P.close()

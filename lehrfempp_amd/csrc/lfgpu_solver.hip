@@ -537,10 +537,7 @@ __device__ __forceinline__ void reduce_partials_k(const double *part, int ngroup
 
 // partial slots (FP64 mode); P_RR and P_RZ are followed by their parity twin
 constexpr int P_PQ = 0, P_QDR = 1, P_QDQ = 2, P_RR = 3, P_RZ = 5, P_BB = 7;
-// mixed mode (after the shared setup)
-constexpr int M_PQ = 0, M_RR = 1, M_RZ = 2, M_RR64 = 3, M_RZ64 = 4;
 
-__device__ __forceinline__ double *slot(double *part, int k) { return part + k * kMaxGrid; }
 
 struct CgArgs {
   MatView m;
@@ -646,7 +643,6 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double pq = 0.0, qdr = 0.0, qdq = 0.0;
-#pragma unroll 2
   for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
     const double pi = a.p[i];
     const double qi = apply64<FMT>(a.m, a.p, i, pi);
@@ -680,7 +676,6 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
-#pragma unroll 2
   for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
     const double pi = a.p[i];
     a.x[i] += alpha * pi;
@@ -979,131 +974,144 @@ __device__ __forceinline__ double apply_mixed(const MatView &m, const double *__
   return acc + (double)m.rowsum32[r] * xr;
 }
 
+// Two sweeps per iteration, as in the FP64 solver.  Sweep A decides, from
+// the previous sweep's partials (identically in every workgroup), whether
+// this slot is a CG step or a reliable update:
+//   step:    q = A32 p, partials p.q, (M q, r), (M q, q)          (M = D^-1 in FP32)
+//   update:  r64 = b - A x - A xa - A xlo in FP64 (x, xa, xlo as left by the
+//            previous slot), partial ||r64||^2
+// Sweep B completes the slot:
+//   step:    alpha = rho / p.q, beta from rho' = rho - 2 alpha (Mq, r) +
+//            alpha^2 (Mq, q); xa += alpha p, r -= alpha q, p = M r + beta p
+//   update:  fold xa into the double-double (x, xlo) with TwoSum; stop on the
+//            true residual, else replace the recurrence residual r by r64 and
+//            the M r part of the search direction with M r64:
+//            p += M (r64 - r), which is the direction the last step would
+//            have produced from the true residual
+// A reliable update is taken when the recurrence residual has fallen by
+// delta since the last one, reaches rtol, after max_interval steps, or at
+// maxit.  iterp[par] counts the CG steps up to and including slot par.
+constexpr int X_PQ = 0, X_QDR = 1, X_QDQ = 2, X_RT = 0, X_RR = 3, X_RZ = 5;
+
 template <int FMT, class VT>
-__global__ __launch_bounds__(256) void k_mix_spmv(MixArgs<VT> a) {
-  __shared__ double sh[8];
-  if (a.st->done) return;
-  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
-  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
-  double v = 0.0;
-  for (int32_t r = r0 + threadIdx.x; r < r1; r += 256) {
-    const VT pr = a.p[r];
-    const VT qr = apply_mixed<FMT>(a.m, a.p, r, pr);
-    a.q[r] = qr;
-    v += (double)pr * (double)qr;
-  }
-  store_partial(slot(a.part, M_PQ), v, sh);
-}
-
-template <class VT>
-__global__ __launch_bounds__(256) void k_mix_update(MixArgs<VT> a, int par) {
-  __shared__ double sh[8];
-  if (a.st->done) return;
-  const VT alpha = (VT)(a.st->rho[par] / all_reduce(slot(a.part, M_PQ), a.sw.ngroups, sh));
-  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
-  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
-  double rr = 0.0, rz = 0.0;
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
-    a.xa[i] += alpha * a.p[i];
-    const VT ri = a.r[i] - alpha * a.q[i];
-    a.r[i] = ri;
-    rr += (double)ri * (double)ri;
-    rz += (double)ri * (double)((VT)a.dinv32[i] * ri);
-  }
-  store_partial(slot(a.part, M_RR), rr, sh);
-  store_partial(slot(a.part, M_RZ), rz, sh);
-}
-
-// Without a reliable update: beta and the p update.  With one: fold the FP32
-// increment into the double-double solution (TwoSum).
-template <class VT>
-__global__ __launch_bounds__(256) void k_mix_decide(MixArgs<VT> a, int par) {
-  __shared__ double sh[8];
+__global__ __launch_bounds__(256) void k_mx_a(MixArgs<VT> a, int par) {
+  __shared__ double sh[16];
   PcgState *st = a.st;
   if (st->done) return;
-  const double rr = all_reduce(slot(a.part, M_RR), a.sw.ngroups, sh);
-  const double rz = all_reduce(slot(a.part, M_RZ), a.sw.ngroups, sh);
-  const int it = st->iterp[par] + 1;
+  double t[1];
+  const int sl_in[1] = {X_RR + (par ^ 1)};
+  reduce_partials_k<1>(a.part, a.sw.ngroups, sl_in, t, sh);
+  const double rr = t[0];
+  const int itp = st->iterp[par ^ 1];
   const bool refine = rr < st->thr || rr < a.delta2 * st->rr_ref ||
-                      it - st->last_refine >= a.max_interval || it >= st->maxit;
-  const double beta = rz / st->rho[par];
+                      itp - st->last_refine >= a.max_interval || itp >= st->maxit;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->iterp[par ^ 1] = it;
-    st->iter = it;
     st->refine[par] = refine ? 1 : 0;
-    if (!refine) st->rho[par ^ 1] = rz;
+    st->iterp[par] = refine ? itp : itp + 1;
+    if (!refine) st->iter = itp + 1;
   }
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   if (!refine) {
-    const VT fb = (VT)beta;
-    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256)
-      a.p[i] = (VT)a.dinv32[i] * a.r[i] + fb * a.p[i];
-  } else {
+    double pq = 0.0, qdr = 0.0, qdq = 0.0;
     for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
-      const double hi = a.x[i], d = (double)a.xa[i];
-      const double s = hi + d;
-      const double bp = s - hi;
-      const double err = (hi - (s - bp)) + (d - bp);
-      a.x[i] = s;
-      a.xlo[i] += err;
-      a.xa[i] = (VT)0;
+      const VT pi = a.p[i];
+      const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
+      a.q[i] = qi;
+      const double dq = (double)((VT)a.dinv32[i] * qi);
+      pq += (double)pi * (double)qi;
+      qdr += dq * (double)a.r[i];
+      qdq += dq * (double)qi;
     }
+    double v[3] = {pq, qdr, qdq};
+    const int sl[3] = {X_PQ, X_QDR, X_QDQ};
+    store_partials_k<3>(a.part, v, sl, sh);
+  } else {
+    double rt = 0.0;
+    const VT *__restrict__ xa = a.xa;
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+      const double ah = apply64<FMT>(a.m, a.x, i, a.x[i]);
+      const double aa =
+          apply64_fn<FMT>(a.m, i, (double)xa[i], [&](int32_t j) { return (double)xa[j]; });
+      const double al = apply64<FMT>(a.m, a.xlo, i, a.xlo[i]);
+      const double ri = ((a.b[i] - ah) - aa) - al;
+      a.r64[i] = ri;
+      rt += ri * ri;
+    }
+    double v[1] = {rt};
+    const int sl[1] = {X_RT};
+    store_partials_k<1>(a.part, v, sl, sh);
   }
 }
 
-// reliable update: true residual r64 = b - A x - A xlo (FP64)
-template <int FMT, class VT>
-__global__ __launch_bounds__(256) void k_mix_resid(MixArgs<VT> a, int par) {
-  __shared__ double sh[8];
-  if (a.st->done || !a.st->refine[par]) return;
+template <class VT>
+__global__ __launch_bounds__(256) void k_mx_b(MixArgs<VT> a, int par) {
+  __shared__ double sh[16];
+  PcgState *st = a.st;
+  if (st->done) return;
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
-    const double ah = apply64<FMT>(a.m, a.x, i, a.x[i]);
-    const double al = apply64<FMT>(a.m, a.xlo, i, a.xlo[i]);
-    const double ri = (a.b[i] - ah) - al;
-    a.r64[i] = ri;
-    rr += ri * ri;
-    rz += ri * (a.dinv[i] * ri);
-  }
-  store_partial(slot(a.part, M_RR64), rr, sh);
-  store_partial(slot(a.part, M_RZ64), rz, sh);
-}
-
-// reliable update: convergence on the true residual, or restart the FP32
-// recurrence from it with beta rescaled by the new (r, D^-1 r).
-template <class VT>
-__global__ __launch_bounds__(256) void k_mix_finish(MixArgs<VT> a, int par) {
-  __shared__ double sh[8];
-  PcgState *st = a.st;
-  if (st->done || !st->refine[par]) return;
-  const double rt = all_reduce(slot(a.part, M_RR64), a.sw.ngroups, sh);
-  const double rz = all_reduce(slot(a.part, M_RZ64), a.sw.ngroups, sh);
-  const bool conv = rt < st->thr;
-  const bool stop = conv || st->iterp[par ^ 1] >= st->maxit;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->rr = rt;
-    st->refinements += 1;
-    st->last_refine = st->iterp[par ^ 1];
-    if (stop) {
-      st->done = 1;
-      st->converged = conv ? 1 : 0;
-    } else {
-      st->rr_ref = rt;
-      st->rho[par ^ 1] = rz;
+  if (!st->refine[par]) {
+    double t[4];
+    const int sl_in[4] = {X_RZ + (par ^ 1), X_PQ, X_QDR, X_QDQ};
+    reduce_partials_k<4>(a.part, a.sw.ngroups, sl_in, t, sh);
+    const double rho = t[0], pq = t[1], qdr = t[2], qdq = t[3];
+    const double alpha = rho / pq;
+    const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
+    const VT fa = (VT)alpha, fb = (VT)(rho_next / rho);
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+      const VT pi = a.p[i];
+      a.xa[i] += fa * pi;
+      const VT ri = a.r[i] - fa * a.q[i];
+      a.r[i] = ri;
+      const VT zi = (VT)a.dinv32[i] * ri;
+      a.p[i] = zi + fb * pi;
+      rr += (double)ri * (double)ri;
+      rz += (double)ri * (double)zi;
     }
+  } else {
+    double t[1];
+    const int sl_in[1] = {X_RT};
+    reduce_partials_k<1>(a.part, a.sw.ngroups, sl_in, t, sh);
+    const double rt = t[0];
+    const bool conv = rt < st->thr;
+    const int its = st->iterp[par];
+    const bool stop = conv || its >= st->maxit;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rt;
+      st->refinements += 1;
+      st->last_refine = its;
+      if (stop) {
+        st->done = 1;
+        st->converged = conv ? 1 : 0;
+      } else {
+        st->rr_ref = rt;
+      }
+    }
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+      const double hi = a.x[i], d = (double)a.xa[i];
+      const double sm = hi + d;
+      const double bp = sm - hi;
+      const double err = (hi - (sm - bp)) + (d - bp);
+      a.x[i] = sm;
+      a.xlo[i] += err;
+      a.xa[i] = (VT)0;
+      if (!stop) {
+        const VT ri = (VT)a.r64[i];
+        const VT mi = (VT)a.dinv32[i];
+        a.p[i] += mi * (ri - a.r[i]);
+        a.r[i] = ri;
+        const VT zi = mi * ri;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+      }
+    }
+    if (stop) return;
   }
-  if (stop) return;
-  const VT fb = (VT)(rz / st->rho[par]);
-  const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
-  const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
-    const VT ri = (VT)a.r64[i];
-    a.r[i] = ri;
-    a.p[i] = (VT)a.dinv32[i] * ri + fb * a.p[i];
-  }
+  double v[2] = {rr, rz};
+  const int sl[2] = {X_RR + par, X_RZ + par};
+  store_partials_k<2>(a.part, v, sl, sh);
 }
 
 template <class VT>
@@ -1127,11 +1135,8 @@ void launch_mix_chunk(hipStream_t s, const Sweep &sw, const MixArgs<VT> &m, int 
   const dim3 G(sw.ngroups), B(256);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    hipLaunchKernelGGL((k_mix_spmv<FMT, VT>), G, B, 0, s, m);
-    hipLaunchKernelGGL(k_mix_update<VT>, G, B, 0, s, m, par);
-    hipLaunchKernelGGL(k_mix_decide<VT>, G, B, 0, s, m, par);
-    hipLaunchKernelGGL((k_mix_resid<FMT, VT>), G, B, 0, s, m, par);
-    hipLaunchKernelGGL(k_mix_finish<VT>, G, B, 0, s, m, par);
+    hipLaunchKernelGGL((k_mx_a<FMT, VT>), G, B, 0, s, m, par);
+    hipLaunchKernelGGL(k_mx_b<VT>, G, B, 0, s, m, par);
   }
 }
 
@@ -1334,8 +1339,8 @@ void launch_bcg_chunk(hipStream_t s, const BArgs &a, int count) {
 }
 
 // ---------------------------------------------------------------------------
-// Batched mixed precision: the reliable-update scheme of the single solve
-// (k_mix_*), run per problem on the workgroup table.  VT = float: FP32
+// Batched mixed precision: a reliable-update scheme like the single solve's,
+// run per problem on the workgroup table.  VT = float: FP32
 // matrix, recurrence vectors and increment; VT = double: FP32 matrix only.
 // FP64 true residuals, dot products and a double-double solution throughout.
 // ---------------------------------------------------------------------------
@@ -1347,7 +1352,8 @@ struct BMState {
   int last_refine, refinements;
 };
 
-constexpr int MB_BB = 0, MB_PQ = 1, MB_RR = 2, MB_RZ = 3, MB_RR64 = 4, MB_RZ64 = 5;
+// partial slots: the X_* slots of the single solve, plus b.b
+constexpr int MB_BB = 7;
 
 template <class VT>
 struct BMArgs {
@@ -1396,8 +1402,8 @@ __global__ __launch_bounds__(256) void k_bmix_init(BMArgs<VT> a) {
     rz += ri * (a.dinv[i] * ri);
   }
   store_partial(mslot(a, MB_BB), bb, sh);
-  store_partial(mslot(a, MB_RR64), rr, sh);
-  store_partial(mslot(a, MB_RZ64), rz, sh);
+  store_partial(mslot(a, X_RR + 1), rr, sh);
+  store_partial(mslot(a, X_RZ + 1), rz, sh);
 }
 
 // one workgroup per problem: Eigen's early exits as in k_cg_init_finish
@@ -1406,8 +1412,8 @@ __global__ __launch_bounds__(256) void k_bmix_init_finish(BMArgs<VT> a) {
   __shared__ double sh[8];
   const BProb P = a.probs[blockIdx.x];
   const double bb = seg_reduce(mslot(a, MB_BB), P.gb, P.ge, sh);
-  const double rr = seg_reduce(mslot(a, MB_RR64), P.gb, P.ge, sh);
-  const double rz = seg_reduce(mslot(a, MB_RZ64), P.gb, P.ge, sh);
+  const double rr = seg_reduce(mslot(a, X_RR + 1), P.gb, P.ge, sh);
+  const double rz = seg_reduce(mslot(a, X_RZ + 1), P.gb, P.ge, sh);
   if (threadIdx.x == 0) {
     BMState *st = a.st + blockIdx.x;
     st->bb = bb;
@@ -1441,129 +1447,121 @@ __global__ void k_bmix_zero_if_bb0(BMArgs<VT> a) {
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) a.x[i] = 0.0;
 }
 
+// One iteration slot per problem in two sweeps, as k_mx_a / k_mx_b: every
+// workgroup of a problem takes the same step-or-update decision from that
+// problem's segmented partials.
 template <int FMT, class VT>
-__global__ __launch_bounds__(256) void k_bmix_spmv(BMArgs<VT> a) {
-  __shared__ double sh[8];
-  const BGroup G = a.groups[blockIdx.x];
-  if (a.st[G.prob].done) return;
-  double v = 0.0;
-  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-    const VT pi = a.p[i];
-    const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
-    a.q[i] = qi;
-    v += (double)pi * (double)qi;
-  }
-  store_partial(mslot(a, MB_PQ), v, sh);
-}
-
-template <class VT>
-__global__ __launch_bounds__(256) void k_bmix_update(BMArgs<VT> a, int par) {
-  __shared__ double sh[8];
-  const BGroup G = a.groups[blockIdx.x];
-  const BMState *st = a.st + G.prob;
-  if (st->done) return;
-  const BProb P = a.probs[G.prob];
-  const VT alpha = (VT)(st->rho[par] / seg_reduce(mslot(a, MB_PQ), P.gb, P.ge, sh));
-  double rr = 0.0, rz = 0.0;
-  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-    a.xa[i] += alpha * a.p[i];
-    const VT ri = a.r[i] - alpha * a.q[i];
-    a.r[i] = ri;
-    rr += (double)ri * (double)ri;
-    rz += (double)ri * (double)((VT)a.dinv32[i] * ri);
-  }
-  store_partial(mslot(a, MB_RR), rr, sh);
-  store_partial(mslot(a, MB_RZ), rz, sh);
-}
-
-template <class VT>
-__global__ __launch_bounds__(256) void k_bmix_decide(BMArgs<VT> a, int par) {
+__global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
   __shared__ double sh[8];
   const BGroup G = a.groups[blockIdx.x];
   BMState *st = a.st + G.prob;
   if (st->done) return;
   const BProb P = a.probs[G.prob];
   const bool writer = blockIdx.x == (unsigned)G.gb && threadIdx.x == 0;
-  const double rr = seg_reduce(mslot(a, MB_RR), P.gb, P.ge, sh);
-  const double rz = seg_reduce(mslot(a, MB_RZ), P.gb, P.ge, sh);
-  const int it = st->iterp[par] + 1;
+  const double rr = seg_reduce(mslot(a, X_RR + (par ^ 1)), P.gb, P.ge, sh);
+  const int itp = st->iterp[par ^ 1];
   const bool refine = rr < st->thr || rr < a.delta2 * st->rr_ref ||
-                      it - st->last_refine >= a.max_interval || it >= st->maxit;
-  const double beta = rz / st->rho[par];
+                      itp - st->last_refine >= a.max_interval || itp >= st->maxit;
   if (writer) {
-    st->iterp[par ^ 1] = it;
-    st->iter = it;
     st->refine[par] = refine ? 1 : 0;
-    if (!refine) st->rho[par ^ 1] = rz;
+    st->iterp[par] = refine ? itp : itp + 1;
+    if (!refine) st->iter = itp + 1;
   }
   if (!refine) {
-    const VT fb = (VT)beta;
-    for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256)
-      a.p[i] = (VT)a.dinv32[i] * a.r[i] + fb * a.p[i];
-  } else {
+    double pq = 0.0, qdr = 0.0, qdq = 0.0;
     for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-      const double hi = a.x[i], d = (double)a.xa[i];
-      const double s = hi + d;
-      const double bp = s - hi;
-      const double err = (hi - (s - bp)) + (d - bp);
-      a.x[i] = s;
-      a.xlo[i] += err;
-      a.xa[i] = (VT)0;
+      const VT pi = a.p[i];
+      const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
+      a.q[i] = qi;
+      const double dq = (double)((VT)a.dinv32[i] * qi);
+      pq += (double)pi * (double)qi;
+      qdr += dq * (double)a.r[i];
+      qdq += dq * (double)qi;
     }
+    store_partial(mslot(a, X_PQ), pq, sh);
+    store_partial(mslot(a, X_QDR), qdr, sh);
+    store_partial(mslot(a, X_QDQ), qdq, sh);
+  } else {
+    double rt = 0.0;
+    const VT *__restrict__ xa = a.xa;
+    for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+      const double ah = apply64<FMT>(a.m, a.x, i, a.x[i]);
+      const double aa =
+          apply64_fn<FMT>(a.m, i, (double)xa[i], [&](int32_t j) { return (double)xa[j]; });
+      const double al = apply64<FMT>(a.m, a.xlo, i, a.xlo[i]);
+      const double ri = ((a.b[i] - ah) - aa) - al;
+      a.r64[i] = ri;
+      rt += ri * ri;
+    }
+    store_partial(mslot(a, X_RT), rt, sh);
   }
-}
-
-template <int FMT, class VT>
-__global__ __launch_bounds__(256) void k_bmix_resid(BMArgs<VT> a, int par) {
-  __shared__ double sh[8];
-  const BGroup G = a.groups[blockIdx.x];
-  const BMState *st = a.st + G.prob;
-  if (st->done || !st->refine[par]) return;
-  double rr = 0.0, rz = 0.0;
-  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-    const double ah = apply64<FMT>(a.m, a.x, i, a.x[i]);
-    const double al = apply64<FMT>(a.m, a.xlo, i, a.xlo[i]);
-    const double ri = (a.b[i] - ah) - al;
-    a.r64[i] = ri;
-    rr += ri * ri;
-    rz += ri * (a.dinv[i] * ri);
-  }
-  store_partial(mslot(a, MB_RR64), rr, sh);
-  store_partial(mslot(a, MB_RZ64), rz, sh);
 }
 
 template <class VT>
-__global__ __launch_bounds__(256) void k_bmix_finish(BMArgs<VT> a, int par) {
+__global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
   __shared__ double sh[8];
   const BGroup G = a.groups[blockIdx.x];
   BMState *st = a.st + G.prob;
-  if (st->done || !st->refine[par]) return;
+  if (st->done) return;
   const BProb P = a.probs[G.prob];
   const bool writer = blockIdx.x == (unsigned)G.gb && threadIdx.x == 0;
-  const double rt = seg_reduce(mslot(a, MB_RR64), P.gb, P.ge, sh);
-  const double rz = seg_reduce(mslot(a, MB_RZ64), P.gb, P.ge, sh);
-  const bool conv = rt < st->thr;
-  const bool stop = conv || st->iterp[par ^ 1] >= st->maxit;
-  const double rho = st->rho[par];
-  if (writer) {
-    st->rr = rt;
-    st->refinements += 1;
-    st->last_refine = st->iterp[par ^ 1];
-    if (stop) {
-      st->done = 1;
-      st->converged = conv ? 1 : 0;
-    } else {
-      st->rr_ref = rt;
-      st->rho[par ^ 1] = rz;
+  double rr = 0.0, rz = 0.0;
+  if (!st->refine[par]) {
+    const double rho = seg_reduce(mslot(a, X_RZ + (par ^ 1)), P.gb, P.ge, sh);
+    const double pq = seg_reduce(mslot(a, X_PQ), P.gb, P.ge, sh);
+    const double qdr = seg_reduce(mslot(a, X_QDR), P.gb, P.ge, sh);
+    const double qdq = seg_reduce(mslot(a, X_QDQ), P.gb, P.ge, sh);
+    const double alpha = rho / pq;
+    const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
+    const VT fa = (VT)alpha, fb = (VT)(rho_next / rho);
+    for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+      const VT pi = a.p[i];
+      a.xa[i] += fa * pi;
+      const VT ri = a.r[i] - fa * a.q[i];
+      a.r[i] = ri;
+      const VT zi = (VT)a.dinv32[i] * ri;
+      a.p[i] = zi + fb * pi;
+      rr += (double)ri * (double)ri;
+      rz += (double)ri * (double)zi;
     }
+  } else {
+    const double rt = seg_reduce(mslot(a, X_RT), P.gb, P.ge, sh);
+    const bool conv = rt < st->thr;
+    const int its = st->iterp[par];
+    const bool stop = conv || its >= st->maxit;
+    if (writer) {
+      st->rr = rt;
+      st->refinements += 1;
+      st->last_refine = its;
+      if (stop) {
+        st->done = 1;
+        st->converged = conv ? 1 : 0;
+      } else {
+        st->rr_ref = rt;
+      }
+    }
+    for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+      const double hi = a.x[i], d = (double)a.xa[i];
+      const double sm = hi + d;
+      const double bp = sm - hi;
+      const double err = (hi - (sm - bp)) + (d - bp);
+      a.x[i] = sm;
+      a.xlo[i] += err;
+      a.xa[i] = (VT)0;
+      if (!stop) {
+        const VT ri = (VT)a.r64[i];
+        const VT mi = (VT)a.dinv32[i];
+        a.p[i] += mi * (ri - a.r[i]);
+        a.r[i] = ri;
+        const VT zi = mi * ri;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+      }
+    }
+    if (stop) return;
   }
-  if (stop) return;
-  const VT fb = (VT)(rz / rho);
-  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-    const VT ri = (VT)a.r64[i];
-    a.r[i] = ri;
-    a.p[i] = (VT)a.dinv32[i] * ri + fb * a.p[i];
-  }
+  store_partial(mslot(a, X_RR + par), rr, sh);
+  store_partial(mslot(a, X_RZ + par), rz, sh);
 }
 
 template <class VT>
@@ -1577,11 +1575,8 @@ void launch_bmix_chunk(hipStream_t s, const BMArgs<VT> &a, int count) {
   const dim3 G(a.ngroups), B(256);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    hipLaunchKernelGGL((k_bmix_spmv<FMT, VT>), G, B, 0, s, a);
-    hipLaunchKernelGGL(k_bmix_update<VT>, G, B, 0, s, a, par);
-    hipLaunchKernelGGL(k_bmix_decide<VT>, G, B, 0, s, a, par);
-    hipLaunchKernelGGL((k_bmix_resid<FMT, VT>), G, B, 0, s, a, par);
-    hipLaunchKernelGGL(k_bmix_finish<VT>, G, B, 0, s, a, par);
+    hipLaunchKernelGGL((k_bmx_a<FMT, VT>), G, B, 0, s, a, par);
+    hipLaunchKernelGGL(k_bmx_b<VT>, G, B, 0, s, a, par);
   }
 }
 

@@ -2,7 +2,7 @@
  * lfref.h -- CPU oracle for the LehrFEM++ assembly + solve path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain C++ restatement of the
- * reference's algorithms, used as the checker for the HIP path.  Only the
+ * reference's algorithms, against which the HIP path is tested.  Only the
  * test suite, the smoke check and the CPU-baseline leg of the benchmark may
  * load it.  The product library (lehrfempp_amd/liblfgpu.so) never links,
  * loads or calls anything here.

@@ -254,7 +254,7 @@ def test_spmv_matches_oracle(ctx):
 
 
 def test_pcg_fp64_c1_against_direct_solve(ctx):
-    # configs[0]: -Laplace u = 1, 64x64, zero Dirichlet; SparseLU's role is
+    # -Laplace u = 1, 64x64, zero Dirichlet; SparseLU's role is
     # played by the oracle's banded Cholesky
     m, g, fe, plan, va, b = _c1_system(ctx)
     rp, col, val = plan.csr.download()
