@@ -286,6 +286,25 @@ int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec *b,
                     lfgpu_vec *x, const lfgpu_pcg_opts *opts,
                     lfgpu_pcg_info *infos, lfgpu_pcg_info *summary);
 
+/* ---- error norms ---------------------------------------------------------
+ * Squared L2 and H1-seminorm errors of the finite element function with DOF
+ * vector uh [ndofs] against an exact solution, integrated with the finite
+ * element's quadrature rule: the post-processing of the demo drivers
+ * (examples/ellbvp_linfe/homDir_linfe_demo.cc:179-188), i.e.
+ * IntegrateMeshFunction (fe/fe_tools.h:34-66, :107-125) of
+ * squaredNorm(MeshFunctionFE - u) (fe/mesh_function_fe.h:54-70) and of
+ * squaredNorm(MeshFunctionGradFE - grad u) (fe/mesh_function_grad_fe.h).
+ * exact LFGPU_EXACT_SINSIN: u = sin(pi x) sin(pi y) (homDir_linfe_demo.cc:
+ * 44-52); LFGPU_EXACT_QP: u_qp [ncells x nqp] and grad_qp [ncells x nqp x 2]
+ * hold the exact values at the rule's points.  Cells are summed by a fixed
+ * reduction tree, so totals agree with the reference's cell-by-cell sum to
+ * rounding. */
+#define LFGPU_EXACT_SINSIN 0
+#define LFGPU_EXACT_QP 1
+int lfgpu_error_norms(lfgpu_ctx *ctx, const lfgpu_mesh *m, const lfgpu_fe *fe,
+                      const lfgpu_vec *uh, int exact, const lfgpu_vec *u_qp,
+                      const lfgpu_vec *grad_qp, double *l2sq, double *h1sq);
+
 /* ---- adjoint gradient ----------------------------------------------------
  * For the P1 problem A(theta) u = f with A(theta) = sum_e exp(theta_e) K_e
  * (+ reaction) assembled by the plan, J(u) = 1/2 (u-u_obs)^T M (u-u_obs) and

@@ -92,6 +92,8 @@ SIGNATURES = {
     "lfgpu_batch_destroy": (C.c_int, [_P]),
     "lfgpu_batch_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo),
                                   C.POINTER(PcgInfo)]),
+    "lfgpu_error_norms": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]),
     "lfgpu_adjoint_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "lfgpu_axpby": (C.c_int, [_P, C.c_double, _P, C.c_double, _P]),
     "lfgpu_dot": (C.c_int, [_P, _P, _P, _d]),
@@ -382,6 +384,22 @@ class Batch:
         if self.h:
             lib().lfgpu_batch_destroy(self.h)
             self.h = _P()
+
+
+EXACT_SINSIN, EXACT_QP = 0, 1
+
+
+def error_norms(ctx, mesh: Mesh, fe: FE, uh: Vec, u_qp: Vec | None = None,
+                grad_qp: Vec | None = None):
+    """(L2 error^2, H1-seminorm error^2) of uh (lfgpu_error_norms): against
+    sin(pi x) sin(pi y) without u_qp, else against the given point values."""
+    l2, h1 = C.c_double(), C.c_double()
+    kind = EXACT_SINSIN if u_qp is None else EXACT_QP
+    check(lib().lfgpu_error_norms(ctx.h, mesh.h, fe.h, uh.h, kind,
+                                  None if u_qp is None else u_qp.h,
+                                  None if grad_qp is None else grad_qp.h,
+                                  C.byref(l2), C.byref(h1)))
+    return l2.value, h1.value
 
 
 def adjoint_gradient(ctx, mesh: Mesh, fe: FE, theta: Vec, u: Vec, lam: Vec, grad: Vec):

@@ -193,7 +193,19 @@ __device__ __forceinline__ float apply32(const MatView &m, const float *__restri
     const int32_t w = (m.slice_off[s + 1] - o) >> 6;
     const float *vp = m.val32 + o + l;
     const int32_t *cp = m.s_col + o + l;
-    for (int32_t k = 0; k < w; ++k) acc += (double)vp[64 * k] * ((double)x[cp[64 * k]] - xd);
+    int32_t k = 0;
+    for (; k + 4 <= w; k += 4) {  // four independent gathers in flight
+      const int32_t c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)],
+                    c3 = cp[64 * (k + 3)];
+      const float v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)],
+                  v3 = vp[64 * (k + 3)];
+      const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+      acc += (double)v0 * ((double)x0 - xd);
+      acc += (double)v1 * ((double)x1 - xd);
+      acc += (double)v2 * ((double)x2 - xd);
+      acc += (double)v3 * ((double)x3 - xd);
+    }
+    for (; k < w; ++k) acc += (double)vp[64 * k] * ((double)x[cp[64 * k]] - xd);
   }
   return (float)(acc + (double)m.rowsum32[r] * xd);
 }
@@ -442,19 +454,24 @@ int launch_spmv(hipStream_t s, const lfgpu_solver_cache *c, int32_t n, const dou
 
 // ---------------------------------------------------------------------------
 // PCG sweeps.  A fixed grid of G workgroups; workgroup g owns the contiguous
-// rows [g R, (g+1) R) (R a multiple of 256, so each wavefront covers whole
+// rows [g R, (g+1) R) (R a multiple of the workgroup size, so each wavefront covers whole
 // 64-row slices) and leaves one partial sum per reduced quantity.
 // ---------------------------------------------------------------------------
+// Threads per sweep workgroup: 256 for the FP64 sweeps, 512 for the mixed
+// sweeps (each measured fastest for its kernels on MI355X at 2048^2).
+constexpr int kSB64 = 256;
+constexpr int kSBMix = 512;
+
 struct Sweep {
   int32_t n;
   int32_t rows_per_group;
   int ngroups;
 };
 
-Sweep make_sweep(int32_t n) {
+Sweep make_sweep(int32_t n, int sb) {
   int64_t per = (n + kMaxGrid - 1) / kMaxGrid;
-  per = ((per + 255) / 256) * 256;
-  if (per < 256) per = 256;
+  per = ((per + sb - 1) / sb) * sb;
+  if (per < sb) per = sb;
   Sweep s;
   s.n = n;
   s.rows_per_group = (int32_t)per;
@@ -474,27 +491,16 @@ __device__ __forceinline__ int32_t sweep_chunk(int b, int G) {
   return b < 8 * per ? (b & 7) * per + (b >> 3) : b;
 }
 
-// Sum of the ngroups partials, identical in every workgroup, broadcast to all
-// threads of the workgroup.
-__device__ __forceinline__ double all_reduce(const double *part, int ngroups, double *sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < ngroups; i += 256) v += part[i];
-  v = block_sum<256>(v, sh);
-  __syncthreads();
-  if (threadIdx.x == 0) sh[4] = v;
-  __syncthreads();
-  return sh[4];
-}
-
+// one partial per 256-thread workgroup (batched solver)
 __device__ __forceinline__ void store_partial(double *part, double v, double *sh) {
   v = block_sum<256>(v, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-// Several sums at once over a 256-thread workgroup; every thread ends with
-// the totals (summed in the same order in every workgroup).  sh needs 4*K
-// doubles.
-template <int K>
+// Several sums at once over a sweep workgroup of SB threads; every thread
+// ends with the totals (summed in the same order in every workgroup).  sh
+// needs (SB/64)*K doubles.
+template <int SB, int K>
 __device__ __forceinline__ void block_sum_k(double (&v)[K], double *sh) {
 #pragma unroll
   for (int k = 0; k < K; ++k)
@@ -504,35 +510,39 @@ __device__ __forceinline__ void block_sum_k(double (&v)[K], double *sh) {
   __syncthreads();
   if (l == 0)
 #pragma unroll
-    for (int k = 0; k < K; ++k) sh[4 * k + w] = v[k];
+    for (int k = 0; k < K; ++k) sh[(SB / 64) * k + w] = v[k];
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = ((sh[4 * k] + sh[4 * k + 1]) + sh[4 * k + 2]) + sh[4 * k + 3];
+  for (int k = 0; k < K; ++k) {
+    double t = sh[(SB / 64) * k];
+#pragma unroll
+    for (int q = 1; q < (SB / 64); ++q) t += sh[(SB / 64) * k + q];
+    v[k] = t;
+  }
 }
 
-// Interleaved partials of the FP64 iteration: group g's slots at part[8 g + k].
 // Partial sums are stored slot-major (part[slot * kMaxGrid + group]): the
 // redundant re-reduction at the start of the next sweep reads, per slot, one
 // contiguous run of ngroups doubles in every workgroup.
-template <int K>
+template <int SB, int K>
 __device__ __forceinline__ void store_partials_k(double *part, double (&v)[K], const int (&slots)[K],
                                                  double *sh) {
-  block_sum_k<K>(v, sh);
+  block_sum_k<SB, K>(v, sh);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < K; ++k) part[slots[k] * kMaxGrid + blockIdx.x] = v[k];
 }
 
-template <int K>
+template <int SB, int K>
 __device__ __forceinline__ void reduce_partials_k(const double *part, int ngroups,
                                                   const int (&slots)[K], double (&out)[K],
                                                   double *sh) {
 #pragma unroll
   for (int k = 0; k < K; ++k) out[k] = 0.0;
-  for (int g = threadIdx.x; g < ngroups; g += 256)
+  for (int g = threadIdx.x; g < ngroups; g += SB)
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] += part[slots[k] * kMaxGrid + g];
-  block_sum_k<K>(out, sh);
+  block_sum_k<SB, K>(out, sh);
 }
 
 // partial slots (FP64 mode); P_RR and P_RZ are followed by their parity twin
@@ -553,13 +563,13 @@ struct CgArgs {
 
 // setup: r = b - A x, p = D^-1 r; partials of b.b, r.r, r.D^-1 r (the parity
 // 1 slots, which iteration 0 reads)
-template <int FMT>
-__global__ __launch_bounds__(256) void k_cg_init(CgArgs a, const double *__restrict__ b) {
-  __shared__ double sh[16];
+template <int FMT, int SB>
+__global__ __launch_bounds__(SB) void k_cg_init(CgArgs a, const double *__restrict__ b) {
+  __shared__ double sh[4 * (SB / 64)];
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double bb = 0.0, rr = 0.0, rz = 0.0;
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
     const double xi = a.x[i];
     const double bi = b[i];
     const double ri = bi - apply64<FMT>(a.m, a.x, i, xi);
@@ -572,15 +582,16 @@ __global__ __launch_bounds__(256) void k_cg_init(CgArgs a, const double *__restr
   }
   double v[3] = {bb, rr, rz};
   const int sl[3] = {P_BB, P_RR + 1, P_RZ + 1};
-  store_partials_k<3>(a.part, v, sl, sh);
+  store_partials_k<SB, 3>(a.part, v, sl, sh);
 }
 
 // one workgroup: Eigen's early exits (zero rhs, converged start, maxit 0)
-__global__ __launch_bounds__(256) void k_cg_init_finish(CgArgs a) {
-  __shared__ double sh[16];
+template <int SB>
+__global__ __launch_bounds__(SB) void k_cg_init_finish(CgArgs a) {
+  __shared__ double sh[4 * (SB / 64)];
   double t[3];
   const int sl[3] = {P_BB, P_RR + 1, P_RZ + 1};
-  reduce_partials_k<3>(a.part, a.sw.ngroups, sl, t, sh);
+  reduce_partials_k<SB, 3>(a.part, a.sw.ngroups, sl, t, sh);
   const double bb = t[0], rr = t[1], rz = t[2];
   if (threadIdx.x == 0) {
     PcgState *st = a.st;
@@ -617,14 +628,14 @@ __global__ void k_zero_if_bb0(double *x, int32_t n, const PcgState *st) {
 // FP64 iteration k, sweep 1 (par = k & 1): Eigen's convergence test on r_k
 // (||r_k||^2 left by the previous sweep 2), then q = A p with the partials
 // p.q, q.D^-1 r and q.D^-1 q.
-template <int FMT>
-__global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
-  __shared__ double sh[16];
+template <int FMT, int SB>
+__global__ __launch_bounds__(SB) void k_cg_spmv(CgArgs a, int par) {
+  __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
   if (st->done) return;
   double t[1];
   const int sl_in[1] = {P_RR + (par ^ 1)};
-  reduce_partials_k<1>(a.part, a.sw.ngroups, sl_in, t, sh);
+  reduce_partials_k<SB, 1>(a.part, a.sw.ngroups, sl_in, t, sh);
   const double rr = t[0];
   if (rr < st->thr) {  // converged: the iteration count stays as it is
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -643,7 +654,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double pq = 0.0, qdr = 0.0, qdq = 0.0;
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
     const double pi = a.p[i];
     const double qi = apply64<FMT>(a.m, a.p, i, pi);
     a.q[i] = qi;
@@ -654,7 +665,7 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
   }
   double v[3] = {pq, qdr, qdq};
   const int sl[3] = {P_PQ, P_QDR, P_QDQ};
-  store_partials_k<3>(a.part, v, sl, sh);
+  store_partials_k<SB, 3>(a.part, v, sl, sh);
 }
 
 // FP64 iteration k, sweep 2: alpha = rho_k / (p.q);
@@ -662,13 +673,14 @@ __global__ __launch_bounds__(256) void k_cg_spmv(CgArgs a, int par) {
 //   expansion of (r - alpha q, D^-1 (r - alpha q)); beta = rho_{k+1} / rho_k;
 //   x += alpha p ; r -= alpha q ; p = D^-1 r + beta p ;
 //   partials ||r||^2 and (r, D^-1 r) of the new residual (parity par).
-__global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
-  __shared__ double sh[16];
+template <int SB>
+__global__ __launch_bounds__(SB) void k_cg_update(CgArgs a, int par) {
+  __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
   if (st->done) return;
   double t[4];
   const int sl_in[4] = {P_RZ + (par ^ 1), P_PQ, P_QDR, P_QDQ};
-  reduce_partials_k<4>(a.part, a.sw.ngroups, sl_in, t, sh);
+  reduce_partials_k<SB, 4>(a.part, a.sw.ngroups, sl_in, t, sh);
   const double rho = t[0], pq = t[1], qdr = t[2], qdq = t[3];
   const double alpha = rho / pq;
   const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
@@ -676,7 +688,7 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
     const double pi = a.p[i];
     a.x[i] += alpha * pi;
     const double ri = a.r[i] - alpha * a.q[i];
@@ -688,7 +700,7 @@ __global__ __launch_bounds__(256) void k_cg_update(CgArgs a, int par) {
   }
   double v[2] = {rr, rz};
   const int sl[2] = {P_RR + par, P_RZ + par};
-  store_partials_k<2>(a.part, v, sl, sh);
+  store_partials_k<SB, 2>(a.part, v, sl, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -753,13 +765,13 @@ __device__ __forceinline__ double apply64_fn(const MatView &m, int32_t r, double
 }
 
 // setup 1: r = b - A x into r[1]; p = 0, s[1] = 0; partials b.b
-template <int FMT>
-__global__ __launch_bounds__(256) void k_cg1_init_r(Cg1Args a, const double *__restrict__ b) {
-  __shared__ double sh[16];
+template <int FMT, int SB>
+__global__ __launch_bounds__(SB) void k_cg1_init_r(Cg1Args a, const double *__restrict__ b) {
+  __shared__ double sh[4 * (SB / 64)];
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double bb = 0.0;
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
     const double bi = b[i];
     a.r[1][i] = bi - apply64<FMT>(a.m, a.x, i, a.x[i]);
     a.p[i] = 0.0;
@@ -768,19 +780,19 @@ __global__ __launch_bounds__(256) void k_cg1_init_r(Cg1Args a, const double *__r
   }
   double v[1] = {bb};
   const int sl[1] = {P_BB};
-  store_partials_k<1>(a.part, v, sl, sh);
+  store_partials_k<SB, 1>(a.part, v, sl, sh);
 }
 
 // setup 2: w = A D^-1 r into w[1]; partials r.r, r.u, w.u (parity 1)
-template <int FMT>
-__global__ __launch_bounds__(256) void k_cg1_init_w(Cg1Args a) {
-  __shared__ double sh[16];
+template <int FMT, int SB>
+__global__ __launch_bounds__(SB) void k_cg1_init_w(Cg1Args a) {
+  __shared__ double sh[4 * (SB / 64)];
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   const double *__restrict__ rv = a.r[1];
   const double *__restrict__ dinv = a.dinv;
   double rr = 0.0, gam = 0.0, dlt = 0.0;
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
     const double ri = rv[i];
     const double ui = dinv[i] * ri;
     const double wi =
@@ -792,15 +804,16 @@ __global__ __launch_bounds__(256) void k_cg1_init_w(Cg1Args a) {
   }
   double v[3] = {rr, gam, dlt};
   const int sl[3] = {C_RR + 1, C_G + 1, C_D + 1};
-  store_partials_k<3>(a.part, v, sl, sh);
+  store_partials_k<SB, 3>(a.part, v, sl, sh);
 }
 
 // one workgroup: Eigen's early exits
-__global__ __launch_bounds__(256) void k_cg1_init_finish(Cg1Args a) {
-  __shared__ double sh[16];
+template <int SB>
+__global__ __launch_bounds__(SB) void k_cg1_init_finish(Cg1Args a) {
+  __shared__ double sh[4 * (SB / 64)];
   double t[2];
   const int sl[2] = {P_BB, C_RR + 1};
-  reduce_partials_k<2>(a.part, a.sw.ngroups, sl, t, sh);
+  reduce_partials_k<SB, 2>(a.part, a.sw.ngroups, sl, t, sh);
   const double bb = t[0], rr = t[1];
   if (threadIdx.x == 0) {
     PcgState *st = a.st;
@@ -828,14 +841,14 @@ __global__ __launch_bounds__(256) void k_cg1_init_finish(Cg1Args a) {
 // partials, writes parity par.  The stopping test and the iteration count
 // follow k_cg_spmv: a converged residual leaves the count as it is, and the
 // maxit-th update is not applied.
-template <int FMT>
-__global__ __launch_bounds__(256) void k_cg1(Cg1Args a, int par) {
-  __shared__ double sh[16];
+template <int FMT, int SB>
+__global__ __launch_bounds__(SB) void k_cg1(Cg1Args a, int par) {
+  __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
   if (st->done) return;
   double t[3];
   const int sl_in[3] = {C_RR + (par ^ 1), C_G + (par ^ 1), C_D + (par ^ 1)};
-  reduce_partials_k<3>(a.part, a.sw.ngroups, sl_in, t, sh);
+  reduce_partials_k<SB, 3>(a.part, a.sw.ngroups, sl_in, t, sh);
   const double rr = t[0], gam = t[1], dlt = t[2];
   const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
   const int itp = st->iterp[par ^ 1];
@@ -876,7 +889,7 @@ __global__ __launch_bounds__(256) void k_cg1(Cg1Args a, int par) {
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr2 = 0.0, gam2 = 0.0, dlt2 = 0.0;
-  for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
     const double di = dinv[i];
     const double ri = ro[i];
     const double pi = di * ri + beta * a.p[i];
@@ -897,13 +910,14 @@ __global__ __launch_bounds__(256) void k_cg1(Cg1Args a, int par) {
   }
   double v[3] = {rr2, gam2, dlt2};
   const int sl[3] = {C_RR + par, C_G + par, C_D + par};
-  store_partials_k<3>(a.part, v, sl, sh);
+  store_partials_k<SB, 3>(a.part, v, sl, sh);
 }
 
 template <int FMT>
 void launch_cg1_chunk(hipStream_t s, const Sweep &sw, const Cg1Args &a, int count) {
-  const dim3 G(sw.ngroups), B(256);
-  for (int k = 0; k < count; ++k) hipLaunchKernelGGL(k_cg1<FMT>, G, B, 0, s, a, k & 1);
+  const dim3 G(sw.ngroups), B(kSB64);
+  for (int k = 0; k < count; ++k)
+    hipLaunchKernelGGL((k_cg1<FMT, kSB64>), G, B, 0, s, a, k & 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -969,7 +983,19 @@ __device__ __forceinline__ double apply_mixed(const MatView &m, const double *__
     const int32_t w = (m.slice_off[s + 1] - o) >> 6;
     const float *vp = m.val32 + o + l;
     const int32_t *cp = m.s_col + o + l;
-    for (int32_t k = 0; k < w; ++k) acc += (double)vp[64 * k] * (x[cp[64 * k]] - xr);
+    int32_t k = 0;
+    for (; k + 4 <= w; k += 4) {  // four independent gathers in flight
+      const int32_t c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)],
+                    c3 = cp[64 * (k + 3)];
+      const float v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)],
+                  v3 = vp[64 * (k + 3)];
+      const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+      acc += (double)v0 * (x0 - xr);
+      acc += (double)v1 * (x1 - xr);
+      acc += (double)v2 * (x2 - xr);
+      acc += (double)v3 * (x3 - xr);
+    }
+    for (; k < w; ++k) acc += (double)vp[64 * k] * (x[cp[64 * k]] - xr);
   }
   return acc + (double)m.rowsum32[r] * xr;
 }
@@ -993,14 +1019,14 @@ __device__ __forceinline__ double apply_mixed(const MatView &m, const double *__
 // maxit.  iterp[par] counts the CG steps up to and including slot par.
 constexpr int X_PQ = 0, X_QDR = 1, X_QDQ = 2, X_RT = 0, X_RR = 3, X_RZ = 5;
 
-template <int FMT, class VT>
-__global__ __launch_bounds__(256) void k_mx_a(MixArgs<VT> a, int par) {
-  __shared__ double sh[16];
+template <int FMT, class VT, int SB>
+__global__ __launch_bounds__(SB) void k_mx_a(MixArgs<VT> a, int par) {
+  __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
   if (st->done) return;
   double t[1];
   const int sl_in[1] = {X_RR + (par ^ 1)};
-  reduce_partials_k<1>(a.part, a.sw.ngroups, sl_in, t, sh);
+  reduce_partials_k<SB, 1>(a.part, a.sw.ngroups, sl_in, t, sh);
   const double rr = t[0];
   const int itp = st->iterp[par ^ 1];
   const bool refine = rr < st->thr || rr < a.delta2 * st->rr_ref ||
@@ -1014,7 +1040,7 @@ __global__ __launch_bounds__(256) void k_mx_a(MixArgs<VT> a, int par) {
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   if (!refine) {
     double pq = 0.0, qdr = 0.0, qdq = 0.0;
-    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
       const VT pi = a.p[i];
       const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
       a.q[i] = qi;
@@ -1025,11 +1051,11 @@ __global__ __launch_bounds__(256) void k_mx_a(MixArgs<VT> a, int par) {
     }
     double v[3] = {pq, qdr, qdq};
     const int sl[3] = {X_PQ, X_QDR, X_QDQ};
-    store_partials_k<3>(a.part, v, sl, sh);
+    store_partials_k<SB, 3>(a.part, v, sl, sh);
   } else {
     double rt = 0.0;
     const VT *__restrict__ xa = a.xa;
-    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
       const double ah = apply64<FMT>(a.m, a.x, i, a.x[i]);
       const double aa =
           apply64_fn<FMT>(a.m, i, (double)xa[i], [&](int32_t j) { return (double)xa[j]; });
@@ -1040,13 +1066,13 @@ __global__ __launch_bounds__(256) void k_mx_a(MixArgs<VT> a, int par) {
     }
     double v[1] = {rt};
     const int sl[1] = {X_RT};
-    store_partials_k<1>(a.part, v, sl, sh);
+    store_partials_k<SB, 1>(a.part, v, sl, sh);
   }
 }
 
-template <class VT>
-__global__ __launch_bounds__(256) void k_mx_b(MixArgs<VT> a, int par) {
-  __shared__ double sh[16];
+template <class VT, int SB>
+__global__ __launch_bounds__(SB) void k_mx_b(MixArgs<VT> a, int par) {
+  __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
   if (st->done) return;
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
@@ -1055,12 +1081,12 @@ __global__ __launch_bounds__(256) void k_mx_b(MixArgs<VT> a, int par) {
   if (!st->refine[par]) {
     double t[4];
     const int sl_in[4] = {X_RZ + (par ^ 1), X_PQ, X_QDR, X_QDQ};
-    reduce_partials_k<4>(a.part, a.sw.ngroups, sl_in, t, sh);
+    reduce_partials_k<SB, 4>(a.part, a.sw.ngroups, sl_in, t, sh);
     const double rho = t[0], pq = t[1], qdr = t[2], qdq = t[3];
     const double alpha = rho / pq;
     const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
     const VT fa = (VT)alpha, fb = (VT)(rho_next / rho);
-    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
       const VT pi = a.p[i];
       a.xa[i] += fa * pi;
       const VT ri = a.r[i] - fa * a.q[i];
@@ -1073,7 +1099,7 @@ __global__ __launch_bounds__(256) void k_mx_b(MixArgs<VT> a, int par) {
   } else {
     double t[1];
     const int sl_in[1] = {X_RT};
-    reduce_partials_k<1>(a.part, a.sw.ngroups, sl_in, t, sh);
+    reduce_partials_k<SB, 1>(a.part, a.sw.ngroups, sl_in, t, sh);
     const double rt = t[0];
     const bool conv = rt < st->thr;
     const int its = st->iterp[par];
@@ -1089,7 +1115,7 @@ __global__ __launch_bounds__(256) void k_mx_b(MixArgs<VT> a, int par) {
         st->rr_ref = rt;
       }
     }
-    for (int32_t i = r0 + threadIdx.x; i < r1; i += 256) {
+    for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
       const double hi = a.x[i], d = (double)a.xa[i];
       const double sm = hi + d;
       const double bp = sm - hi;
@@ -1111,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_mx_b(MixArgs<VT> a, int par) {
   }
   double v[2] = {rr, rz};
   const int sl[2] = {X_RR + par, X_RZ + par};
-  store_partials_k<2>(a.part, v, sl, sh);
+  store_partials_k<SB, 2>(a.part, v, sl, sh);
 }
 
 template <class VT>
@@ -1122,21 +1148,21 @@ __global__ __launch_bounds__(256) void k_mix_final(MixArgs<VT> a) {
 
 template <int FMT>
 void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count) {
-  const dim3 G(sw.ngroups), B(256);
+  const dim3 G(sw.ngroups), B(kSB64);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    hipLaunchKernelGGL(k_cg_spmv<FMT>, G, B, 0, s, a, par);
-    hipLaunchKernelGGL(k_cg_update, G, B, 0, s, a, par);
+    hipLaunchKernelGGL((k_cg_spmv<FMT, kSB64>), G, B, 0, s, a, par);
+    hipLaunchKernelGGL(k_cg_update<kSB64>, G, B, 0, s, a, par);
   }
 }
 
 template <int FMT, class VT>
 void launch_mix_chunk(hipStream_t s, const Sweep &sw, const MixArgs<VT> &m, int count) {
-  const dim3 G(sw.ngroups), B(256);
+  const dim3 G(sw.ngroups), B(kSBMix);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    hipLaunchKernelGGL((k_mx_a<FMT, VT>), G, B, 0, s, m, par);
-    hipLaunchKernelGGL(k_mx_b<VT>, G, B, 0, s, m, par);
+    hipLaunchKernelGGL((k_mx_a<FMT, VT, kSBMix>), G, B, 0, s, m, par);
+    hipLaunchKernelGGL((k_mx_b<VT, kSBMix>), G, B, 0, s, m, par);
   }
 }
 
@@ -1903,15 +1929,18 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   lfgpu_solver_cache *c = A->cache;
   hipStream_t s = ctx->stream;
   const int32_t n = A->nrows;
-  const Sweep sw = make_sweep(n);
-  const dim3 G(sw.ngroups), B(256);
+  const bool single = opts->mode == LFGPU_PCG_FP64 && opts->single_reduction;
+  // FP64 sweeps (and the set-up they share with the mixed modes) run on the
+  // workgroup shape of the iteration that follows
+  const int sb = mixed ? kSBMix : kSB64;
+  const Sweep sw = make_sweep(n, sb);
+  const dim3 G(sw.ngroups), BF(256);
   const unsigned gflat = grid_for(n, 256);
   // an even number of iterations per check keeps the launch parity aligned
   // with the iteration count
   int check = opts->check_every > 0 ? opts->check_every : 64;
   check += check & 1;
 
-  const bool single = opts->mode == LFGPU_PCG_FP64 && opts->single_reduction;
   if (single && !c->cg1)
     LFG_TRY(dalloc(&c->cg1, (size_t)6 * n, "single-reduction PCG vectors"));
   PcgState h{};
@@ -1936,15 +1965,16 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
     a1.part = c->part;
     a1.st = c->st;
     a1.sw = sw;
+    const dim3 B(kSB64);
     if (c->fmt == FMT_DIA) {
-      hipLaunchKernelGGL(k_cg1_init_r<FMT_DIA>, G, B, 0, s, a1, b->d);
-      hipLaunchKernelGGL(k_cg1_init_w<FMT_DIA>, G, B, 0, s, a1);
+      hipLaunchKernelGGL((k_cg1_init_r<FMT_DIA, kSB64>), G, B, 0, s, a1, b->d);
+      hipLaunchKernelGGL((k_cg1_init_w<FMT_DIA, kSB64>), G, B, 0, s, a1);
     } else {
-      hipLaunchKernelGGL(k_cg1_init_r<FMT_ELL>, G, B, 0, s, a1, b->d);
-      hipLaunchKernelGGL(k_cg1_init_w<FMT_ELL>, G, B, 0, s, a1);
+      hipLaunchKernelGGL((k_cg1_init_r<FMT_ELL, kSB64>), G, B, 0, s, a1, b->d);
+      hipLaunchKernelGGL((k_cg1_init_w<FMT_ELL, kSB64>), G, B, 0, s, a1);
     }
-    hipLaunchKernelGGL(k_cg1_init_finish, dim3(1), B, 0, s, a1);
-    hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), B, 0, s, x->d, n, c->st);
+    hipLaunchKernelGGL(k_cg1_init_finish<kSB64>, dim3(1), B, 0, s, a1);
+    hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, c->st);
     LFG_HIP(hipGetLastError());
     for (;;) {
       if (c->fmt == FMT_DIA)
@@ -1962,12 +1992,22 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   // its true residual from
   double *r_init = opts->mode == LFGPU_PCG_MIXED_MATRIX ? c->r64 : c->r;
   CgArgs a{mv, c->dinv, x->d, r_init, c->p, c->q, c->part, c->st, sw};
-  if (c->fmt == FMT_DIA)
-    hipLaunchKernelGGL(k_cg_init<FMT_DIA>, G, B, 0, s, a, b->d);
-  else
-    hipLaunchKernelGGL(k_cg_init<FMT_ELL>, G, B, 0, s, a, b->d);
-  hipLaunchKernelGGL(k_cg_init_finish, dim3(1), B, 0, s, a);
-  hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), B, 0, s, x->d, n, c->st);
+  if (mixed) {
+    const dim3 B(kSBMix);
+    if (c->fmt == FMT_DIA)
+      hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSBMix>), G, B, 0, s, a, b->d);
+    else
+      hipLaunchKernelGGL((k_cg_init<FMT_ELL, kSBMix>), G, B, 0, s, a, b->d);
+    hipLaunchKernelGGL(k_cg_init_finish<kSBMix>, dim3(1), B, 0, s, a);
+  } else {
+    const dim3 B(kSB64);
+    if (c->fmt == FMT_DIA)
+      hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSB64>), G, B, 0, s, a, b->d);
+    else
+      hipLaunchKernelGGL((k_cg_init<FMT_ELL, kSB64>), G, B, 0, s, a, b->d);
+    hipLaunchKernelGGL(k_cg_init_finish<kSB64>, dim3(1), B, 0, s, a);
+  }
+  hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, c->st);
   LFG_HIP(hipGetLastError());
 
   const double delta2 =

@@ -742,6 +742,59 @@ int64_t lfref_cpu_path_assemble(int degree, int ncells, const double *xy,
                                  Vv.data(), (int32_t)ndofs, row_ptr, col, val);
 }
 
+int lfref_error_norms(int degree, int ncells, const double *xy,
+                      const int32_t *cell_v, const int64_t *cell_dofs,
+                      const double *uh, int exact_kind, const double *u_qp,
+                      const double *g_qp, double *out) {
+  FeTables t;
+  if (!make_tables(degree, t)) return -1000;
+  const int n = t.nsf, q = t.nqp;
+  double l2 = 0.0, h1 = 0.0;
+  std::vector<double> u(n);
+  for (int c = 0; c < ncells; ++c) {
+    const double *p0 = P(xy, cell_v[3 * c]), *p1 = P(xy, cell_v[3 * c + 1]),
+                 *p2 = P(xy, cell_v[3 * c + 2]);
+    if (!non_degenerate(p0, p1, p2)) return -(1 + c);
+    Geo g;
+    tria_o1(p0, p1, p2, g);
+    for (int i = 0; i < n; ++i) u[i] = uh[cell_dofs[(int64_t)c * n + i]];
+    double cl2 = 0.0, ch1 = 0.0;
+    for (int k = 0; k < q; ++k) {
+      // MeshFunctionFE: local_dofs * N;  MeshFunctionGradFE: J^{-T} (local_dofs * grad N)
+      double v = 0.0, r0 = 0.0, r1 = 0.0;
+      for (int i = 0; i < n; ++i) {
+        v += u[i] * t.N[n * k + i];
+        r0 += u[i] * t.dN[n * (2 * k) + i];
+        r1 += u[i] * t.dN[n * (2 * k + 1) + i];
+      }
+      const double g0 = g.JinvT[0][0] * r0 + g.JinvT[0][1] * r1;
+      const double g1 = g.JinvT[1][0] * r0 + g.JinvT[1][1] * r1;
+      double ue, ge0, ge1;
+      if (exact_kind == 0) {
+        double x[2];
+        tria_global(p0, p1, p2, t.rule.pts[2 * k], t.rule.pts[2 * k + 1], x);
+        const double sx = std::sin(M_PI * x[0]), sy = std::sin(M_PI * x[1]);
+        ue = sx * sy;
+        ge0 = M_PI * (std::cos(M_PI * x[0]) * sy);
+        ge1 = M_PI * (sx * std::cos(M_PI * x[1]));
+      } else {
+        ue = u_qp[(int64_t)c * q + k];
+        ge0 = g_qp[((int64_t)c * q + k) * 2];
+        ge1 = g_qp[((int64_t)c * q + k) * 2 + 1];
+      }
+      const double wk = t.rule.w[k] * g.det;
+      const double d = v - ue, e0 = g0 - ge0, e1 = g1 - ge1;
+      cl2 += (d * d) * wk;
+      ch1 += (e0 * e0 + e1 * e1) * wk;
+    }
+    l2 += cl2;
+    h1 += ch1;
+  }
+  out[0] = l2;
+  out[1] = h1;
+  return 0;
+}
+
 int lfref_adjoint_gradient_p1(int ncells, const double *xy,
                               const int32_t *cell_v, const double *theta,
                               const double *u, const double *lam,

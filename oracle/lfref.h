@@ -170,6 +170,22 @@ int lfref_adjoint_gradient_p1(int ncells, const double *xy,
                               const double *u, const double *lam,
                               double *grad);
 
+/* Squared error norms of a finite element function u_h (DOF vector uh) against
+ * an exact solution, as the demo drivers compute them
+ * (examples/ellbvp_linfe/homDir_linfe_demo.cc:179-188):
+ *   out[0] = IntegrateMeshFunction(mesh, squaredNorm(MeshFunctionFE - u), rule)
+ *   out[1] = the same for MeshFunctionGradFE - grad u
+ * (fe/fe_tools.h:34-66 LocalIntegral, :107-125; fe/mesh_function_fe.h:54-70;
+ * fe/mesh_function_grad_fe.h operator()), with the finite element's own
+ * quadrature rule.  Per cell: values at the points, sum_k value_k (w_k |det J|),
+ * cells added in index order.  exact_kind 0: u = sin(pi x) sin(pi y) and its
+ * gradient (homDir_linfe_demo.cc:44-52); 1: u_qp [ncells x nqp] and
+ * g_qp [ncells x nqp x 2] given. */
+int lfref_error_norms(int degree, int ncells, const double *xy,
+                      const int32_t *cell_v, const int64_t *cell_dofs,
+                      const double *uh, int exact_kind, const double *u_qp,
+                      const double *g_qp, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,8 @@ def lib():
                                                     C.c_int64, _d, C.c_double, C.c_int,
                                                     C.c_double, _u8, _i32, _i32, _d, _d]),
             "lfref_adjoint_gradient_p1": (C.c_int, [C.c_int, _d, _i32, _d, _d, _d, _d]),
+            "lfref_error_norms": (C.c_int, [C.c_int, C.c_int, _d, _i32, _i64, _d, C.c_int,
+                                            C.c_void_p, C.c_void_p, _d]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -316,6 +318,25 @@ def adjoint_gradient_p1(mesh: TriMesh, theta, u, lam):
                                          np.ascontiguousarray(lam, np.float64), g)
     assert rc == 0
     return g
+
+
+def error_norms(mesh: TriMesh, degree: int, uh, u_qp=None, g_qp=None):
+    """Squared L2 and H1-seminorm errors of the FE function uh (lfref_error_norms):
+    against u = sin(pi x) sin(pi y) when u_qp is None, else against the given
+    per-point values u_qp [ne x nqp] and gradients g_qp [ne x nqp x 2]."""
+    cd, nd = mesh.dofs(degree)
+    out = np.zeros(2)
+    if u_qp is None:
+        rc = lib().lfref_error_norms(degree, mesh.ne, mesh.xy.ravel(), mesh.cells.ravel(), cd,
+                                     np.ascontiguousarray(uh, np.float64), 0, None, None, out)
+    else:
+        uq = np.ascontiguousarray(u_qp, np.float64)
+        gq = np.ascontiguousarray(g_qp, np.float64)
+        rc = lib().lfref_error_norms(degree, mesh.ne, mesh.xy.ravel(), mesh.cells.ravel(), cd,
+                                     np.ascontiguousarray(uh, np.float64), 1,
+                                     uq.ctypes.data, gq.ctypes.data, out)
+    assert rc == 0, rc
+    return out[0], out[1]
 
 
 def galerkin_system(mesh: TriMesh, degree: int, alpha, gamma, f_kind=F_CONST, f_const=1.0,

@@ -757,3 +757,57 @@ def test_mfma_assembly_matches_exact_assembly(ctx, degree, n):
     assert np.abs(x.numpy() - xo).max() <= 1e-8 * np.abs(xo).max()
     for h in (gv, x, b1, b2, th, plan, fe, g):
         h.close()
+
+
+# ------------------------------------------------------------- error norms
+@pytest.mark.parametrize("degree", [1, 3])
+def test_error_norms_match_oracle(ctx, degree):
+    # lfgpu_error_norms vs the oracle's cell-by-cell IntegrateMeshFunction
+    # restatement; the sums differ only in summation order (tolerance 1e-12)
+    m = jittered_mesh(20 if degree == 1 else 9, seed=8)
+    cd, nd = m.dofs(degree)
+    g = gpu_mesh(ctx, m, degree)
+    fe = capi.FE(ctx, degree)
+    rng = np.random.default_rng(5)
+    uh = rng.standard_normal(nd) * 0.1
+    vu = capi.Vec(ctx, uh)
+    l2, h1 = capi.error_norms(ctx, g, fe, vu)
+    l2o, h1o = L.error_norms(m, degree, uh)
+    assert abs(l2 - l2o) <= 1e-12 * l2o and abs(h1 - h1o) <= 1e-12 * h1o
+    q = fe.nqp
+    uq = rng.standard_normal((m.ne, q))
+    gq = rng.standard_normal((m.ne, q, 2))
+    vq, vg = capi.Vec(ctx, uq), capi.Vec(ctx, gq)
+    l2, h1 = capi.error_norms(ctx, g, fe, vu, vq, vg)
+    l2o, h1o = L.error_norms(m, degree, uh, uq, gq)
+    assert abs(l2 - l2o) <= 1e-12 * l2o and abs(h1 - h1o) <= 1e-12 * h1o
+    # size errors are rejected
+    with pytest.raises(capi.LfgpuError):
+        capi.error_norms(ctx, g, fe, vu, vq, vq)
+    for h in (vq, vg, vu, fe, g):
+        h.close()
+
+
+def test_manufactured_solution_errors_on_gpu(ctx):
+    # the homDir_linfe_demo.cc:114-188 loop end to end on the device: assemble
+    # -Laplace u = 2 pi^2 sin sin, solve, integrate the errors; L2 O(h^2),
+    # H1-seminorm O(h), and the numbers equal the oracle pipeline's
+    errs = []
+    for n in (16, 32, 64):
+        m = L.tp_triag_mesh(n)
+        g = gpu_mesh(ctx, m, 1)
+        fe = capi.FE(ctx, 1)
+        plan = capi.Plan(ctx, g, m.dirichlet_flags(1))
+        b, x = capi.Vec(ctx, m.nv), capi.Vec(ctx, m.nv)
+        plan.assemble(fe, capi.coeff(capi.COEFF_CONST, 1.0), capi.coeff(capi.COEFF_CONST, 0.0),
+                      capi.coeff(capi.COEFF_SINSIN, 2 * np.pi ** 2), b)
+        capi.pcg(ctx, plan.csr, b, x, rtol=1e-13)
+        e = np.sqrt(capi.error_norms(ctx, g, fe, x))
+        eo = np.sqrt(L.error_norms(m, 1, x.numpy()))
+        assert np.allclose(e, eo, rtol=1e-12)
+        errs.append(e)
+        for h in (b, x, plan, fe, g):
+            h.close()
+    e = np.array(errs)
+    r = e[:-1] / e[1:]
+    assert np.all((r[:, 0] > 3.7) & (r[:, 0] < 4.3)) and np.all((r[:, 1] > 1.9) & (r[:, 1] < 2.1)), r

@@ -389,3 +389,79 @@ def test_adjoint_gradient_matches_finite_differences():
         eps = 1e-6
         fd = (J(theta + eps * d) - J(theta - eps * d)) / (2 * eps)
         assert abs(fd - g @ d) < 1e-6 * max(1.0, abs(fd))
+
+
+# ------------------------------------------------------------- error norms
+def _interpolant(m, degree, fn):
+    """DOF vector of the nodal interpolant of fn (every cell writes the values
+    at its evaluation nodes, lagr_fe.h:1115-1133; a consistent DOF map makes
+    the writes agree)."""
+    cd, nd = m.dofs(degree)
+    nodes = np.array([[0, 0], [1, 0], [0, 1]]) if degree == 1 else _p3_nodes()
+    u = np.full(nd, np.nan)
+    p0, p1, p2 = m.xy[m.cells[:, 0]], m.xy[m.cells[:, 1]], m.xy[m.cells[:, 2]]
+    for i, (a, b) in enumerate(nodes):
+        x = p0 * (1 - a - b) + p1 * a + p2 * b
+        v = fn(x[:, 0], x[:, 1])
+        prev = u[cd[:, i]]
+        ok = np.isnan(prev) | (np.abs(prev - v) <= 1e-12 * (1 + np.abs(v)))
+        assert ok.all(), "DOF map gives one DOF two different nodes"
+        u[cd[:, i]] = v
+    assert not np.isnan(u).any()
+    return u
+
+
+def _exact_qp(m, degree, fn, grad):
+    q = L.qp_coords(m, degree)
+    return fn(q[..., 0], q[..., 1]), np.stack(grad(q[..., 0], q[..., 1]), axis=-1)
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_error_norms_vanish_on_the_fe_space(degree):
+    # a polynomial of the element's degree is reproduced exactly, so both
+    # error integrals vanish (up to rounding); jittered mesh
+    m = L.tp_triag_mesh(9)
+    rng = np.random.default_rng(2)
+    xy = m.xy.copy()
+    inner = (xy[:, 0] > 0) & (xy[:, 0] < 1) & (xy[:, 1] > 0) & (xy[:, 1] < 1)
+    xy[inner] += rng.uniform(-0.03, 0.03, (inner.sum(), 2))
+    m = L.TriMesh(xy, m.cells, m.edges)
+    if degree == 1:
+        fn = lambda x, y: 2 * x + y - 0.5
+        gr = lambda x, y: (2 + 0 * x, 1 + 0 * y)
+    else:
+        fn = lambda x, y: x ** 3 - 2 * x * y * y + y - 1
+        gr = lambda x, y: (3 * x * x - 2 * y * y, -4 * x * y + 1)
+    uh = _interpolant(m, degree, fn)
+    uq, gq = _exact_qp(m, degree, fn, gr)
+    l2, h1 = L.error_norms(m, degree, uh, uq, gq)
+    assert l2 < 1e-26 and h1 < 1e-24, (l2, h1)
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_error_norms_of_zero_are_the_norms_of_u(degree):
+    # u_h = 0: the integrals are the quadrature of |u|^2 = 1/4 and of
+    # |grad u|^2 = pi^2/2 for u = sin(pi x) sin(pi y) on the unit square
+    m = L.tp_triag_mesh(48)
+    cd, nd = m.dofs(degree)
+    l2, h1 = L.error_norms(m, degree, np.zeros(nd))
+    tol = 2e-4 if degree == 1 else 1e-10
+    assert abs(l2 - 0.25) < tol and abs(h1 - np.pi ** 2 / 2) < 20 * tol, (l2, h1)
+
+
+def test_error_norms_convergence_rates():
+    # homDir_linfe_demo.cc:114-188: -Laplace u = 2 pi^2 sin sin, P1, L2 error
+    # O(h^2) and H1-seminorm error O(h)
+    errs = []
+    for n in (16, 32, 64):
+        m = L.tp_triag_mesh(n)
+        q = L.qp_coords(m, 1)
+        f = 2 * np.pi ** 2 * np.sin(np.pi * q[..., 0]) * np.sin(np.pi * q[..., 1])
+        rp, col, val, b = L.galerkin_system(m, 1, 1.0, 0.0, L.F_QP, f_qp=f,
+                                            flags=m.dirichlet_flags(1))
+        u = L.band_cholesky_solve(rp, col, val, b)
+        errs.append(np.sqrt(L.error_norms(m, 1, u)))
+    e = np.array(errs)
+    r = e[:-1] / e[1:]
+    assert np.all((r[:, 0] > 3.7) & (r[:, 0] < 4.3)), r
+    assert np.all((r[:, 1] > 1.9) & (r[:, 1] < 2.1)), r
