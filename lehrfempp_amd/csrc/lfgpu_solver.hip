@@ -1146,13 +1146,13 @@ __global__ __launch_bounds__(256) void k_mix_final(MixArgs<VT> a) {
   if (i < a.sw.n) a.x[i] += a.xlo[i];
 }
 
-template <int FMT>
+template <int FMT, int SB>
 void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count) {
-  const dim3 G(sw.ngroups), B(kSB64);
+  const dim3 G(sw.ngroups), B(SB);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    hipLaunchKernelGGL((k_cg_spmv<FMT, kSB64>), G, B, 0, s, a, par);
-    hipLaunchKernelGGL(k_cg_update<kSB64>, G, B, 0, s, a, par);
+    hipLaunchKernelGGL((k_cg_spmv<FMT, SB>), G, B, 0, s, a, par);
+    hipLaunchKernelGGL(k_cg_update<SB>, G, B, 0, s, a, par);
   }
 }
 
@@ -1930,9 +1930,10 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   hipStream_t s = ctx->stream;
   const int32_t n = A->nrows;
   const bool single = opts->mode == LFGPU_PCG_FP64 && opts->single_reduction;
-  // FP64 sweeps (and the set-up they share with the mixed modes) run on the
-  // workgroup shape of the iteration that follows
-  const int sb = mixed ? kSBMix : kSB64;
+  // the set-up sweep runs on the workgroup shape of the iteration that
+  // follows: 512 threads for the mixed sweeps and for FP64 on sliced ELL,
+  // 256 for FP64 on DIA (and the single-reduction variant)
+  const int sb = (mixed || (c->fmt == FMT_ELL && !single)) ? kSBMix : kSB64;
   const Sweep sw = make_sweep(n, sb);
   const dim3 G(sw.ngroups), BF(256);
   const unsigned gflat = grid_for(n, 256);
@@ -1992,7 +1993,7 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   // its true residual from
   double *r_init = opts->mode == LFGPU_PCG_MIXED_MATRIX ? c->r64 : c->r;
   CgArgs a{mv, c->dinv, x->d, r_init, c->p, c->q, c->part, c->st, sw};
-  if (mixed) {
+  if (sb == kSBMix) {
     const dim3 B(kSBMix);
     if (c->fmt == FMT_DIA)
       hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSBMix>), G, B, 0, s, a, b->d);
@@ -2026,9 +2027,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   } else {
     for (;;) {
       if (c->fmt == FMT_DIA)
-        launch_cg_chunk<FMT_DIA>(s, sw, a, check);
+        launch_cg_chunk<FMT_DIA, kSB64>(s, sw, a, check);
       else
-        launch_cg_chunk<FMT_ELL>(s, sw, a, check);
+        launch_cg_chunk<FMT_ELL, kSBMix>(s, sw, a, check);
       LFG_HIP(hipGetLastError());
       LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
       LFG_HIP(hipStreamSynchronize(s));
