@@ -51,6 +51,7 @@ typedef struct lfgpu_mesh lfgpu_mesh; /* vertex coords + cell->vertex/DOF   */
 typedef struct lfgpu_fe lfgpu_fe;     /* quadrature rule + shape tables     */
 typedef struct lfgpu_plan lfgpu_plan; /* CSR pattern + scatter records      */
 typedef struct lfgpu_csr lfgpu_csr;   /* device CSR matrix                  */
+typedef struct lfgpu_edges lfgpu_edges; /* codim-1 entities + segment FE    */
 
 /* ---- context ----------------------------------------------------------- */
 int lfgpu_ctx_create(int device, lfgpu_ctx **out);
@@ -195,6 +196,49 @@ int lfgpu_assemble_galerkin_mfma(lfgpu_ctx *ctx, lfgpu_plan *p,
                                  const lfgpu_fe *fe, const lfgpu_coeff *alpha,
                                  const lfgpu_coeff *gamma, const lfgpu_coeff *f,
                                  const lfgpu_vec *fixed_vals, lfgpu_vec *rhs);
+
+/* ---- boundary (codim-1) terms -------------------------------------------
+ * MassEdgeMatrixProvider (uscalfe/loc_comp_ellbvp.h:367-527) and
+ * ScalarLoadEdgeVectorProvider (:784-918), assembled with
+ * AssembleMatrixLocally(1, ...) / AssembleVectorLocally(1, ...) after the
+ * cell terms, as the impedance/Neumann part of
+ * examples/ellbvp_linfe/ellbvp_linfe_demo.cc:272-315.
+ * lfgpu_edges_create: edge_v [nedges x 2] the endpoints of every edge in its
+ * orientation (lfgpu_mesh_edges returns them), edge_dofs [nedges x (degree+1)]
+ * the edge's GlobalDofIndices (dofhandler.cc:160-181: endpoint DOFs, then
+ * interior DOFs; NULL for degree 1, where they are the endpoints).  The
+ * segment element is FeLagrangeO1Segment / FeLagrangeO3Segment
+ * (lagr_fe.h:284-330, :1155-1230) with make_QuadRule(kSegment, 2 degree)
+ * (Gauss-Legendre, degree + 1 points; quad/make_quad_rule.cc:13-18). */
+int lfgpu_edges_create(lfgpu_ctx *ctx, const lfgpu_mesh *m, int degree,
+                       int64_t nedges, const int32_t *edge_v,
+                       const int64_t *edge_dofs, lfgpu_edges **out);
+int lfgpu_edges_destroy(lfgpu_edges *e);
+/* MassEdgeMatrixProvider::Eval / ScalarLoadEdgeVectorProvider::Eval for every
+ * edge: out [nedges x nle x nle] row-major, [nedges x nle].  Coefficient
+ * kinds as for cells with "cell" read as "edge": CONST, CELL (per edge), QP
+ * (per edge and point), SINSIN (at SegmentO1::Global of the point), EXPCELL. */
+int lfgpu_edge_matrices(lfgpu_ctx *ctx, const lfgpu_edges *e,
+                        const lfgpu_coeff *eta, lfgpu_vec *out);
+int lfgpu_edge_vectors(lfgpu_ctx *ctx, const lfgpu_edges *e,
+                       const lfgpu_coeff *h, lfgpu_vec *out);
+/* Attach boundary terms to a plan: mat_flags [nedges] selects the edges of
+ * the edge mass term, vec_flags [nedges] those of the edge load term (host
+ * arrays, NULL = no such term) -- the edge selectors of the two providers.
+ * The edge DOFs of a cell mesh couple only DOFs of one cell, so the pattern
+ * is unchanged; the plan gains per-row edge records in edge order. */
+int lfgpu_plan_set_edges(lfgpu_plan *p, const lfgpu_edges *e,
+                         const uint8_t *mat_flags, const uint8_t *vec_flags);
+/* lfgpu_assemble_galerkin plus the plan's edge terms: eta the coefficient
+ * of the edge mass term, h the edge source (either may be NULL when the plan
+ * has no such term).  Triplet order as the reference: cell triplets, then
+ * edge triplets; rhs = cell loads, then edge loads, then the Dirichlet
+ * update over all removed triplets in that order. */
+int lfgpu_assemble_galerkin_bd(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
+                               const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,
+                               const lfgpu_coeff *f, const lfgpu_coeff *eta,
+                               const lfgpu_coeff *h, const lfgpu_vec *fixed_vals,
+                               lfgpu_vec *rhs);
 
 /* ---- CSR matrices ------------------------------------------------------ */
 int lfgpu_csr_info(const lfgpu_csr *A, int32_t *nrows, int32_t *ncols,

@@ -74,6 +74,14 @@ SIGNATURES = {
                                               _P]),
     "lfgpu_element_vectors": (C.c_int, [_P, _P, _P, C.POINTER(Coeff), _P]),
     "lfgpu_plan_create": (C.c_int, [_P, _P, _u8, C.POINTER(_P)]),
+    "lfgpu_edges_create": (C.c_int, [_P, _P, C.c_int, C.c_int64, _i32, _i64, C.POINTER(_P)]),
+    "lfgpu_edges_destroy": (C.c_int, [_P]),
+    "lfgpu_edge_matrices": (C.c_int, [_P, _P, C.POINTER(Coeff), _P]),
+    "lfgpu_edge_vectors": (C.c_int, [_P, _P, C.POINTER(Coeff), _P]),
+    "lfgpu_plan_set_edges": (C.c_int, [_P, _P, _u8, _u8]),
+    "lfgpu_assemble_galerkin_bd": (C.c_int, [_P, _P, _P, C.POINTER(Coeff), C.POINTER(Coeff),
+                                             C.POINTER(Coeff), C.POINTER(Coeff),
+                                             C.POINTER(Coeff), _P, _P]),
     "lfgpu_plan_destroy": (C.c_int, [_P]),
     "lfgpu_plan_csr": (C.c_int, [_P, C.POINTER(_P)]),
     "lfgpu_assemble_galerkin": (C.c_int, [_P, _P, _P, C.POINTER(Coeff), C.POINTER(Coeff),
@@ -293,10 +301,65 @@ class Plan:
                  None if rhs is None else rhs.h))
         return self.ctx.last_ms()
 
+    def set_edges(self, edges, mat_flags=None, vec_flags=None):
+        """lfgpu_plan_set_edges: edge mass term on mat_flags, edge load on vec_flags."""
+        mf = None if mat_flags is None else np.ascontiguousarray(mat_flags, np.uint8)
+        vf = None if vec_flags is None else np.ascontiguousarray(vec_flags, np.uint8)
+        check(lib().lfgpu_plan_set_edges(self.h, edges.h,
+                                         None if mf is None else _ptr(mf, C.c_uint8),
+                                         None if vf is None else _ptr(vf, C.c_uint8)))
+
+    def assemble_bd(self, fe: FE, alpha: Coeff, gamma: Coeff, f: Coeff | None,
+                    eta: Coeff | None, h: Coeff | None, rhs: Vec | None,
+                    fixed_vals: Vec | None = None):
+        """lfgpu_assemble_galerkin_bd: cell terms plus the plan's edge terms."""
+        check(lib().lfgpu_assemble_galerkin_bd(
+            self.ctx.h, self.h, fe.h, C.byref(alpha), C.byref(gamma),
+            None if f is None else C.byref(f), None if eta is None else C.byref(eta),
+            None if h is None else C.byref(h), None if fixed_vals is None else fixed_vals.h,
+            None if rhs is None else rhs.h))
+        return self.ctx.last_ms()
+
     def close(self):
         if self.h:
             lib().lfgpu_plan_destroy(self.h)
             self.h = _P()
+
+
+class Edges:
+    """Codim-1 entities with the segment element (lfgpu_edges_create)."""
+
+    def __init__(self, ctx: Context, mesh: Mesh, degree, edge_v, edge_dofs=None):
+        self.ctx = ctx
+        self.ev = np.ascontiguousarray(edge_v, dtype=np.int32)
+        self.n = self.ev.shape[0]
+        self.nle = degree + 1
+        ed = None if edge_dofs is None else np.ascontiguousarray(edge_dofs, dtype=np.int64)
+        self.h = _P()
+        check(lib().lfgpu_edges_create(ctx.h, mesh.h, degree, self.n, _ptr(self.ev, C.c_int32),
+                                       None if ed is None else _ptr(ed, C.c_int64),
+                                       C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().lfgpu_edges_destroy(self.h)
+            self.h = _P()
+
+
+def edge_matrices(ctx, edges: Edges, eta: Coeff):
+    out = Vec(ctx, edges.n * edges.nle * edges.nle)
+    check(lib().lfgpu_edge_matrices(ctx.h, edges.h, C.byref(eta), out.h))
+    a = out.numpy().reshape(edges.n, edges.nle, edges.nle)
+    out.close()
+    return a
+
+
+def edge_vectors(ctx, edges: Edges, h: Coeff):
+    out = Vec(ctx, edges.n * edges.nle)
+    check(lib().lfgpu_edge_vectors(ctx.h, edges.h, C.byref(h), out.h))
+    a = out.numpy().reshape(edges.n, edges.nle)
+    out.close()
+    return a
 
 
 def element_matrices(ctx, mesh: Mesh, fe: FE, alpha: Coeff, gamma: Coeff, mfma: bool = False):

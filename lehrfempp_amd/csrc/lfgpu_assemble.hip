@@ -162,6 +162,7 @@ extern "C" int lfgpu_plan_destroy(lfgpu_plan *p) {
   dfree(p->fixed);
   dfree(p->ke);
   dfree(p->fe);
+  plan_free_edges(p);
   delete p;
   return LFGPU_OK;
 }
@@ -316,6 +317,13 @@ struct AsmArgs {
   CoeffD alpha, gamma, f;
   int has_f;
   int64_t nrows;
+  // boundary edge terms (null pointers: none)
+  const int32_t *erec_ptr, *erec_key;
+  const uint8_t *erec_meta;
+  const int32_t *vrec_ptr, *vrec_key;
+  const int32_t *edge_v, *edge_dofs;
+  const double *etab;
+  CoeffD eta, h;
 };
 
 template <int NLOC, int MS>
@@ -335,11 +343,29 @@ __device__ __forceinline__ void load_meta(const uint8_t *meta, int32_t p, uint8_
   }
 }
 
-template <int NLOC, int NQP, int MS, int BLOCK>
+// Row i of edge e's mass matrix and its CSR slots (edge record q).
+template <int NLE, int NQE>
+__device__ __forceinline__ void edge_record_row(const AsmArgs &a, const ETab<NLE, NQE> &ET,
+                                                int32_t key, int64_t &e, double *row) {
+  e = key / NLE;
+  const int i = key - (int)e * NLE;
+  const double2 pa = reinterpret_cast<const double2 *>(a.xy)[a.edge_v[2 * e]];
+  const double2 pb = reinterpret_cast<const double2 *>(a.xy)[a.edge_v[2 * e + 1]];
+  double gk[NQE];
+#pragma unroll
+  for (int k = 0; k < NQE; ++k) gk[k] = edge_coeff<NQE>(a.eta, e, k, pa, pb, ET.px(k));
+  edge_row<NLE, NQE>(ET, seg_length(pa, pb), gk, i, row);
+}
+
+template <int NLOC, int NQP, int MS, int BLOCK, bool EDGES>
 __global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
+  constexpr int NLE = NLOC == 3 ? 2 : 4, NQE = NLE;
   extern __shared__ double s_acc[];
   __shared__ double s_tab[Tab<NLOC, NQP>::kLen];
+  __shared__ double s_etab[ETab<NLE, NQE>::kLen];
   for (int t = threadIdx.x; t < Tab<NLOC, NQP>::kLen; t += BLOCK) s_tab[t] = a.tab[t];
+  if (EDGES)
+    for (int t = threadIdx.x; t < ETab<NLE, NQE>::kLen; t += BLOCK) s_etab[t] = a.etab[t];
   const int64_t r0 = (int64_t)blockIdx.x * BLOCK;
   const int64_t r1 = r0 + BLOCK < a.nrows ? r0 + BLOCK : a.nrows;
   const int32_t base = a.row_ptr[r0];
@@ -381,6 +407,37 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
           b += element_load<NLOC, NQP>(T, g, fk, i);
         }
       }
+      if constexpr (EDGES) {
+        // AssembleMatrixLocally(1, ...) / AssembleVectorLocally(1, ...) come
+        // after the cell terms: edge triplets and edge loads in edge order
+        const ETab<NLE, NQE> ET{s_etab};
+        if (a.erec_ptr) {
+          for (int32_t q = a.erec_ptr[r]; q < a.erec_ptr[r + 1]; ++q) {
+            int64_t e;
+            double row[NLE];
+            edge_record_row<NLE, NQE>(a, ET, a.erec_key[q], e, row);
+            const uint32_t w = reinterpret_cast<const uint32_t *>(a.erec_meta)[q];
+#pragma unroll
+            for (int j = 0; j < NLE; ++j) {
+              const uint8_t sl = (uint8_t)(w >> (8 * j));
+              if (sl != kDrop) s_acc[off + sl] += row[j];
+            }
+          }
+        }
+        if (a.vrec_ptr) {
+          for (int32_t q = a.vrec_ptr[r]; q < a.vrec_ptr[r + 1]; ++q) {
+            const int32_t key = a.vrec_key[q];
+            const int64_t e = key / NLE;
+            const int i = key - (int)e * NLE;
+            const double2 pa = reinterpret_cast<const double2 *>(a.xy)[a.edge_v[2 * e]];
+            const double2 pb = reinterpret_cast<const double2 *>(a.xy)[a.edge_v[2 * e + 1]];
+            double hk[NQE];
+#pragma unroll
+            for (int k = 0; k < NQE; ++k) hk[k] = edge_coeff<NQE>(a.h, e, k, pa, pb, ET.px(k));
+            b += edge_load<NLE, NQE>(ET, seg_length(pa, pb), hk, i);
+          }
+        }
+      }
       if (a.fixed_vals) {
         // b += A(r, col) * (-1 * x_fixed[col]) over the removed triplets, in
         // triplet order (fix_dof.h:97-106 via COOMatrix::MatVecMult).
@@ -406,6 +463,25 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
             if (mt[1 + j] == kDrop)
               b += row[j] * (-1.0 * a.fixed_vals[a.cell_dofs[cell * NLOC + j]]);
         }
+        if constexpr (EDGES) {
+          if (a.erec_ptr) {
+            const ETab<NLE, NQE> ET{s_etab};
+            for (int32_t q = a.erec_ptr[r]; q < a.erec_ptr[r + 1]; ++q) {
+              const uint32_t w = reinterpret_cast<const uint32_t *>(a.erec_meta)[q];
+              bool any = false;
+#pragma unroll
+              for (int j = 0; j < NLE; ++j) any |= ((uint8_t)(w >> (8 * j)) == kDrop);
+              if (!any) continue;
+              int64_t e;
+              double row[NLE];
+              edge_record_row<NLE, NQE>(a, ET, a.erec_key[q], e, row);
+#pragma unroll
+              for (int j = 0; j < NLE; ++j)
+                if ((uint8_t)(w >> (8 * j)) == kDrop)
+                  b += row[j] * (-1.0 * a.fixed_vals[a.edge_dofs[e * NLE + j]]);
+            }
+          }
+        }
       }
       if (a.rhs) a.rhs[r] = b;
     }
@@ -414,19 +490,22 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
   for (int t = threadIdx.x; t < cnt; t += BLOCK) a.val[base + t] = s_acc[t];
 }
 
-template <int NLOC, int NQP, int MS>
+template <int NLOC, int NQP, int MS, bool EDGES>
 int launch_assemble(hipStream_t s, const lfgpu_plan *p, const AsmArgs &a) {
   const size_t lds = (size_t)std::max(p->max_block_nnz, 1) * sizeof(double);
   const unsigned grid = grid_for(a.nrows, p->block);
   switch (p->block) {
     case 256:
-      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 256>), dim3(grid), dim3(256), lds, s, a);
+      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 256, EDGES>), dim3(grid), dim3(256), lds,
+                         s, a);
       break;
     case 128:
-      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 128>), dim3(grid), dim3(128), lds, s, a);
+      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 128, EDGES>), dim3(grid), dim3(128), lds,
+                         s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 64>), dim3(grid), dim3(64), lds, s, a);
+      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 64, EDGES>), dim3(grid), dim3(64), lds, s,
+                         a);
       break;
   }
   LFG_HIP(hipGetLastError());
@@ -570,18 +649,18 @@ extern "C" int lfgpu_assemble_galerkin_mfma(lfgpu_ctx *ctx, lfgpu_plan *p, const
   return LFGPU_OK;
 }
 
-extern "C" int lfgpu_assemble_galerkin(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
-                                       const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,
-                                       const lfgpu_coeff *f, const lfgpu_vec *fixed_vals,
-                                       lfgpu_vec *rhs) {
-  LFG_REQUIRE(ctx && p && fe, LFGPU_EINVAL, "NULL argument to lfgpu_assemble_galerkin");
+namespace {
+
+// Arguments of the row-owner assembly for a plan; edge pointers null.
+int asm_args(lfgpu_plan *p, const lfgpu_fe *fe, const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,
+             const lfgpu_coeff *f, const lfgpu_vec *fixed_vals, lfgpu_vec *rhs, AsmArgs &a) {
   const lfgpu_mesh *m = p->mesh;
   LFG_REQUIRE(fe->nloc == m->nloc, LFGPU_EINVAL,
               "finite element and DOF layout disagree on the number of local DOFs");
   LFG_REQUIRE(!rhs || rhs->n == m->ndofs, LFGPU_EINVAL, "rhs must have ndofs entries");
   LFG_REQUIRE(!fixed_vals || fixed_vals->n == m->ndofs, LFGPU_EINVAL,
               "fixed_vals must have ndofs entries");
-  AsmArgs a;
+  a = AsmArgs{};
   a.xy = m->xy;
   a.cell_v = m->cell_v;
   a.cell_dofs = m->cell_dofs;
@@ -599,11 +678,16 @@ extern "C" int lfgpu_assemble_galerkin(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgp
   LFG_TRY(resolve_coeff(f, m->ncells, fe->nqp, true, &a.f, "f"));
   a.has_f = f != nullptr;
   a.nrows = m->ndofs;
+  return LFGPU_OK;
+}
+
+template <bool EDGES>
+int run_assembly(lfgpu_ctx *ctx, lfgpu_plan *p, const AsmArgs &a) {
   LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-  if (m->nloc == 3)
-    LFG_TRY((launch_assemble<3, 3, 4>(ctx->stream, p, a)));
+  if (p->mesh->nloc == 3)
+    LFG_TRY((launch_assemble<3, 3, 4, EDGES>(ctx->stream, p, a)));
   else
-    LFG_TRY((launch_assemble<10, 12, 12>(ctx->stream, p, a)));
+    LFG_TRY((launch_assemble<10, 12, 12, EDGES>(ctx->stream, p, a)));
   LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0.f;
@@ -611,4 +695,45 @@ extern "C" int lfgpu_assemble_galerkin(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgp
   ctx->last_ms = ms;
   csr_values_changed(&p->csr);
   return LFGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int lfgpu_assemble_galerkin(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
+                                       const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,
+                                       const lfgpu_coeff *f, const lfgpu_vec *fixed_vals,
+                                       lfgpu_vec *rhs) {
+  LFG_REQUIRE(ctx && p && fe, LFGPU_EINVAL, "NULL argument to lfgpu_assemble_galerkin");
+  AsmArgs a;
+  LFG_TRY(asm_args(p, fe, alpha, gamma, f, fixed_vals, rhs, a));
+  return run_assembly<false>(ctx, p, a);
+}
+
+extern "C" int lfgpu_assemble_galerkin_bd(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
+                                          const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,
+                                          const lfgpu_coeff *f, const lfgpu_coeff *eta,
+                                          const lfgpu_coeff *h, const lfgpu_vec *fixed_vals,
+                                          lfgpu_vec *rhs) {
+  LFG_REQUIRE(ctx && p && fe, LFGPU_EINVAL, "NULL argument to lfgpu_assemble_galerkin_bd");
+  LFG_REQUIRE(p->edges, LFGPU_EINVAL, "the plan has no edge terms (lfgpu_plan_set_edges)");
+  LFG_REQUIRE(!p->edge_mat || eta, LFGPU_EINVAL, "the plan's edge mass term needs eta");
+  LFG_REQUIRE(!p->edge_vec || h, LFGPU_EINVAL, "the plan's edge load term needs h");
+  AsmArgs a;
+  LFG_TRY(asm_args(p, fe, alpha, gamma, f, fixed_vals, rhs, a));
+  const lfgpu_edges *E = p->edges;
+  a.edge_v = E->edge_v;
+  a.edge_dofs = E->edge_dofs;
+  a.etab = E->tab;
+  if (p->edge_mat) {
+    LFG_TRY(resolve_coeff(eta, E->nedges, E->nqe, false, &a.eta, "eta"));
+    a.erec_ptr = p->erec_ptr;
+    a.erec_key = p->erec_key;
+    a.erec_meta = p->erec_meta;
+  }
+  if (p->edge_vec) {
+    LFG_TRY(resolve_coeff(h, E->nedges, E->nqe, false, &a.h, "h"));
+    a.vrec_ptr = p->vrec_ptr;
+    a.vrec_key = p->vrec_key;
+  }
+  return run_assembly<true>(ctx, p, a);
 }

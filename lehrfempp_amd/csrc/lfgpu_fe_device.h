@@ -189,6 +189,63 @@ __device__ __forceinline__ void coeff_qp(const CoeffD &c, int64_t cell, const Ce
   }
 }
 
+// Segment tables: w[NQE] | pts[NQE] | N[NLE x NQE] with N(i, k) at i + NLE k
+// (PrecomputedScalarReferenceFiniteElement of the segment element).
+template <int NLE, int NQE>
+struct ETab {
+  static constexpr int kLen = 2 * NQE + NLE * NQE;
+  const double *t;
+  __device__ __forceinline__ double w(int k) const { return t[k]; }
+  __device__ __forceinline__ double px(int k) const { return t[NQE + k]; }
+  __device__ __forceinline__ double N(int i, int k) const { return t[2 * NQE + i + NLE * k]; }
+};
+
+// SegmentO1::IntegrationElement (geometry/segment_o1.cc): |v1 - v0|.
+__device__ __forceinline__ double seg_length(double2 a, double2 b) {
+  const double dx = b.x - a.x, dy = b.y - a.y;
+  return sqrt(dx * dx + dy * dy);
+}
+
+// Coefficient of an edge term at point k: SegmentO1::Global(x) = v1 x +
+// v0 (1 - x) for the analytic kind.
+template <int NQE>
+__device__ __forceinline__ double edge_coeff(const CoeffD &c, int64_t e, int k, double2 a,
+                                             double2 b, double xh) {
+  double x = 0.0, y = 0.0;
+  if (c.kind == LFGPU_COEFF_SINSIN) {
+    x = b.x * xh + a.x * (1 - xh);
+    y = b.y * xh + a.y * (1 - xh);
+  }
+  return coeff_at(c, e, k, NQE, x, y);
+}
+
+// Row i of MassEdgeMatrixProvider::Eval (uscalfe/loc_comp_ellbvp.h:510-526):
+//   mat += (N_k N_k^T) * w,  w = (omega_k |det|) gamma_k.
+template <int NLE, int NQE>
+__device__ __forceinline__ void edge_row(const ETab<NLE, NQE> &T, double det, const double *gk,
+                                         int i, double *row) {
+#pragma unroll
+  for (int j = 0; j < NLE; ++j) row[j] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NQE; ++k) {
+    const double w = (T.w(k) * det) * gk[k];
+    const double ni = T.N(i, k);
+#pragma unroll
+    for (int j = 0; j < NLE; ++j) row[j] += (ni * T.N(j, k)) * w;
+  }
+}
+
+// Entry i of ScalarLoadEdgeVectorProvider::Eval (loc_comp_ellbvp.h:905-916):
+//   vec += N_k * w,  w = (omega_k |det|) g_k.
+template <int NLE, int NQE>
+__device__ __forceinline__ double edge_load(const ETab<NLE, NQE> &T, double det, const double *hk,
+                                            int i) {
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < NQE; ++k) v += T.N(i, k) * ((T.w(k) * det) * hk[k]);
+  return v;
+}
+
 // The value coeff_qp<NQP> stores at point k, for kernels that spread the
 // points of one cell over several lanes.
 template <int NQP>

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/lfgpu.h"
 
@@ -130,6 +131,28 @@ struct lfgpu_plan {
   int max_block_nnz = 0;       // LDS doubles needed by one workgroup
   double *ke = nullptr;        // [ncells x nloc^2] element matrices (contraction path)
   double *fe = nullptr;        // [ncells x nloc] element vectors (contraction path)
+  // boundary edge terms (lfgpu_plan_set_edges)
+  const lfgpu_edges *edges = nullptr;
+  int32_t *erec_ptr = nullptr;  // [ndofs + 1] edge-matrix records per row
+  int32_t *erec_key = nullptr;  // [Rm] edge * nle + i, in edge order within a row
+  uint8_t *erec_meta = nullptr; // [Rm x 4] CSR slot of each local column (0xFF: fixed)
+  int32_t *vrec_ptr = nullptr;  // [ndofs + 1] edge-load records per row
+  int32_t *vrec_key = nullptr;  // [Rv] edge * nle + i
+  bool edge_mat = false, edge_vec = false;
+};
+
+// Codim-1 entities for the boundary terms: edge e runs from edge_v[2e] to
+// edge_v[2e+1] (its orientation); its DOFs are the two endpoint DOFs then
+// its interior DOFs (P3).
+struct lfgpu_edges {
+  lfgpu_ctx *ctx = nullptr;
+  const lfgpu_mesh *mesh = nullptr;
+  int degree = 0, nle = 0, nqe = 0;
+  int64_t nedges = 0;
+  int32_t *edge_v = nullptr;     // [E x 2]
+  int32_t *edge_dofs = nullptr;  // [E x nle]
+  double *tab = nullptr;         // w[nqe] | pts[nqe] | N[nle x nqe], N(i,k) at i + nle k
+  std::vector<int32_t> h_dofs;   // host copy of edge_dofs
 };
 
 namespace lfgpu {
@@ -138,5 +161,7 @@ void solver_cache_free(lfgpu_csr *A);
 // reference-tensor contraction on the matrix cores (lfgpu_mfma.hip).
 int element_contract(hipStream_t s, const lfgpu_mesh *m, const lfgpu_fe *fe, const CoeffD &alpha,
                      const CoeffD &gamma, const CoeffD *f, double *ke, double *fvec);
+// Release the edge records of a plan (lfgpu_edges.hip).
+void plan_free_edges(lfgpu_plan *p);
 inline void csr_values_changed(lfgpu_csr *A) { A->val_version++; }
 }  // namespace lfgpu

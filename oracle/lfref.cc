@@ -144,6 +144,75 @@ bool make_tables(int degree, FeTables &t) {
 }
 
 // ---------------------------------------------------------------------------
+// Segments (codim-1 entities): Gauss-Legendre rule, Lagrange shape functions,
+// SegmentO1 geometry.
+// ---------------------------------------------------------------------------
+// GaussLegendre(n) on [0, 1] (quad/gauss_quadrature.cc:19-50): Newton on the
+// Legendre recurrence from z = cos(pi (i + 3/4) / (n + 1/2)), then
+// points 0.5 (1 -+ z), weights 1 / ((1 - z^2) p'^2).  The reference iterates in
+// 57-bit binary floating point; this restatement uses long double (64-bit
+// mantissa on x86), so the doubles agree to rounding (the reference's own test
+// pins them to 4 ulps, quad/test/gauss_legendre_tests.cc:14-57).
+void gauss_legendre01(int n, double *pts, double *w) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  const int m = (n + 1) / 2;
+  for (int i = 0; i < m; ++i) {
+    long double z = std::cos(pi * (i + 0.75L) / (n + 0.5L)), z1, pp;
+    int guard = 0;
+    do {
+      long double p1 = 1.0L, p2 = 0.0L, p3;
+      for (int j = 0; j < n; ++j) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0L * j + 1.0L) * z * p2 - j * p3) / (j + 1.0L);
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0L);
+      z1 = z;
+      z = z1 - p1 / pp;
+    } while (std::fabs(z - z1) > 1e-19L && ++guard < 100);
+    pts[i] = (double)(0.5L * (1.0L - z));
+    pts[n - 1 - i] = (double)(0.5L * (1.0L + z));
+    w[i] = (double)(1.0L / ((1.0L - z * z) * pp * pp));
+    w[n - 1 - i] = w[i];
+  }
+}
+
+// FeLagrangeO1Segment / FeLagrangeO3Segment shape function values
+// (uscalfe/lagr_fe.h:284-330, :1155-1230) at reference point x.
+void segment_shape(int degree, double x, double *N) {
+  if (degree == 1) {
+    N[0] = 1.0 - x;
+    N[1] = x;
+  } else {
+    N[0] = ((4.5 * (1.0 - x)) * (1.0 / 3.0 - x)) * (2.0 / 3.0 - x);
+    N[1] = ((4.5 * x) * (x - 1.0 / 3.0)) * (x - 2.0 / 3.0);
+    N[2] = ((13.5 * x) * (1 - x)) * (2.0 / 3.0 - x);
+    N[3] = ((13.5 * x) * (1 - x)) * (x - 1.0 / 3.0);
+  }
+}
+
+struct SegTables {
+  int nsf = 0, nqp = 0;
+  double pts[8], w[8], N[8 * 4];
+};
+
+// make_QuadRule(kSegment, 2 * degree) (quad/make_quad_rule.cc:13-18): degree + 1
+// Gauss points, as the edge providers' constructors choose
+// (uscalfe/loc_comp_ellbvp.h:383-396, :806-820).
+void make_seg_tables(int degree, SegTables &t) {
+  t.nsf = degree + 1;
+  t.nqp = (2 * degree) / 2 + 1;
+  gauss_legendre01(t.nqp, t.pts, t.w);
+  for (int k = 0; k < t.nqp; ++k) segment_shape(degree, t.pts[k], t.N + t.nsf * k);
+}
+
+// SegmentO1 (geometry/segment_o1.cc): |v1 - v0| and Global(x) = v1 x + v0 (1 - x).
+double seg_length(const double *a, const double *b) {
+  const double dx = b[0] - a[0], dy = b[1] - a[1];
+  return std::sqrt(dx * dx + dy * dy);
+}
+
+// ---------------------------------------------------------------------------
 // TriaO1 (geometry/tria_o1.cc:10-68).
 // ---------------------------------------------------------------------------
 struct Geo {
@@ -740,6 +809,68 @@ int64_t lfref_cpu_path_assemble(int degree, int ncells, const double *xy,
   }
   return lfref_set_from_triplets((int64_t)trips.size(), R.data(), C.data(),
                                  Vv.data(), (int32_t)ndofs, row_ptr, col, val);
+}
+
+int lfref_segment_rule(int n, double *pts, double *w) {
+  if (n < 1 || n > 8) return -1;
+  gauss_legendre01(n, pts, w);
+  return 0;
+}
+
+// Coefficient on edge e at point k: 0 constant c, 1 c * data[e],
+// 2 c * data[e * nq + k], 3 c * sin(pi x) sin(pi y) at the mapped point.
+static double edge_coeff(int kind, double c, const double *data, int e, int k, int nq,
+                         const double *a, const double *b, double xhat) {
+  if (kind == 0) return c;
+  if (kind == 1) return c * data[e];
+  if (kind == 2) return c * data[(int64_t)e * nq + k];
+  const double x = b[0] * xhat + a[0] * (1 - xhat), y = b[1] * xhat + a[1] * (1 - xhat);
+  return c * (std::sin(M_PI * x) * std::sin(M_PI * y));
+}
+
+int lfref_edge_matrices(int degree, int nedges, const double *xy, const int32_t *edge_v,
+                        int g_kind, double g_const, const double *g_data, double *out) {
+  if (degree != 1 && degree != 3) return -1000;
+  SegTables t;
+  make_seg_tables(degree, t);
+  const int n = t.nsf;
+  for (int e = 0; e < nedges; ++e) {
+    const double *a = P(xy, edge_v[2 * e]), *b = P(xy, edge_v[2 * e + 1]);
+    const double det = seg_length(a, b);
+    double *mat = out + (int64_t)e * n * n;
+    for (int q = 0; q < n * n; ++q) mat[q] = 0.0;
+    for (int k = 0; k < t.nqp; ++k) {
+      // mat += (N_k N_k^T) * w, w = (omega_k det_k) gamma_k (loc_comp_ellbvp.h:510-526)
+      const double w =
+          (t.w[k] * det) * edge_coeff(g_kind, g_const, g_data, e, k, t.nqp, a, b, t.pts[k]);
+      const double *Nk = t.N + n * k;
+      for (int i = 0; i < n; ++i)
+        for (int jj = 0; jj < n; ++jj) mat[i * n + jj] += (Nk[i] * Nk[jj]) * w;
+    }
+  }
+  return 0;
+}
+
+int lfref_edge_vectors(int degree, int nedges, const double *xy, const int32_t *edge_v,
+                       int g_kind, double g_const, const double *g_data, double *out) {
+  if (degree != 1 && degree != 3) return -1000;
+  SegTables t;
+  make_seg_tables(degree, t);
+  const int n = t.nsf;
+  for (int e = 0; e < nedges; ++e) {
+    const double *a = P(xy, edge_v[2 * e]), *b = P(xy, edge_v[2 * e + 1]);
+    const double det = seg_length(a, b);
+    double *vec = out + (int64_t)e * n;
+    for (int i = 0; i < n; ++i) vec[i] = 0.0;
+    for (int k = 0; k < t.nqp; ++k) {
+      // vec += N_k * w, w = (omega_k det_k) g_k (loc_comp_ellbvp.h:905-916)
+      const double w =
+          (t.w[k] * det) * edge_coeff(g_kind, g_const, g_data, e, k, t.nqp, a, b, t.pts[k]);
+      const double *Nk = t.N + n * k;
+      for (int i = 0; i < n; ++i) vec[i] += Nk[i] * w;
+    }
+  }
+  return 0;
 }
 
 int lfref_error_norms(int degree, int ncells, const double *xy,

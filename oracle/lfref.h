@@ -170,6 +170,27 @@ int lfref_adjoint_gradient_p1(int ncells, const double *xy,
                               const double *u, const double *lam,
                               double *grad);
 
+/* GaussLegendre(n) on [0, 1] (quad/gauss_quadrature.cc:19-50), the rule
+ * make_QuadRule(kSegment, 2p) returns for n = p + 1 (quad/make_quad_rule.cc:
+ * 13-18).  n in 1..8. */
+int lfref_segment_rule(int n, double *pts, double *w);
+
+/* MassEdgeMatrixProvider::Eval (uscalfe/loc_comp_ellbvp.h:467-527) and
+ * ScalarLoadEdgeVectorProvider::Eval (:864-918) for every edge: edge e runs
+ * from vertex edge_v[2e] to edge_v[2e+1] (its orientation), FeLagrangeO1Segment
+ * or FeLagrangeO3Segment (lagr_fe.h:284-330, :1155-1230), rule
+ * make_QuadRule(kSegment, 2 degree), SegmentO1 geometry
+ * (geometry/segment_o1.cc).  Coefficient g_kind: 0 constant g_const,
+ * 1 g_const * g_data[e], 2 g_const * g_data[e * nqp + k], 3 g_const *
+ * sin(pi x) sin(pi y) at SegmentO1::Global of the point.  out: row-major
+ * nloc x nloc (matrices) or nloc (vectors) per edge, nloc = degree + 1. */
+int lfref_edge_matrices(int degree, int nedges, const double *xy,
+                        const int32_t *edge_v, int g_kind, double g_const,
+                        const double *g_data, double *out);
+int lfref_edge_vectors(int degree, int nedges, const double *xy,
+                       const int32_t *edge_v, int g_kind, double g_const,
+                       const double *g_data, double *out);
+
 /* Squared error norms of a finite element function u_h (DOF vector uh) against
  * an exact solution, as the demo drivers compute them
  * (examples/ellbvp_linfe/homDir_linfe_demo.cc:179-188):

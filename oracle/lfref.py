@@ -61,6 +61,11 @@ def lib():
             "lfref_adjoint_gradient_p1": (C.c_int, [C.c_int, _d, _i32, _d, _d, _d, _d]),
             "lfref_error_norms": (C.c_int, [C.c_int, C.c_int, _d, _i32, _i64, _d, C.c_int,
                                             C.c_void_p, C.c_void_p, _d]),
+            "lfref_segment_rule": (C.c_int, [C.c_int, _d, _d]),
+            "lfref_edge_matrices": (C.c_int, [C.c_int, C.c_int, _d, _i32, C.c_int, C.c_double,
+                                              C.c_void_p, _d]),
+            "lfref_edge_vectors": (C.c_int, [C.c_int, C.c_int, _d, _i32, C.c_int, C.c_double,
+                                             C.c_void_p, _d]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -133,6 +138,18 @@ class TriMesh:
                                    self.cell_e.ravel(), self.cell_ori.ravel(), ndof_edge,
                                    ndof_cell, cd)
         return cd.reshape(self.ne, nloc), int(nd)
+
+    def edge_dofs(self, degree: int):
+        """GlobalDofIndices of every edge (dofhandler.cc:160-181): its two
+        endpoint DOFs in the edge's orientation, then its interior DOFs."""
+        E = self.nedges
+        if degree == 1:
+            return self.edges.astype(np.int64).copy()
+        ed = np.zeros((E, 4), np.int64)
+        ed[:, :2] = self.edges
+        ed[:, 2] = self.nv + 2 * np.arange(E)
+        ed[:, 3] = self.nv + 2 * np.arange(E) + 1
+        return ed
 
     def dirichlet_flags(self, degree: int):
         """Boundary DOF flags: boundary nodes, plus interior DOFs of boundary edges."""
@@ -339,15 +356,88 @@ def error_norms(mesh: TriMesh, degree: int, uh, u_qp=None, g_qp=None):
     return out[0], out[1]
 
 
+def segment_rule(n: int):
+    pts, w = np.zeros(n), np.zeros(n)
+    assert lib().lfref_segment_rule(n, pts, w) == 0
+    return pts, w
+
+
+G_CONST, G_EDGE, G_QP, G_SINSIN = 0, 1, 2, 3
+
+
+def _edge_call(fn, degree, xy, edge_v, kind, const, data, width):
+    ev = np.ascontiguousarray(edge_v, np.int32)
+    E = ev.shape[0]
+    out = np.zeros(E * width)
+    d = None
+    if data is not None:
+        d = np.ascontiguousarray(data, np.float64)
+    rc = fn(degree, E, np.ascontiguousarray(xy, np.float64).ravel(), ev.ravel(), kind,
+            float(const), None if d is None else d.ctypes.data, out)
+    assert rc == 0, rc
+    return out
+
+
+def edge_matrices(xy, edge_v, degree, kind=G_CONST, const=1.0, data=None):
+    """MassEdgeMatrixProvider::Eval for every edge (lfref_edge_matrices)."""
+    n = degree + 1
+    return _edge_call(lib().lfref_edge_matrices, degree, xy, edge_v, kind, const, data,
+                      n * n).reshape(-1, n, n)
+
+
+def edge_vectors(xy, edge_v, degree, kind=G_CONST, const=1.0, data=None):
+    """ScalarLoadEdgeVectorProvider::Eval for every edge (lfref_edge_vectors)."""
+    n = degree + 1
+    return _edge_call(lib().lfref_edge_vectors, degree, xy, edge_v, kind, const, data,
+                      n).reshape(-1, n)
+
+
+def eta_sub(coef, sel):
+    """(kind, const, data) of an edge coefficient restricted to the edges sel."""
+    kind, const, data = coef
+    if kind == G_EDGE:
+        data = np.asarray(data)[sel]
+    elif kind == G_QP:
+        data = np.asarray(data)[sel]
+    return kind, const, data
+
+
+def _add_vector(b, edofs, evecs):
+    """b[dof] += v in edge order (AssembleVectorLocally, assembler.h:298-327)."""
+    b = b.copy()
+    for e in range(edofs.shape[0]):
+        for i in range(edofs.shape[1]):
+            b[edofs[e, i]] += evecs[e, i]
+    return b
+
+
 def galerkin_system(mesh: TriMesh, degree: int, alpha, gamma, f_kind=F_CONST, f_const=1.0,
-                    f_qp=None, flags=None, fixed_vals=None):
+                    f_qp=None, flags=None, fixed_vals=None, edge_terms=None):
     """The homDir_linfe_demo.cc:130-170 pipeline: element matrices -> COO ->
-    load vector -> FixFlaggedSolutionComponents -> setFromTriplets (CSR)."""
+    load vector -> FixFlaggedSolutionComponents -> setFromTriplets (CSR).
+    edge_terms = (mat_flags, (kind, const, data), vec_flags, (kind, const,
+    data)) adds MassEdgeMatrixProvider / ScalarLoadEdgeVectorProvider terms on
+    the flagged edges (None flags: no term)."""
     cd, nd = mesh.dofs(degree)
     mats = element_matrices(mesh, degree, alpha, gamma)
     r, c, v = assemble_coo(cd, mats)
     vecs = element_vectors(mesh, degree, f_kind, f_const, f_qp)
     b = assemble_vector(cd, vecs, nd)
+    if edge_terms is not None:
+        # AssembleMatrixLocally(1, ...) and AssembleVectorLocally(1, ...) after
+        # the cell terms (ellbvp_linfe_demo.cc:272-315): edge triplets follow
+        # the cell triplets, edge loads are added after the cell loads
+        ed = mesh.edge_dofs(degree)
+        mat_on, eta, vec_on, h = edge_terms
+        if mat_on is not None:
+            sel = np.nonzero(mat_on)[0]
+            em = edge_matrices(mesh.xy, mesh.edges[sel], degree, *eta_sub(eta, sel))
+            re, ce, ve = assemble_coo(ed[sel], em)
+            r, c, v = np.concatenate([r, re]), np.concatenate([c, ce]), np.concatenate([v, ve])
+        if vec_on is not None:
+            sel = np.nonzero(vec_on)[0]
+            evv = edge_vectors(mesh.xy, mesh.edges[sel], degree, *eta_sub(h, sel))
+            b = _add_vector(b, ed[sel], evv)
     if flags is None:
         flags = mesh.dirichlet_flags(degree)
     if fixed_vals is None:

@@ -811,3 +811,149 @@ def test_manufactured_solution_errors_on_gpu(ctx):
     e = np.array(errs)
     r = e[:-1] / e[1:]
     assert np.all((r[:, 0] > 3.7) & (r[:, 0] < 4.3)) and np.all((r[:, 1] > 1.9) & (r[:, 1] < 2.1)), r
+
+
+# ------------------------------------------------------ boundary edge terms
+def _edge_coeffs(ctx, kind, n_edges, nq, rng):
+    """(device coefficient, oracle (kind, const, data), keep-alive vectors)."""
+    if kind == "const":
+        return capi.coeff(capi.COEFF_CONST, 1.7), (L.G_CONST, 1.7, None), []
+    if kind == "edge":
+        d = rng.uniform(0.5, 2.0, n_edges)
+        v = capi.Vec(ctx, d)
+        return capi.coeff(capi.COEFF_CELL, 1.0, v), (L.G_EDGE, 1.0, d), [v]
+    if kind == "qp":
+        d = rng.uniform(-1.0, 1.0, (n_edges, nq))
+        v = capi.Vec(ctx, d)
+        return capi.coeff(capi.COEFF_QP, 1.0, v), (L.G_QP, 1.0, d), [v]
+    return capi.coeff(capi.COEFF_SINSIN, 2.0), (L.G_SINSIN, 2.0, None), []
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+@pytest.mark.parametrize("kind", ["const", "edge", "qp", "sinsin"])
+def test_edge_matrices_and_vectors_match_oracle(ctx, degree, kind):
+    # MassEdgeMatrixProvider / ScalarLoadEdgeVectorProvider for every edge
+    m = jittered_mesh(7, seed=3)
+    g = gpu_mesh(ctx, m, degree)
+    ed = m.edge_dofs(degree)
+    E = capi.Edges(ctx, g, degree, m.edges, None if degree == 1 else ed)
+    rng = np.random.default_rng(6)
+    c, co, keep = _edge_coeffs(ctx, kind, m.nedges, degree + 1, rng)
+    Mg = capi.edge_matrices(ctx, E, c)
+    vg = capi.edge_vectors(ctx, E, c)
+    Mo = L.edge_matrices(m.xy, m.edges, degree, *co)
+    vo = L.edge_vectors(m.xy, m.edges, degree, *co)
+    if kind == "sinsin":  # device sin vs glibc sin may differ in the last bit
+        assert np.abs(Mg - Mo).max() <= 1e-15 * np.abs(Mo).max()
+        assert np.abs(vg - vo).max() <= 1e-15 * np.abs(vo).max()
+    else:
+        assert np.array_equal(Mg, Mo) and np.array_equal(vg, vo)
+    for h in keep + [E, g]:
+        h.close()
+
+
+def _mixed_bc_setup(m, degree):
+    """Dirichlet on x = 0 (with values), impedance (edge mass + load) on
+    y = 0 and x = 1, Neumann (load only) on y = 1."""
+    _, ef = m.boundary_flags()
+    mid = 0.5 * (m.xy[m.edges[:, 0]] + m.xy[m.edges[:, 1]])
+    left = (ef == 1) & (mid[:, 0] < 1e-9)
+    imp = (ef == 1) & ((mid[:, 1] < 1e-9) | (mid[:, 0] > 1 - 1e-9)) & ~left
+    neu = (ef == 1) & ~left & ~imp
+    ed = m.edge_dofs(degree)
+    cd, nd = m.dofs(degree)
+    flags = np.zeros(nd, np.uint8)
+    flags[ed[left].ravel()] = 1
+    return flags, imp.astype(np.uint8), (imp | neu).astype(np.uint8)
+
+
+@pytest.mark.parametrize("degree,n", [(1, 24), (3, 7)])
+def test_galerkin_with_edge_terms_matches_oracle(ctx, degree, n):
+    # ellbvp_linfe_demo.cc:272-330 pattern: cell terms, edge mass on impedance
+    # edges, edge loads on Neumann + impedance edges, Dirichlet values on the
+    # rest; CSR and rhs bit for bit against the oracle pipeline
+    m = jittered_mesh(n, seed=9)
+    cd, nd = m.dofs(degree)
+    flags, mat_on, vec_on = _mixed_bc_setup(m, degree)
+    rng = np.random.default_rng(n)
+    alpha = np.exp(rng.standard_normal(m.ne))
+    eta_d = rng.uniform(0.5, 2.0, m.nedges)
+    h_d = rng.uniform(-1.0, 1.0, (m.nedges, degree + 1))
+    gvals = rng.standard_normal(nd) * flags
+    g = gpu_mesh(ctx, m, degree)
+    E = capi.Edges(ctx, g, degree, m.edges, None if degree == 1 else m.edge_dofs(degree))
+    fe = capi.FE(ctx, degree)
+    plan = capi.Plan(ctx, g, flags)
+    plan.set_edges(E, mat_on, vec_on)
+    va, ve, vh, vg = (capi.Vec(ctx, alpha), capi.Vec(ctx, eta_d), capi.Vec(ctx, h_d),
+                      capi.Vec(ctx, gvals))
+    rhs = capi.Vec(ctx, nd)
+    plan.assemble_bd(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 0.5),
+                     capi.coeff(capi.COEFF_CONST, 1.0), capi.coeff(capi.COEFF_CELL, 1.0, ve),
+                     capi.coeff(capi.COEFF_QP, 1.0, vh), rhs, vg)
+    rp, col, val = plan.csr.download()
+    rpo, colo, valo, bo = L.galerkin_system(
+        m, degree, alpha, 0.5, L.F_CONST, 1.0, flags=flags, fixed_vals=gvals,
+        edge_terms=(mat_on, (L.G_EDGE, 1.0, eta_d), vec_on, (L.G_QP, 1.0, h_d)))
+    assert np.array_equal(rp, rpo) and np.array_equal(col, colo)
+    assert np.array_equal(val, valo), row_scaled_err(rp, val, valo)
+    assert np.array_equal(rhs.numpy(), bo), np.abs(rhs.numpy() - bo).max()
+    # the plain assembly of the same plan leaves the edge terms out
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 0.5),
+                  capi.coeff(capi.COEFF_CONST, 1.0), rhs, vg)
+    _, _, val2 = plan.csr.download()
+    _, _, valo2, _ = L.galerkin_system(m, degree, alpha, 0.5, L.F_CONST, 1.0, flags=flags,
+                                       fixed_vals=gvals)
+    assert np.array_equal(val2, valo2)
+    # an edge term without its coefficient is an error
+    with pytest.raises(capi.LfgpuError):
+        plan.assemble_bd(fe, capi.coeff(capi.COEFF_CELL, 1.0, va),
+                         capi.coeff(capi.COEFF_CONST, 0.5), None, None, None, rhs)
+    for h in (rhs, va, ve, vh, vg, plan, fe, E, g):
+        h.close()
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_robin_problem_solution_on_gpu(ctx, degree):
+    # -Laplace u + u = f, du/dn + u = h on the whole boundary, u = 1 + x + 2y
+    # in the FE space: assembled with the edge terms and solved on the
+    # device, the solution is the nodal interpolant of u
+    n = 10
+    m = L.tp_triag_mesh(n)
+    cd, nd = m.dofs(degree)
+    q = L.qp_coords(m, degree)
+    u = lambda x, y: 1 + x + 2 * y
+    _, ef = m.boundary_flags()
+    sel = np.nonzero(ef)[0]
+    p, _ = L.segment_rule(degree + 1)
+    a, b = m.xy[m.edges[sel, 0]], m.xy[m.edges[sel, 1]]
+    pts = b[:, None, :] * p[None, :, None] + a[:, None, :] * (1 - p)[None, :, None]
+    mid = 0.5 * (a + b)
+    dudn = np.select([mid[:, 0] < 1e-12, mid[:, 0] > 1 - 1e-12, mid[:, 1] < 1e-12],
+                     [-1.0, 1.0, -2.0], 2.0)
+    h = np.zeros((m.nedges, degree + 1))
+    h[sel] = dudn[:, None] + u(pts[..., 0], pts[..., 1])
+    g = gpu_mesh(ctx, m, degree)
+    E = capi.Edges(ctx, g, degree, m.edges, None if degree == 1 else m.edge_dofs(degree))
+    fe = capi.FE(ctx, degree)
+    plan = capi.Plan(ctx, g, np.zeros(nd, np.uint8))
+    plan.set_edges(E, ef, ef)
+    vf, vh = capi.Vec(ctx, u(q[..., 0], q[..., 1])), capi.Vec(ctx, h)
+    rhs, x = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
+    plan.assemble_bd(fe, capi.coeff(capi.COEFF_CONST, 1.0), capi.coeff(capi.COEFF_CONST, 1.0),
+                     capi.coeff(capi.COEFF_QP, 1.0, vf), capi.coeff(capi.COEFF_CONST, 1.0),
+                     capi.coeff(capi.COEFF_QP, 1.0, vh), rhs)
+    info = capi.pcg(ctx, plan.csr, rhs, x, rtol=1e-13)
+    assert info.converged
+    # nodal interpolant: every cell writes u at its evaluation nodes
+    nodes = (np.array([[0, 0], [1, 0], [0, 1]]) if degree == 1 else
+             np.array([[0, 0], [1, 0], [0, 1], [1 / 3, 0], [2 / 3, 0], [2 / 3, 1 / 3],
+                       [1 / 3, 2 / 3], [0, 2 / 3], [0, 1 / 3], [1 / 3, 1 / 3]]))
+    xi = np.zeros(nd)
+    p0, p1, p2 = m.xy[m.cells[:, 0]], m.xy[m.cells[:, 1]], m.xy[m.cells[:, 2]]
+    for i, (s, t) in enumerate(nodes):
+        pt = p0 * (1 - s - t) + p1 * s + p2 * t
+        xi[cd[:, i]] = u(pt[:, 0], pt[:, 1])
+    assert np.abs(x.numpy() - xi).max() < 1e-10
+    for hh in (x, rhs, vf, vh, plan, fe, E, g):
+        hh.close()

@@ -465,3 +465,100 @@ def test_error_norms_convergence_rates():
     r = e[:-1] / e[1:]
     assert np.all((r[:, 0] > 3.7) & (r[:, 0] < 4.3)), r
     assert np.all((r[:, 1] > 1.9) & (r[:, 1] < 2.1)), r
+
+
+# ------------------------------------------------------ boundary edge terms
+SEG_RULES = {  # correctly rounded Gauss-Legendre points / weights on [0, 1]
+    2: ([0.2113248654051871, 0.7886751345948129], [0.5, 0.5]),
+    4: ([0.06943184420297371, 0.33000947820757187, 0.6699905217924281, 0.9305681557970263],
+        [0.17392742256872692, 0.32607257743127305, 0.32607257743127305, 0.17392742256872692]),
+}
+
+
+def test_segment_rule_closed_forms():
+    # quad/test/gauss_legendre_tests.cc:14-57 (EXPECT_DOUBLE_EQ = 4 ulps)
+    def close(a, b):
+        return abs(a - b) <= 4 * np.spacing(max(abs(a), abs(b)))
+    p, w = L.segment_rule(1)
+    assert p[0] == 0.5 and w[0] == 1.0
+    p, w = L.segment_rule(2)
+    assert close(p[0], (1 - np.sqrt(1 / 3)) / 2) and close(p[1], (1 + np.sqrt(1 / 3)) / 2)
+    assert close(w[0], 0.5) and close(w[1], 0.5)
+    p, w = L.segment_rule(3)
+    assert close(p[0], (1 - np.sqrt(3 / 5)) / 2) and p[1] == 0.5
+    assert close(w[0], 5 / 18) and close(w[1], 4 / 9)
+    p, w = L.segment_rule(4)
+    assert close(p[0], (1 - np.sqrt(3 / 7 + 2 / 7 * np.sqrt(6 / 5))) / 2)
+    assert close(p[1], (1 - np.sqrt(3 / 7 - 2 / 7 * np.sqrt(6 / 5))) / 2)
+    assert close(p[2], 1 - p[1]) and close(p[3], 1 - p[0])
+    assert close(w[0], (18 - np.sqrt(30)) / 72) and close(w[1], (18 + np.sqrt(30)) / 72)
+    # the device tables hold the correctly rounded values: within 1 ulp here
+    for n, (pp, ww) in SEG_RULES.items():
+        p, w = L.segment_rule(n)
+        assert np.all(np.abs(p - pp) <= np.spacing(np.abs(p))), (n, p - pp)
+        assert np.all(np.abs(w - ww) <= np.spacing(np.abs(w))), (n, w - ww)
+    # exactness: degree 2n - 1
+    for n in (1, 2, 3, 4):
+        p, w = L.segment_rule(n)
+        for k in range(2 * n):
+            assert abs(np.dot(w, p ** k) - 1 / (k + 1)) < 1e-15
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_edge_mass_and_load_closed_forms(degree):
+    rng = np.random.default_rng(4)
+    xy = rng.uniform(0, 1, (6, 2))
+    ev = np.array([[0, 1], [2, 3], [4, 5], [1, 0]], np.int32)
+    L_e = np.linalg.norm(xy[ev[:, 1]] - xy[ev[:, 0]], axis=1)
+    M = L.edge_matrices(xy, ev, degree, L.G_CONST, 1.0)
+    v = L.edge_vectors(xy, ev, degree, L.G_CONST, 1.0)
+    # integral of N_i: L/2 (P1); Simpson 3/8: L (1/8, 1/8, 3/8, 3/8) (P3)
+    wts = np.array([0.5, 0.5]) if degree == 1 else np.array([1, 1, 3, 3]) / 8
+    assert np.allclose(v, L_e[:, None] * wts, rtol=1e-14, atol=0)
+    assert np.allclose(M.sum(axis=2), L_e[:, None] * wts, rtol=1e-13, atol=0)
+    assert np.allclose(M, np.transpose(M, (0, 2, 1)), rtol=0, atol=1e-15)
+    if degree == 1:
+        ref = np.array([[2, 1], [1, 2]]) / 6
+        assert np.allclose(M, L_e[:, None, None] * ref, rtol=1e-14, atol=0)
+    # per-edge and per-point coefficients scale linearly
+    c = rng.uniform(0.5, 2, 4)
+    assert np.allclose(L.edge_matrices(xy, ev, degree, L.G_EDGE, 2.0, c),
+                       2.0 * c[:, None, None] * M, rtol=1e-14, atol=0)
+    nq = degree + 1
+    cq = np.repeat(c[:, None], nq, axis=1)
+    assert np.allclose(L.edge_vectors(xy, ev, degree, L.G_QP, 1.0, cq), c[:, None] * v,
+                       rtol=1e-14, atol=0)
+
+
+def _edge_qp_xy(m, sel, degree):
+    p, _ = L.segment_rule(degree + 1)
+    a, b = m.xy[m.edges[sel, 0]], m.xy[m.edges[sel, 1]]
+    return (b[:, None, :] * p[None, :, None] + a[:, None, :] * (1 - p)[None, :, None])
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_robin_problem_reproduces_linear_solution(degree):
+    # -Laplace u + u = f in the unit square, du/dn + u = h on the boundary
+    # (ellbvp_linfe_demo.cc:272-315 with impedance edges everywhere, no
+    # Dirichlet part); u = 1 + x + 2y lies in the FE space, all data are
+    # integrated exactly, so the Galerkin solution is its interpolant
+    n = 8
+    m = L.tp_triag_mesh(n)
+    cd, nd = m.dofs(degree)
+    q = L.qp_coords(m, degree)
+    u = lambda x, y: 1 + x + 2 * y
+    f = u(q[..., 0], q[..., 1])
+    _, ef = m.boundary_flags()
+    sel = np.nonzero(ef)[0]
+    pts = _edge_qp_xy(m, sel, degree)
+    mid = 0.5 * (m.xy[m.edges[sel, 0]] + m.xy[m.edges[sel, 1]])
+    dudn = np.select([mid[:, 0] < 1e-12, mid[:, 0] > 1 - 1e-12, mid[:, 1] < 1e-12],
+                     [-1.0, 1.0, -2.0], 2.0)
+    h = np.zeros((m.nedges, degree + 1))
+    h[sel] = dudn[:, None] + u(pts[..., 0], pts[..., 1])
+    rp, col, val, b = L.galerkin_system(
+        m, degree, 1.0, 1.0, L.F_QP, f_qp=f, flags=np.zeros(nd, np.uint8),
+        edge_terms=(ef, (L.G_CONST, 1.0, None), ef, (L.G_QP, 1.0, h)))
+    x = L.band_cholesky_solve(rp, col, val, b)
+    xi = _interpolant(m, degree, u)
+    assert np.abs(x - xi).max() < 1e-11, np.abs(x - xi).max()
