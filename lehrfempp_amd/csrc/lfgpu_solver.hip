@@ -58,6 +58,8 @@ struct PcgState {
   int refinements;
   int last_refine; // mixed: iteration of the last reliable update
   double alpha[2]; // single-reduction FP64: alpha by parity
+  double alpha_s, beta_s;  // row sweeps (sliced ELL): this iteration's alpha, beta
+  int stop_pending, conv_pending;  // row-sweep mixed: stop after the fold
 };
 
 struct lfgpu_solver_cache {
@@ -84,6 +86,7 @@ struct lfgpu_solver_cache {
   double *xa64 = nullptr;  // mixed with FP64 vectors: increment accumulator
   double *part = nullptr;  // [kPartSlots x kMaxGrid] partial sums
   double *cg1 = nullptr;   // single-reduction PCG: r, w, s by parity [6 n]
+  double *rpart = nullptr; // row-sweep PCG partials [3 x ceil(n / 256)]
   PcgState *st = nullptr;
 };
 
@@ -95,7 +98,7 @@ void solver_cache_free(lfgpu_csr *A) {
                   (void *)c->dinv, (void *)c->dinv32, (void *)c->rowsum, (void *)c->rowsum32,
                   (void *)c->r, (void *)c->p, (void *)c->q, (void *)c->r32, (void *)c->p32,
                   (void *)c->q32, (void *)c->xa32, (void *)c->xlo, (void *)c->r64, (void *)c->xa64,
-                  (void *)c->part, (void *)c->cg1, (void *)c->st})
+                  (void *)c->part, (void *)c->cg1, (void *)c->rpart, (void *)c->st})
     dfree(p);
   delete c;
   A->cache = nullptr;
@@ -704,6 +707,180 @@ __global__ __launch_bounds__(SB) void k_cg_update(CgArgs a, int par) {
 }
 
 // ---------------------------------------------------------------------------
+// FP64 PCG on sliced ELL.  The gathers of a row reach several distant parts
+// of the vector (vertex, edge and cell DOFs at P3), so chunked sweeps, whose
+// active rows are spread over the whole matrix, miss L2 on every column;
+// here both sweeps run one row per thread in index order (the active rows
+// and columns form a compact moving window) and a one-workgroup kernel
+// after each sweep sums the per-workgroup partials in a fixed order and
+// takes the scalar decisions.  Same recurrences and stopping test as
+// k_cg_spmv / k_cg_update.
+// ---------------------------------------------------------------------------
+constexpr int kRB = 256;    // rows per workgroup of the row sweeps
+constexpr int kRedB = 1024; // threads of the reduction kernels
+
+struct RowArgs {
+  MatView m;
+  const double *__restrict__ dinv;
+  double *__restrict__ x;
+  double *__restrict__ r;
+  double *__restrict__ p;
+  double *__restrict__ q;
+  double *part;  // slot k of group g at part[k * nwg + g]
+  int nwg;
+  PcgState *st;
+};
+
+template <int K>
+__device__ __forceinline__ void row_partials(const RowArgs &a, double (&v)[K], double *sh) {
+  block_sum_k<kRB, K>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) a.part[(int64_t)k * a.nwg + blockIdx.x] = v[k];
+}
+
+template <int K>
+__device__ __forceinline__ void row_totals(const RowArgs &a, double (&t)[K], double *sh) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) t[k] = 0.0;
+  for (int g = threadIdx.x; g < a.nwg; g += kRedB)
+#pragma unroll
+    for (int k = 0; k < K; ++k) t[k] += a.part[(int64_t)k * a.nwg + g];
+  block_sum_k<kRedB, K>(t, sh);
+}
+
+template <int FMT>
+__global__ __launch_bounds__(kRB) void k_cgr_init(RowArgs a, const double *__restrict__ b) {
+  __shared__ double sh[4 * (kRB / 64)];
+  const int32_t i = blockIdx.x * kRB + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < a.m.n) {
+    const double bi = b[i];
+    const double ri = bi - apply64<FMT>(a.m, a.x, i, a.x[i]);
+    a.r[i] = ri;
+    const double zi = a.dinv[i] * ri;
+    a.p[i] = zi;
+    v[0] = bi * bi;
+    v[1] = ri * ri;
+    v[2] = ri * zi;
+  }
+  row_partials<3>(a, v, sh);
+}
+
+// Eigen's early exits (zero rhs, converged start, maxit 0), else pass 0 starts
+__global__ __launch_bounds__(kRedB) void k_cgr_red_init(RowArgs a) {
+  __shared__ double sh[4 * (kRedB / 64)];
+  double t[3];
+  row_totals<3>(a, t, sh);
+  if (threadIdx.x == 0) {
+    PcgState *st = a.st;
+    const double bb = t[0], rr = t[1];
+    st->bb = bb;
+    st->rr = rr;
+    st->rho[0] = t[2];
+    st->converged = 0;
+    st->done = 0;
+    st->refinements = 0;
+    const double thr = st->thr * bb;  // thr holds tol^2 on entry
+    st->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+    st->iter = 0;
+    if (bb == 0.0 || rr < st->thr) {
+      st->done = 1;
+      st->converged = 1;
+    } else if (st->maxit <= 0) {
+      st->done = 1;
+    }
+  }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(kRB) void k_cgr_spmv(RowArgs a) {
+  __shared__ double sh[4 * (kRB / 64)];
+  if (a.st->done) return;
+  const int32_t i = blockIdx.x * kRB + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < a.m.n) {
+    const double pi = a.p[i];
+    const double qi = apply64<FMT>(a.m, a.p, i, pi);
+    a.q[i] = qi;
+    const double dq = a.dinv[i] * qi;
+    v[0] = pi * qi;
+    v[1] = dq * a.r[i];
+    v[2] = dq * qi;
+  }
+  row_partials<3>(a, v, sh);
+}
+
+// alpha = rho / (p, q); beta from rho' = rho - 2 alpha (q, D^-1 r) +
+// alpha^2 (q, D^-1 q), as k_cg_update
+__global__ __launch_bounds__(kRedB) void k_cgr_red_s(RowArgs a) {
+  __shared__ double sh[4 * (kRedB / 64)];
+  if (a.st->done) return;
+  double t[3];
+  row_totals<3>(a, t, sh);
+  if (threadIdx.x == 0) {
+    PcgState *st = a.st;
+    const double rho = st->rho[0], pq = t[0], qdr = t[1], qdq = t[2];
+    const double alpha = rho / pq;
+    const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
+    st->alpha_s = alpha;
+    st->beta_s = rho_next / rho;
+  }
+}
+
+__global__ __launch_bounds__(kRB) void k_cgr_update(RowArgs a) {
+  __shared__ double sh[4 * (kRB / 64)];
+  const PcgState *st = a.st;
+  if (st->done) return;
+  const double alpha = st->alpha_s, beta = st->beta_s;
+  const int32_t i = blockIdx.x * kRB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < a.m.n) {
+    const double pi = a.p[i];
+    a.x[i] += alpha * pi;
+    const double ri = a.r[i] - alpha * a.q[i];
+    a.r[i] = ri;
+    const double zi = a.dinv[i] * ri;
+    a.p[i] = zi + beta * pi;
+    v[0] = ri * ri;
+    v[1] = ri * zi;
+  }
+  row_partials<2>(a, v, sh);
+}
+
+// Eigen's test on the new residual; otherwise the next pass starts (and
+// stops before its update once maxit passes are done)
+__global__ __launch_bounds__(kRedB) void k_cgr_red_u(RowArgs a) {
+  __shared__ double sh[4 * (kRedB / 64)];
+  if (a.st->done) return;
+  double t[2];
+  row_totals<2>(a, t, sh);
+  if (threadIdx.x == 0) {
+    PcgState *st = a.st;
+    st->rr = t[0];
+    st->rho[0] = t[1];
+    if (t[0] < st->thr) {
+      st->done = 1;
+      st->converged = 1;
+    } else {
+      st->iter += 1;
+      if (st->iter >= st->maxit) st->done = 1;
+    }
+  }
+}
+
+template <int FMT>
+void launch_cgr_chunk(hipStream_t s, const RowArgs &a, int count) {
+  const dim3 G(a.nwg), B(kRB), B1(kRedB);
+  for (int k = 0; k < count; ++k) {
+    hipLaunchKernelGGL(k_cgr_spmv<FMT>, G, B, 0, s, a);
+    hipLaunchKernelGGL(k_cgr_red_s, dim3(1), B1, 0, s, a);
+    hipLaunchKernelGGL(k_cgr_update, G, B, 0, s, a);
+    hipLaunchKernelGGL(k_cgr_red_u, dim3(1), B1, 0, s, a);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 single-reduction PCG (Chronopoulos-Gear): one sweep per iteration.
 //   u = D^-1 r, w = A u;  gamma = (r, u), delta = (w, u)
 //   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
@@ -946,6 +1123,8 @@ struct MixArgs {
   Sweep sw;
   double delta2;
   int max_interval;
+  double *rpart;  // row sweeps: slot k of group g at rpart[k * nwg + g]
+  int nwg;
 };
 
 template <class VT>
@@ -1146,6 +1325,238 @@ __global__ __launch_bounds__(256) void k_mix_final(MixArgs<VT> a) {
   if (i < a.sw.n) a.x[i] += a.xlo[i];
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision on sliced ELL: the slots of k_mx_a / k_mx_b as one-row-
+// per-thread sweeps (see k_cgr_*), each followed by a one-workgroup kernel
+// that sums the partials and takes the decisions: after sweep B it decides
+// whether the next slot is a CG step or a reliable update; after sweep A it
+// forms alpha and beta, or judges the true residual.  A stop found there
+// takes effect after sweep B has folded the last increment into x.
+// ---------------------------------------------------------------------------
+template <class VT, int K>
+__device__ __forceinline__ void mxr_partials(const MixArgs<VT> &a, double (&v)[K], double *sh) {
+  block_sum_k<kRB, K>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) a.rpart[(int64_t)k * a.nwg + blockIdx.x] = v[k];
+}
+
+template <class VT, int K>
+__device__ __forceinline__ void mxr_totals(const MixArgs<VT> &a, double (&t)[K], double *sh) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) t[k] = 0.0;
+  for (int g = threadIdx.x; g < a.nwg; g += kRedB)
+#pragma unroll
+    for (int k = 0; k < K; ++k) t[k] += a.rpart[(int64_t)k * a.nwg + g];
+  block_sum_k<kRedB, K>(t, sh);
+}
+
+// r64 = b - A x, the recurrence started from it; partials b.b, r.r, r.D^-1 r
+template <int FMT, class VT>
+__global__ __launch_bounds__(kRB) void k_mxr_init(MixArgs<VT> a) {
+  __shared__ double sh[4 * (kRB / 64)];
+  const int32_t i = blockIdx.x * kRB + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < a.m.n) {
+    const double bi = a.b[i];
+    const double ri = bi - apply64<FMT>(a.m, a.x, i, a.x[i]);
+    a.r64[i] = ri;
+    const VT rv = (VT)ri;
+    a.r[i] = rv;
+    a.p[i] = (VT)a.dinv32[i] * rv;
+    a.xa[i] = (VT)0;
+    a.xlo[i] = 0.0;
+    v[0] = bi * bi;
+    v[1] = ri * ri;
+    v[2] = ri * (a.dinv[i] * ri);
+  }
+  mxr_partials<VT, 3>(a, v, sh);
+}
+
+template <class VT>
+__global__ __launch_bounds__(kRedB) void k_mxr_red_init(MixArgs<VT> a) {
+  __shared__ double sh[4 * (kRedB / 64)];
+  double t[3];
+  mxr_totals<VT, 3>(a, t, sh);
+  if (threadIdx.x == 0) {
+    PcgState *st = a.st;
+    const double bb = t[0], rr = t[1];
+    st->bb = bb;
+    st->rr = rr;
+    st->rr_ref = rr;
+    st->rho[0] = t[2];
+    st->iter = 0;
+    st->iterp[0] = 0;
+    st->refine[0] = 0;
+    st->refinements = 0;
+    st->last_refine = 0;
+    st->stop_pending = 0;
+    st->conv_pending = 0;
+    st->converged = 0;
+    st->done = 0;
+    const double thr = st->thr * bb;  // thr holds tol^2 on entry
+    st->thr = thr > 2.2250738585072014e-308 ? thr : 2.2250738585072014e-308;
+    if (bb == 0.0 || rr < st->thr) {
+      st->done = 1;
+      st->converged = 1;
+    } else if (st->maxit <= 0) {
+      st->done = 1;
+    }
+  }
+}
+
+template <int FMT, class VT>
+__global__ __launch_bounds__(kRB) void k_mxr_a(MixArgs<VT> a) {
+  __shared__ double sh[4 * (kRB / 64)];
+  const PcgState *st = a.st;
+  if (st->done) return;
+  const int32_t i = blockIdx.x * kRB + threadIdx.x;
+  if (!st->refine[0]) {
+    double v[3] = {0.0, 0.0, 0.0};
+    if (i < a.m.n) {
+      const VT pi = a.p[i];
+      const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
+      a.q[i] = qi;
+      const double dq = (double)((VT)a.dinv32[i] * qi);
+      v[0] = (double)pi * (double)qi;
+      v[1] = dq * (double)a.r[i];
+      v[2] = dq * (double)qi;
+    }
+    mxr_partials<VT, 3>(a, v, sh);
+  } else {
+    double v[1] = {0.0};
+    if (i < a.m.n) {
+      const VT *__restrict__ xa = a.xa;
+      const double ah = apply64<FMT>(a.m, a.x, i, a.x[i]);
+      const double aa =
+          apply64_fn<FMT>(a.m, i, (double)xa[i], [&](int32_t j) { return (double)xa[j]; });
+      const double al = apply64<FMT>(a.m, a.xlo, i, a.xlo[i]);
+      const double ri = ((a.b[i] - ah) - aa) - al;
+      a.r64[i] = ri;
+      v[0] = ri * ri;
+    }
+    mxr_partials<VT, 1>(a, v, sh);
+  }
+}
+
+template <class VT>
+__global__ __launch_bounds__(kRedB) void k_mxr_red_a(MixArgs<VT> a) {
+  __shared__ double sh[4 * (kRedB / 64)];
+  PcgState *st = a.st;
+  if (st->done) return;
+  if (!st->refine[0]) {
+    double t[3];
+    mxr_totals<VT, 3>(a, t, sh);
+    if (threadIdx.x == 0) {
+      const double rho = st->rho[0], pq = t[0], qdr = t[1], qdq = t[2];
+      const double alpha = rho / pq;
+      const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
+      st->alpha_s = alpha;
+      st->beta_s = rho_next / rho;
+      st->iterp[0] += 1;
+      st->iter = st->iterp[0];
+    }
+  } else {
+    double t[1];
+    mxr_totals<VT, 1>(a, t, sh);
+    if (threadIdx.x == 0) {
+      const double rt = t[0];
+      const bool conv = rt < st->thr;
+      const int its = st->iterp[0];
+      st->rr = rt;
+      st->refinements += 1;
+      st->last_refine = its;
+      if (conv || its >= st->maxit) {
+        st->stop_pending = 1;
+        st->conv_pending = conv ? 1 : 0;
+      } else {
+        st->rr_ref = rt;
+      }
+    }
+  }
+}
+
+template <class VT>
+__global__ __launch_bounds__(kRB) void k_mxr_b(MixArgs<VT> a) {
+  __shared__ double sh[4 * (kRB / 64)];
+  const PcgState *st = a.st;
+  if (st->done) return;
+  const int32_t i = blockIdx.x * kRB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (!st->refine[0]) {
+    const VT fa = (VT)st->alpha_s, fb = (VT)st->beta_s;
+    if (i < a.m.n) {
+      const VT pi = a.p[i];
+      a.xa[i] += fa * pi;
+      const VT ri = a.r[i] - fa * a.q[i];
+      a.r[i] = ri;
+      const VT zi = (VT)a.dinv32[i] * ri;
+      a.p[i] = zi + fb * pi;
+      v[0] = (double)ri * (double)ri;
+      v[1] = (double)ri * (double)zi;
+    }
+  } else {
+    const bool stop = st->stop_pending != 0;
+    if (i < a.m.n) {
+      const double hi = a.x[i], d = (double)a.xa[i];
+      const double sm = hi + d;
+      const double bp = sm - hi;
+      const double err = (hi - (sm - bp)) + (d - bp);
+      a.x[i] = sm;
+      a.xlo[i] += err;
+      a.xa[i] = (VT)0;
+      if (!stop) {
+        const VT ri = (VT)a.r64[i];
+        const VT mi = (VT)a.dinv32[i];
+        a.p[i] += mi * (ri - a.r[i]);
+        a.r[i] = ri;
+        const VT zi = mi * ri;
+        v[0] = (double)ri * (double)ri;
+        v[1] = (double)ri * (double)zi;
+      }
+    }
+  }
+  mxr_partials<VT, 2>(a, v, sh);
+}
+
+// ends a slot: a pending stop, or (r, D^-1 r) for the next alpha and the
+// step-or-update decision for the next slot
+template <class VT>
+__global__ __launch_bounds__(kRedB) void k_mxr_red_b(MixArgs<VT> a) {
+  __shared__ double sh[4 * (kRedB / 64)];
+  PcgState *st = a.st;
+  if (st->done) return;
+  if (st->stop_pending) {
+    if (threadIdx.x == 0) {
+      st->done = 1;
+      st->converged = st->conv_pending;
+    }
+    return;
+  }
+  double t[2];
+  mxr_totals<VT, 2>(a, t, sh);
+  if (threadIdx.x == 0) {
+    const double rr = t[0];
+    st->rho[0] = t[1];
+    const int itp = st->iterp[0];
+    st->refine[0] = (rr < st->thr || rr < a.delta2 * st->rr_ref ||
+                     itp - st->last_refine >= a.max_interval || itp >= st->maxit)
+                        ? 1
+                        : 0;
+  }
+}
+
+template <int FMT, class VT>
+void launch_mxr_chunk(hipStream_t s, const MixArgs<VT> &m, int count) {
+  const dim3 G(m.nwg), B(kRB), B1(kRedB);
+  for (int k = 0; k < count; ++k) {
+    hipLaunchKernelGGL((k_mxr_a<FMT, VT>), G, B, 0, s, m);
+    hipLaunchKernelGGL(k_mxr_red_a<VT>, dim3(1), B1, 0, s, m);
+    hipLaunchKernelGGL(k_mxr_b<VT>, G, B, 0, s, m);
+    hipLaunchKernelGGL(k_mxr_red_b<VT>, dim3(1), B1, 0, s, m);
+  }
+}
+
 template <int FMT, int SB>
 void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count) {
   const dim3 G(sw.ngroups), B(SB);
@@ -1177,6 +1588,27 @@ int run_mixed(hipStream_t s, const lfgpu_solver_cache *c, const Sweep &sw, const
       launch_mix_chunk<FMT_DIA, VT>(s, sw, m, check);
     else
       launch_mix_chunk<FMT_ELL, VT>(s, sw, m, check);
+    LFG_HIP(hipGetLastError());
+    LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    LFG_HIP(hipStreamSynchronize(s));
+    if (hs->done) break;
+  }
+  hipLaunchKernelGGL(k_mix_final<VT>, dim3(gflat), dim3(256), 0, s, m);
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
+}
+
+// mixed precision on sliced ELL (k_mxr_*): set-up, slots until done, x += xlo
+template <class VT>
+int run_mixed_rows(hipStream_t s, const lfgpu_solver_cache *c, const MixArgs<VT> &m, int32_t n,
+                   int check, PcgState *hs) {
+  const unsigned gflat = grid_for(n, 256);
+  hipLaunchKernelGGL((k_mxr_init<FMT_ELL, VT>), dim3(m.nwg), dim3(kRB), 0, s, m);
+  hipLaunchKernelGGL(k_mxr_red_init<VT>, dim3(1), dim3(kRedB), 0, s, m);
+  hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), dim3(256), 0, s, m.x, n, c->st);
+  LFG_HIP(hipGetLastError());
+  for (;;) {
+    launch_mxr_chunk<FMT_ELL, VT>(s, m, check);
     LFG_HIP(hipGetLastError());
     LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     LFG_HIP(hipStreamSynchronize(s));
@@ -1944,6 +2376,11 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
 
   if (single && !c->cg1)
     LFG_TRY(dalloc(&c->cg1, (size_t)6 * n, "single-reduction PCG vectors"));
+  // sliced ELL: one-row-per-thread sweeps (k_cgr_*, k_mxr_*)
+  const bool rowsweep = !single && c->fmt == FMT_ELL;
+  const int nwg = (int)((n + kRB - 1) / kRB);
+  if (rowsweep && !c->rpart)
+    LFG_TRY(dalloc(&c->rpart, (size_t)3 * std::max(nwg, 1), "row-sweep partials"));
   PcgState h{};
   h.thr = opts->rtol * opts->rtol;
   h.maxit = opts->maxit;
@@ -1982,6 +2419,35 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
         launch_cg1_chunk<FMT_DIA>(s, sw, a1, check);
       else
         launch_cg1_chunk<FMT_ELL>(s, sw, a1, check);
+      LFG_HIP(hipGetLastError());
+      LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+      LFG_HIP(hipStreamSynchronize(s));
+      if (hs->done) break;
+    }
+    return pcg_finish(ctx, hs, false, info);
+  }
+  if (rowsweep && mixed) {
+    const double d2 = opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
+    const int mi = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
+    if (opts->mode == LFGPU_PCG_MIXED) {
+      MixArgs<float> m{mv, c->dinv32, c->dinv, x->d, c->xlo, b->d, c->r, c->r32, c->p32,
+                       c->q32, c->xa32, c->part, c->st, sw, d2, mi, c->rpart, nwg};
+      LFG_TRY(run_mixed_rows(s, c, m, n, check, hs));
+    } else {
+      MixArgs<double> m{mv, c->dinv32, c->dinv, x->d, c->xlo, b->d, c->r64, c->r, c->p,
+                        c->q, c->xa64, c->part, c->st, sw, d2, mi, c->rpart, nwg};
+      LFG_TRY(run_mixed_rows(s, c, m, n, check, hs));
+    }
+    return pcg_finish(ctx, hs, true, info);
+  }
+  if (rowsweep) {
+    const RowArgs ra{mv, c->dinv, x->d, c->r, c->p, c->q, c->rpart, nwg, c->st};
+    hipLaunchKernelGGL(k_cgr_init<FMT_ELL>, dim3(nwg), dim3(kRB), 0, s, ra, b->d);
+    hipLaunchKernelGGL(k_cgr_red_init, dim3(1), dim3(kRedB), 0, s, ra);
+    hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, c->st);
+    LFG_HIP(hipGetLastError());
+    for (;;) {
+      launch_cgr_chunk<FMT_ELL>(s, ra, check);
       LFG_HIP(hipGetLastError());
       LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
       LFG_HIP(hipStreamSynchronize(s));
