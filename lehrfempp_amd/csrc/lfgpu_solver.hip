@@ -1668,9 +1668,32 @@ __device__ __forceinline__ double seg_reduce(const double *part, int gb, int ge,
   return sh[4];
 }
 
+// K segmented sums in one pass (one pair of barriers); every thread gets the
+// totals.  sh needs 4*K doubles.
+template <int K>
+__device__ __forceinline__ void seg_reduce_k(const double *const (&slots)[K], int gb, int ge,
+                                             double (&out)[K], double *sh) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = 0.0;
+  for (int i = gb + threadIdx.x; i < ge; i += 256)
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] += slots[k][i];
+  block_sum_k<256, K>(out, sh);
+}
+
+// this workgroup's K partials
+template <int K>
+__device__ __forceinline__ void store_partials_seg(double *const (&slots)[K], double (&v)[K],
+                                                   double *sh) {
+  block_sum_k<256, K>(v, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) slots[k][blockIdx.x] = v[k];
+}
+
 template <int FMT>
 __global__ __launch_bounds__(256) void k_bcg_init(BArgs a, const double *__restrict__ b) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   double bb = 0.0, rr = 0.0, rz = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
@@ -1684,17 +1707,19 @@ __global__ __launch_bounds__(256) void k_bcg_init(BArgs a, const double *__restr
     rr += ri * ri;
     rz += ri * zi;
   }
-  store_partial(bslot(a, P_BB), bb, sh);
-  store_partial(bslot(a, P_RR + 1), rr, sh);
-  store_partial(bslot(a, P_RZ + 1), rz, sh);
+  double v[3] = {bb, rr, rz};
+  double *const sl[3] = {bslot(a, P_BB), bslot(a, P_RR + 1), bslot(a, P_RZ + 1)};
+  store_partials_seg<3>(sl, v, sh);
 }
 
 // one workgroup per problem
 __global__ __launch_bounds__(256) void k_bcg_init_finish(BArgs a) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const BProb P = a.probs[blockIdx.x];
-  const double bb = seg_reduce(bslot(a, P_BB), P.gb, P.ge, sh);
-  const double rr = seg_reduce(bslot(a, P_RR + 1), P.gb, P.ge, sh);
+  double t[2];
+  const double *const sl[2] = {bslot(a, P_BB), bslot(a, P_RR + 1)};
+  seg_reduce_k<2>(sl, P.gb, P.ge, t, sh);
+  const double bb = t[0], rr = t[1];
   if (threadIdx.x == 0) {
     BState *st = a.st + blockIdx.x;
     st->bb = bb;
@@ -1723,7 +1748,7 @@ __global__ void k_bcg_zero_if_bb0(BArgs a) {
 
 template <int FMT>
 __global__ __launch_bounds__(256) void k_bcg_spmv(BArgs a, int par) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   BState *st = a.st + G.prob;
   if (st->done) return;
@@ -1754,21 +1779,22 @@ __global__ __launch_bounds__(256) void k_bcg_spmv(BArgs a, int par) {
     qdr += dq * a.r[i];
     qdq += dq * qi;
   }
-  store_partial(bslot(a, P_PQ), pq, sh);
-  store_partial(bslot(a, P_QDR), qdr, sh);
-  store_partial(bslot(a, P_QDQ), qdq, sh);
+  double v[3] = {pq, qdr, qdq};
+  double *const sl[3] = {bslot(a, P_PQ), bslot(a, P_QDR), bslot(a, P_QDQ)};
+  store_partials_seg<3>(sl, v, sh);
 }
 
 __global__ __launch_bounds__(256) void k_bcg_update(BArgs a, int par) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   const BState *st = a.st + G.prob;
   if (st->done) return;
   const BProb P = a.probs[G.prob];
-  const double rho = seg_reduce(bslot(a, P_RZ + (par ^ 1)), P.gb, P.ge, sh);
-  const double pq = seg_reduce(bslot(a, P_PQ), P.gb, P.ge, sh);
-  const double qdr = seg_reduce(bslot(a, P_QDR), P.gb, P.ge, sh);
-  const double qdq = seg_reduce(bslot(a, P_QDQ), P.gb, P.ge, sh);
+  double t[4];
+  const double *const sl[4] = {bslot(a, P_RZ + (par ^ 1)), bslot(a, P_PQ), bslot(a, P_QDR),
+                               bslot(a, P_QDQ)};
+  seg_reduce_k<4>(sl, P.gb, P.ge, t, sh);
+  const double rho = t[0], pq = t[1], qdr = t[2], qdq = t[3];
   const double alpha = rho / pq;
   const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
   const double beta = rho_next / rho;
@@ -1783,8 +1809,9 @@ __global__ __launch_bounds__(256) void k_bcg_update(BArgs a, int par) {
     rr += ri * ri;
     rz += ri * zi;
   }
-  store_partial(bslot(a, P_RR + par), rr, sh);
-  store_partial(bslot(a, P_RZ + par), rz, sh);
+  double v[2] = {rr, rz};
+  double *const so[2] = {bslot(a, P_RR + par), bslot(a, P_RZ + par)};
+  store_partials_seg<2>(so, v, sh);
 }
 
 template <int FMT>
@@ -1843,7 +1870,7 @@ __device__ __forceinline__ double *mslot(const BMArgs<VT> &a, int k) {
 // r64 = b - A x in FP64, the FP32 recurrence started from it
 template <int FMT, class VT>
 __global__ __launch_bounds__(256) void k_bmix_init(BMArgs<VT> a) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   double bb = 0.0, rr = 0.0, rz = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
@@ -1859,19 +1886,20 @@ __global__ __launch_bounds__(256) void k_bmix_init(BMArgs<VT> a) {
     rr += ri * ri;
     rz += ri * (a.dinv[i] * ri);
   }
-  store_partial(mslot(a, MB_BB), bb, sh);
-  store_partial(mslot(a, X_RR + 1), rr, sh);
-  store_partial(mslot(a, X_RZ + 1), rz, sh);
+  double v[3] = {bb, rr, rz};
+  double *const sl[3] = {mslot(a, MB_BB), mslot(a, X_RR + 1), mslot(a, X_RZ + 1)};
+  store_partials_seg<3>(sl, v, sh);
 }
 
 // one workgroup per problem: Eigen's early exits as in k_cg_init_finish
 template <class VT>
 __global__ __launch_bounds__(256) void k_bmix_init_finish(BMArgs<VT> a) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const BProb P = a.probs[blockIdx.x];
-  const double bb = seg_reduce(mslot(a, MB_BB), P.gb, P.ge, sh);
-  const double rr = seg_reduce(mslot(a, X_RR + 1), P.gb, P.ge, sh);
-  const double rz = seg_reduce(mslot(a, X_RZ + 1), P.gb, P.ge, sh);
+  double t[3];
+  const double *const sl[3] = {mslot(a, MB_BB), mslot(a, X_RR + 1), mslot(a, X_RZ + 1)};
+  seg_reduce_k<3>(sl, P.gb, P.ge, t, sh);
+  const double bb = t[0], rr = t[1], rz = t[2];
   if (threadIdx.x == 0) {
     BMState *st = a.st + blockIdx.x;
     st->bb = bb;
@@ -1910,7 +1938,7 @@ __global__ void k_bmix_zero_if_bb0(BMArgs<VT> a) {
 // problem's segmented partials.
 template <int FMT, class VT>
 __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   BMState *st = a.st + G.prob;
   if (st->done) return;
@@ -1936,9 +1964,9 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
       qdr += dq * (double)a.r[i];
       qdq += dq * (double)qi;
     }
-    store_partial(mslot(a, X_PQ), pq, sh);
-    store_partial(mslot(a, X_QDR), qdr, sh);
-    store_partial(mslot(a, X_QDQ), qdq, sh);
+    double v[3] = {pq, qdr, qdq};
+    double *const sl[3] = {mslot(a, X_PQ), mslot(a, X_QDR), mslot(a, X_QDQ)};
+    store_partials_seg<3>(sl, v, sh);
   } else {
     double rt = 0.0;
     const VT *__restrict__ xa = a.xa;
@@ -1957,7 +1985,7 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
 
 template <class VT>
 __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   BMState *st = a.st + G.prob;
   if (st->done) return;
@@ -1965,10 +1993,11 @@ __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
   const bool writer = blockIdx.x == (unsigned)G.gb && threadIdx.x == 0;
   double rr = 0.0, rz = 0.0;
   if (!st->refine[par]) {
-    const double rho = seg_reduce(mslot(a, X_RZ + (par ^ 1)), P.gb, P.ge, sh);
-    const double pq = seg_reduce(mslot(a, X_PQ), P.gb, P.ge, sh);
-    const double qdr = seg_reduce(mslot(a, X_QDR), P.gb, P.ge, sh);
-    const double qdq = seg_reduce(mslot(a, X_QDQ), P.gb, P.ge, sh);
+    double t[4];
+    const double *const sl[4] = {mslot(a, X_RZ + (par ^ 1)), mslot(a, X_PQ), mslot(a, X_QDR),
+                                 mslot(a, X_QDQ)};
+    seg_reduce_k<4>(sl, P.gb, P.ge, t, sh);
+    const double rho = t[0], pq = t[1], qdr = t[2], qdq = t[3];
     const double alpha = rho / pq;
     const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
     const VT fa = (VT)alpha, fb = (VT)(rho_next / rho);
@@ -2018,8 +2047,9 @@ __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
     }
     if (stop) return;
   }
-  store_partial(mslot(a, X_RR + par), rr, sh);
-  store_partial(mslot(a, X_RZ + par), rz, sh);
+  double v[2] = {rr, rz};
+  double *const so[2] = {mslot(a, X_RR + par), mslot(a, X_RZ + par)};
+  store_partials_seg<2>(so, v, sh);
 }
 
 template <class VT>
