@@ -18,6 +18,8 @@ Other configurations (each prints its own JSON line):
                     one graph-replayed batched PCG
   --config adjoint  1024 x 1024 P1, J = 1/2 (u - u_obs)^T M (u - u_obs), dJ/dtheta
                     by an adjoint solve, checked against central differences
+  --config fkpp     Fisher/KPP Strang splitting (SDIRK-2 diffusion + exact
+                    logistic reaction) on a 1024 x 1024 mesh, 100 time steps
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config p1]
 """
@@ -492,6 +494,108 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
     return out
 
 
+def _fkpp_setup(capi, ctx, n, c=1.2, h=1.0):
+    from lehrfempp_amd import fisher_kpp
+    xy, cells, _ = capi.tp_triag_mesh(n)
+    ev, ce, _ = capi.mesh_edges(xy.shape[0], cells)
+    mesh = capi.Mesh(ctx, xy, cells)
+    edges = capi.Edges(ctx, mesh, 1, ev)
+    ef = fisher_kpp.boundary_edge_flags(ev.shape[0], ce)
+    A, M = fisher_kpp.assemble_galerkin_matrices(ctx, mesh, edges, ef,
+                                                 capi.coeff(capi.COEFF_CONST, c),
+                                                 capi.coeff(capi.COEFF_CONST, h))
+    u0 = np.zeros(xy.shape[0])
+    r2 = (xy[:, 0] - 0.3) ** 2 + (xy[:, 1] - 0.4) ** 2
+    u0[r2 < 0.01] = 0.3
+    return mesh, edges, A, M, u0
+
+
+def run_fkpp(args, capi, ctx):
+    """Fisher/KPP Strang splitting (projects/fisher_kpp): c = 1.2, lambda =
+    2.1, T = 1, m steps (convergence_meshwidth_main.cc), Robin coefficient
+    h = 1 on the boundary, carrying capacity K = 1, u0 = 0.3 on a disc."""
+    from lehrfempp_amd import fisher_kpp
+    n, m_steps, T, lam = args.fkpp_n, args.fkpp_steps, 1.0, 2.1
+    mesh, edges, A, M, u0 = _fkpp_setup(capi, ctx, n)
+    nv = u0.size
+    ss = fisher_kpp.StrangSplit(ctx, A, M, T, m_steps, lam, rtol=1e-10)
+    vK, vu = capi.Vec(ctx, np.ones(nv)), capi.Vec(ctx, u0)
+    out = ss.Evolution(vK, vu)  # warm-up: builds the solver storage of A, S1, S2
+    out.close()
+    runs = []
+    for _ in range(max(1, args.steps)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        out = ss.Evolution(vK, vu)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        info = ss.last_info
+        runs.append((wall, info.ms, info.iters, info.max_iters, info.solves, info.converged))
+        u = out.numpy()
+        out.close()
+    wall = float(np.median([r[0] for r in runs]))
+    _, dev_ms, iters, max_it, solves, conv = runs[-1]
+    nd = solves // 2
+    res = {"metric": "Fisher/KPP Strang splitting, ms per time step", "unit": "ms",
+           "value": 1e3 * wall / nd, "higher_is_better": False,
+           "config": {"workload": "Fisher/KPP on %dx%d P1 structured triangles, c=1.2, "
+                                  "lambda=2.1, Robin h=1, K=1, T=1, %d steps, SDIRK-2 "
+                                  "diffusion + exact logistic reaction, stage solves by "
+                                  "Jacobi-PCG to 1e-10" % (n, n, m_steps),
+                      "dofs": nv, "nnz": int(A.info()[2])},
+           "evolution_s": wall, "device_ms": dev_ms, "diffusion_steps": nd,
+           "solves": solves, "iters_per_solve": iters / max(solves, 1),
+           "max_iters": max_it, "converged": bool(conv),
+           "mass_final": float(u.sum() / nv)}
+    log("fkpp %d^2: %.2f ms/step, %.1f iterations/solve (max %d)"
+        % (n, res["value"], res["iters_per_solve"], max_it))
+    ss.close()
+    for hh in (vK, vu, A, M, edges, mesh):
+        hh.close()
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = fkpp_cpu_sample(capi, ctx, args.fkpp_sample_n, m_steps)
+    return res
+
+
+def fkpp_cpu_sample(capi, ctx, n, m_steps):
+    """The oracle's StrangSplit (SciPy SuperLU, one thread) against the device
+    stepper on the same n x n problem: time per step and the difference."""
+    import scipy.sparse as sp
+    from lehrfempp_amd import fisher_kpp
+    from oracle import lfref as L  # CPU restatement, timed here as the CPU baseline
+    T, lam = 1.0, 2.1
+    mesh, edges, A, M, u0 = _fkpp_setup(capi, ctx, n)
+    nv = u0.size
+    ss = fisher_kpp.StrangSplit(ctx, A, M, T, m_steps, lam, rtol=1e-12)
+    vK, vu = capi.Vec(ctx, np.ones(nv)), capi.Vec(ctx, u0)
+    out = ss.Evolution(vK, vu)
+    out.close()
+    ctx.sync()
+    t0 = time.perf_counter()
+    out = ss.Evolution(vK, vu)
+    ctx.sync()
+    gpu_s = time.perf_counter() - t0
+    nd = ss.last_info.solves // 2
+    ug = out.numpy()
+    rA, cA, vA = A.download()
+    rM, cM, vM = M.download()
+    As = sp.csr_matrix((vA, cA, rA), shape=(nv, nv))
+    Ms = sp.csr_matrix((vM, cM, rM), shape=(nv, nv))
+    t0 = time.perf_counter()
+    ref = L.StrangSplit(As, Ms, T, m_steps, lam)
+    uc = ref.evolution(np.ones(nv), u0)
+    cpu_s = time.perf_counter() - t0
+    for hh in (out, vK, vu, ss, A, M, edges, mesh):
+        hh.close()
+    log("fkpp cpu sample %d^2: cpu %.1f ms/step, gpu %.2f ms/step" %
+        (n, 1e3 * cpu_s / nd, 1e3 * gpu_s / nd))
+    return {"value": 1e3 * cpu_s / nd, "unit": "ms per time step", "cores": 1, "kind": "port",
+            "sample": "oracle StrangSplit with SciPy SuperLU (factorisations included) on the "
+                      "%dx%d problem (%d DOFs), %d steps" % (n, n, nv, m_steps),
+            "gpu_ms_per_step_same_problem": 1e3 * gpu_s / nd,
+            "max_rel_diff_gpu_vs_cpu": float(np.abs(ug - uc).max() / np.abs(uc).max())}
+
+
 def run_traffic(args, capi, ctx):
     """A short fixed sequence for HBM byte counters (rocprofv3 --pmc): the
     FP64 SpMV of the 2048x2048 system, a few PCG iterations and a plain copy
@@ -524,7 +628,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint", "traffic"],
+    ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint", "traffic", "fkpp"],
                     default="p1")
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
     ap.add_argument("--batch-mode", choices=["fp64", "mixed", "mixed_matrix"], default="mixed")
@@ -534,6 +638,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="p3: assembly only")
     ap.add_argument("--fd-dirs", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--fkpp-n", type=int, default=1024)
+    ap.add_argument("--fkpp-steps", type=int, default=100)
+    ap.add_argument("--fkpp-sample-n", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path; "
@@ -550,6 +657,8 @@ def main():
         out = run_batched(args, capi, ctx)
     elif args.config == "traffic":
         out = run_traffic(args, capi, ctx)
+    elif args.config == "fkpp":
+        out = run_fkpp(args, capi, ctx)
     else:
         out = adjoint_run(capi, ctx, n=1024, fd_dirs=args.fd_dirs)
         out = {"metric": "adjoint gradient cost / forward solve", "config": {

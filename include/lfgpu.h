@@ -361,6 +361,60 @@ int lfgpu_adjoint_gradient(lfgpu_ctx *ctx, const lfgpu_mesh *m,
                            const lfgpu_vec *u, const lfgpu_vec *lam,
                            lfgpu_vec *grad);
 
+/* ---- time stepping on a fixed pattern ------------------------------------
+ * Strang splitting of the Fisher/KPP equation
+ *   M u' = -A u  (diffusion)  |  u' = lambda u (1 - u / K)  (reaction)
+ * as FisherKPP::StrangSplit (projects/fisher_kpp/strangsplitting.h:28-76,
+ * :176-201; strangsplitting.cc:15-94).  A and M share one sparsity pattern
+ * (both assembled by one plan).  The stepper keeps its own copy of A and
+ * forms S1 = M + (tau kappa / 2) A and S2 = M + tau kappa A once, with
+ * tau = T / m and kappa = 1 - sqrt(2)/2 (SDIRK-2).  The two SparseLU
+ * factorisations of the reference become Jacobi-PCG solves to opts->rtol on
+ * S1 / S2, warm-started from the previous stage.  The nonlocal loss matrix L
+ * of the reference (a dense N x N matrix subtracted from A) is not
+ * supported: A is used as given (L = 0). */
+typedef struct lfgpu_strang lfgpu_strang;
+typedef struct lfgpu_strang_info {
+  int diffusion_steps; /* calls of the diffusion evolution operator */
+  int solves;          /* PCG solves (two per diffusion step)     */
+  int64_t iters;       /* PCG iterations summed over the solves   */
+  int max_iters;       /* largest iteration count of one solve    */
+  int converged;       /* 1 if every solve reached rtol           */
+  double worst_relres; /* largest final relative residual         */
+  double ms;           /* device time of the call                 */
+} lfgpu_strang_info;
+/* StrangSplit ctor: T the final time, m the number of time steps (>= 1),
+ * lambda the growth factor; opts the PCG settings of every solve. */
+int lfgpu_strang_create(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_csr *M,
+                        double T, int m, double lambda,
+                        const lfgpu_pcg_opts *opts, lfgpu_strang **out);
+int lfgpu_strang_destroy(lfgpu_strang *s);
+/* diffusionEvolutionOperator (strangsplitting.cc:15-50): one SDIRK-2 step of
+ * M u' = -A u over tau / 2 (firstcall != 0) or tau from mu; out may alias
+ * mu. */
+int lfgpu_strang_diffusion(lfgpu_ctx *ctx, lfgpu_strang *s, int firstcall,
+                           const lfgpu_vec *mu, lfgpu_vec *out,
+                           lfgpu_strang_info *info);
+/* Evolution (strangsplitting.cc:53-94): half a diffusion step, then
+ * (m - 2) times { reaction over tau, diffusion over tau }, then a reaction
+ * over tau / 2, where the reaction is the exact logistic flow
+ *   u <- cap / (1 + (cap / u - 1) exp(-lambda t))
+ * with carrying capacity cap [n].  out may alias mu. */
+int lfgpu_strang_evolution(lfgpu_ctx *ctx, lfgpu_strang *s,
+                           const lfgpu_vec *cap, const lfgpu_vec *mu,
+                           lfgpu_vec *out, lfgpu_strang_info *info);
+
+/* ---- CSR helpers for drivers that combine matrices ----------------------- */
+/* A copy of A's pattern and values (Eigen::SparseMatrix copy, as
+ * `A_ = galerkinpair.first` in strangsplitting.h:183-184), e.g. to keep an
+ * assembled matrix while the plan assembles another operator. */
+int lfgpu_csr_clone(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_csr **out);
+/* Z = a X + b Y for X and Y with identical patterns (checked), Z a new
+ * matrix on that pattern; the sum `M_ + 0.5 * tau_ * kappa_ * A_` of
+ * strangsplitting.h:196-199 with a = 1. */
+int lfgpu_csr_axpby(lfgpu_ctx *ctx, double a, const lfgpu_csr *X, double b,
+                    const lfgpu_csr *Y, lfgpu_csr **out);
+
 /* ---- small BLAS-1 helpers used by drivers ------------------------------- */
 /* y = a*x + b*y */
 int lfgpu_axpby(lfgpu_ctx *ctx, double a, const lfgpu_vec *x, double b,

@@ -42,6 +42,12 @@ class PcgInfo(C.Structure):
                 ("relres", C.c_double), ("ms", C.c_double)]
 
 
+class StrangInfo(C.Structure):
+    _fields_ = [("diffusion_steps", C.c_int), ("solves", C.c_int), ("iters", C.c_int64),
+                ("max_iters", C.c_int), ("converged", C.c_int), ("worst_relres", C.c_double),
+                ("ms", C.c_double)]
+
+
 # name -> (restype, argtypes); the list is also the export check of the tests
 SIGNATURES = {
     "lfgpu_last_error": (C.c_char_p, []),
@@ -103,6 +109,13 @@ SIGNATURES = {
     "lfgpu_error_norms": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_double),
                                     C.POINTER(C.c_double)]),
     "lfgpu_adjoint_gradient": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "lfgpu_strang_create": (C.c_int, [_P, _P, _P, C.c_double, C.c_int, C.c_double,
+                                      C.POINTER(PcgOpts), C.POINTER(_P)]),
+    "lfgpu_strang_destroy": (C.c_int, [_P]),
+    "lfgpu_strang_diffusion": (C.c_int, [_P, _P, C.c_int, _P, _P, C.POINTER(StrangInfo)]),
+    "lfgpu_strang_evolution": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(StrangInfo)]),
+    "lfgpu_csr_clone": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "lfgpu_csr_axpby": (C.c_int, [_P, C.c_double, _P, C.c_double, _P, C.POINTER(_P)]),
     "lfgpu_axpby": (C.c_int, [_P, C.c_double, _P, C.c_double, _P]),
     "lfgpu_dot": (C.c_int, [_P, _P, _P, _d]),
 }
@@ -468,6 +481,52 @@ def error_norms(ctx, mesh: Mesh, fe: FE, uh: Vec, u_qp: Vec | None = None,
 def adjoint_gradient(ctx, mesh: Mesh, fe: FE, theta: Vec, u: Vec, lam: Vec, grad: Vec):
     check(lib().lfgpu_adjoint_gradient(ctx.h, mesh.h, fe.h, theta.h, u.h, lam.h, grad.h))
     return ctx.last_ms()
+
+
+def csr_clone(ctx, A: Csr) -> Csr:
+    """lfgpu_csr_clone: an owned copy of A (pattern and values)."""
+    h = _P()
+    check(lib().lfgpu_csr_clone(ctx.h, A.h, C.byref(h)))
+    return Csr(ctx, h, owned=True)
+
+
+def csr_axpby(ctx, a, X: Csr, b, Y: Csr) -> Csr:
+    """lfgpu_csr_axpby: a X + b Y on the common pattern of X and Y."""
+    h = _P()
+    check(lib().lfgpu_csr_axpby(ctx.h, float(a), X.h, float(b), Y.h, C.byref(h)))
+    return Csr(ctx, h, owned=True)
+
+
+class Strang:
+    """Strang-splitting time stepper (lfgpu_strang_*)."""
+
+    def __init__(self, ctx, A: Csr, M: Csr, T, m, lam, rtol=1e-10, maxit=100000,
+                 mode=PCG_FP64, check_every=16):
+        self.ctx = ctx
+        opts = PcgOpts(rtol, int(maxit), mode, check_every, 0.0, 0, 0)
+        self.h = _P()
+        check(lib().lfgpu_strang_create(ctx.h, A.h, M.h, float(T), int(m), float(lam),
+                                        C.byref(opts), C.byref(self.h)))
+
+    def diffusion(self, firstcall, mu: Vec, out: Vec, raise_on_noconv=True):
+        info = StrangInfo()
+        rc = lib().lfgpu_strang_diffusion(self.ctx.h, self.h, int(bool(firstcall)), mu.h, out.h,
+                                          C.byref(info))
+        if rc != OK and (rc != ENOCONV or raise_on_noconv):
+            check(rc)
+        return info
+
+    def evolution(self, cap: Vec, mu: Vec, out: Vec, raise_on_noconv=True):
+        info = StrangInfo()
+        rc = lib().lfgpu_strang_evolution(self.ctx.h, self.h, cap.h, mu.h, out.h, C.byref(info))
+        if rc != OK and (rc != ENOCONV or raise_on_noconv):
+            check(rc)
+        return info
+
+    def close(self):
+        if self.h:
+            lib().lfgpu_strang_destroy(self.h)
+            self.h = _P()
 
 
 def copy(ctx, x: Vec, y: Vec):

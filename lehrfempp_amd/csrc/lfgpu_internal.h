@@ -164,4 +164,7 @@ int element_contract(hipStream_t s, const lfgpu_mesh *m, const lfgpu_fe *fe, con
 // Release the edge records of a plan (lfgpu_edges.hip).
 void plan_free_edges(lfgpu_plan *p);
 inline void csr_values_changed(lfgpu_csr *A) { A->val_version++; }
+// y = A x (FP64) on the context stream, without synchronisation; builds the
+// solver's storage of A on first use (lfgpu_solver.hip).
+int spmv_async(lfgpu_ctx *ctx, lfgpu_csr *A, const double *x, double *y);
 }  // namespace lfgpu

@@ -2312,6 +2312,13 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace lfgpu {
+int spmv_async(lfgpu_ctx *ctx, lfgpu_csr *A, const double *x, double *y) {
+  LFG_TRY(ensure_cache(ctx, A, false));
+  return launch_spmv(ctx->stream, A->cache, A->nrows, x, y);
+}
+}  // namespace lfgpu
+
 extern "C" int lfgpu_spmv(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *x, lfgpu_vec *y) {
   LFG_REQUIRE(ctx && Ac && x && y, LFGPU_EINVAL, "NULL argument to lfgpu_spmv");
   LFG_REQUIRE(x->n == Ac->ncols && y->n == Ac->nrows, LFGPU_EINVAL, "SpMV size mismatch");

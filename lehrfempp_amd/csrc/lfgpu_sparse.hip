@@ -279,3 +279,103 @@ extern "C" int lfgpu_adjoint_gradient(lfgpu_ctx *ctx, const lfgpu_mesh *m, const
   ctx->last_ms = ms;
   return LFGPU_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Matrix copies and same-pattern linear combinations
+// ---------------------------------------------------------------------------
+namespace {
+
+// *bad != 0 if the patterns differ anywhere
+__global__ void k_pattern_diff(const int32_t *rp0, const int32_t *c0, const int32_t *rp1,
+                               const int32_t *c1, int32_t nrows, int64_t nnz, int *bad) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool diff = false;
+  if (t <= nrows) diff = rp0[t] != rp1[t];
+  if (t < nnz) diff = diff || c0[t] != c1[t];
+  if (diff) atomicOr(bad, 1);
+}
+
+__global__ void k_vals_axpby(double a, const double *x, double b, const double *y, double *z,
+                             int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = a * x[i] + b * y[i];
+}
+
+// A new matrix with A's pattern; values copied when copy_vals.
+int csr_new_like(lfgpu_ctx *ctx, const lfgpu_csr *A, bool copy_vals, lfgpu_csr **out) {
+  lfgpu_csr *B = new lfgpu_csr();
+  B->ctx = ctx;
+  B->nrows = A->nrows;
+  B->ncols = A->ncols;
+  B->nnz = A->nnz;
+  const size_t nz = (size_t)std::max<int64_t>(A->nnz, 1);
+  int rc = dalloc(&B->row_ptr, (size_t)A->nrows + 1, "CSR row pointers");
+  if (rc == LFGPU_OK) rc = dalloc(&B->col, nz, "CSR columns");
+  if (rc == LFGPU_OK) rc = dalloc(&B->val, nz, "CSR values");
+  hipError_t e = hipSuccess;
+  if (rc == LFGPU_OK)
+    e = hipMemcpyAsync(B->row_ptr, A->row_ptr, ((size_t)A->nrows + 1) * sizeof(int32_t),
+                       hipMemcpyDeviceToDevice, ctx->stream);
+  if (rc == LFGPU_OK && e == hipSuccess && A->nnz)
+    e = hipMemcpyAsync(B->col, A->col, (size_t)A->nnz * sizeof(int32_t),
+                       hipMemcpyDeviceToDevice, ctx->stream);
+  if (rc == LFGPU_OK && e == hipSuccess && copy_vals && A->nnz)
+    e = hipMemcpyAsync(B->val, A->val, (size_t)A->nnz * sizeof(double), hipMemcpyDeviceToDevice,
+                       ctx->stream);
+  if (rc == LFGPU_OK && e != hipSuccess) rc = hip_fail(e, "CSR copy");
+  if (rc != LFGPU_OK) {
+    lfgpu_csr_destroy(B);
+    return rc;
+  }
+  *out = B;
+  return LFGPU_OK;
+}
+
+}  // namespace
+
+extern "C" int lfgpu_csr_clone(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_csr **out) {
+  LFG_REQUIRE(ctx && A && out, LFGPU_EINVAL, "NULL argument to lfgpu_csr_clone");
+  *out = nullptr;
+  LFG_TRY(csr_new_like(ctx, A, true, out));
+  LFG_HIP(hipStreamSynchronize(ctx->stream));
+  return LFGPU_OK;
+}
+
+extern "C" int lfgpu_csr_axpby(lfgpu_ctx *ctx, double a, const lfgpu_csr *X, double b,
+                               const lfgpu_csr *Y, lfgpu_csr **out) {
+  LFG_REQUIRE(ctx && X && Y && out, LFGPU_EINVAL, "NULL argument to lfgpu_csr_axpby");
+  *out = nullptr;
+  LFG_REQUIRE(X->nrows == Y->nrows && X->ncols == Y->ncols && X->nnz == Y->nnz, LFGPU_EINVAL,
+              "lfgpu_csr_axpby needs matrices of equal size and nonzero count");
+  hipStream_t s = ctx->stream;
+  if (X->row_ptr != Y->row_ptr || X->col != Y->col) {
+    int *bad = nullptr;
+    LFG_TRY(dalloc(&bad, 1, "pattern flag"));
+    int h = 0;
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(int), s);
+    if (e == hipSuccess) {
+      const int64_t cnt = std::max<int64_t>((int64_t)X->nrows + 1, X->nnz);
+      hipLaunchKernelGGL(k_pattern_diff, dim3(grid_for(cnt, 256)), dim3(256), 0, s, X->row_ptr,
+                         X->col, Y->row_ptr, Y->col, X->nrows, X->nnz, bad);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(bad);
+    if (e != hipSuccess) return hip_fail(e, "lfgpu_csr_axpby pattern check");
+    LFG_REQUIRE(h == 0, LFGPU_EINVAL, "lfgpu_csr_axpby: the two matrices have different patterns");
+  }
+  lfgpu_csr *Z = nullptr;
+  LFG_TRY(csr_new_like(ctx, X, false, &Z));
+  if (X->nnz)
+    hipLaunchKernelGGL(k_vals_axpby, dim3(grid_for(X->nnz, 256)), dim3(256), 0, s, a, X->val, b,
+                       Y->val, Z->val, X->nnz);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    lfgpu_csr_destroy(Z);
+    return hip_fail(e, "lfgpu_csr_axpby");
+  }
+  *out = Z;
+  return LFGPU_OK;
+}

@@ -445,3 +445,78 @@ def galerkin_system(mesh: TriMesh, degree: int, alpha, gamma, f_kind=F_CONST, f_
     r, c, v, b = fix_flagged(r, c, v, flags, fixed_vals, b)
     rp, col, val = set_from_triplets(r, c, v, nd)
     return rp, col, val, b
+
+
+# -- Fisher/KPP Strang splitting (projects/fisher_kpp) -------------------------
+def fisher_kpp_matrices(mesh: TriMesh, c, h):
+    """assembleGalerkinMatrices (projects/fisher_kpp/strangsplitting.h:123-174)
+    with L = 0: A = P1 stiffness with diffusion coefficient c (alpha = c,
+    gamma = 0) plus MassEdgeMatrixProvider(h) on the boundary edges, M = P1
+    mass matrix (alpha = 0, gamma = 1), both as setFromTriplets CSR without
+    Dirichlet rows.  c is a cell/point array or a constant; h = (kind, const,
+    data) an edge coefficient.  Returns ((rp, col, val) of A, of M)."""
+    nd = mesh.nv
+    free = np.zeros(nd, np.uint8)
+    _, ef = mesh.boundary_flags()
+    rpA, colA, valA, _ = galerkin_system(mesh, 1, c, 0.0, flags=free,
+                                         edge_terms=(ef, h, None, None))
+    rpM, colM, valM, _ = galerkin_system(mesh, 1, 0.0, 1.0, flags=free)
+    return (rpA, colA, valA), (rpM, colM, valM)
+
+
+SDIRK2_KAPPA = 1.0 - 0.5 * np.sqrt(2.0)  # strangsplitting.h:194
+
+
+class StrangSplit:
+    """FisherKPP::StrangSplit (strangsplitting.h:28-76, :176-201;
+    strangsplitting.cc:15-94) on scipy.sparse matrices, with SciPy's SuperLU
+    (scipy.sparse.linalg.splu) in place of Eigen::SparseLU."""
+
+    def __init__(self, A, M, T, m, lam):
+        from scipy.sparse.linalg import splu
+        self.A = A.tocsr()
+        self.m = int(m)
+        self.lam = float(lam)
+        self.kappa = 1.0 - 0.5 * np.sqrt(2.0)
+        self.tau = T / m
+        # solver1 / solver2 of the constructor (strangsplitting.h:196-200)
+        self.lu1 = splu((M + 0.5 * self.tau * self.kappa * A).tocsc())
+        self.lu2 = splu((M + self.tau * self.kappa * A).tocsc())
+
+    def diffusion(self, firstcall, mu):
+        """diffusionEvolutionOperator (strangsplitting.cc:15-50)."""
+        A, tau, kappa = self.A, self.tau, self.kappa
+        rhs = -(A @ mu)
+        if firstcall:
+            k1 = self.lu1.solve(rhs)
+            k2 = self.lu1.solve(rhs - 0.5 * tau * (1 - kappa) * (A @ k1))
+            return mu + 0.5 * tau * (1 - kappa) * k1 + 0.5 * tau * kappa * k2
+        k1 = self.lu2.solve(rhs)
+        k2 = self.lu2.solve(rhs - tau * (1 - kappa) * (A @ k1))
+        return mu + tau * (1 - kappa) * k1 + tau * kappa * k2
+
+    @staticmethod
+    def reaction(cap, u, e):
+        """cap ./ (1 + (cap ./ u - 1) e) (strangsplitting.cc:75-77, :85-87)."""
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return cap / (1.0 + (cap / u - 1.0) * e)
+
+    def evolution(self, cap, mu):
+        """Evolution (strangsplitting.cc:53-94): half a diffusion step, then
+        for i = 1 .. m-2 a reaction over tau and a diffusion step, then a
+        reaction over tau / 2."""
+        import math
+        cur = self.diffusion(True, mu)
+        for _ in range(1, self.m - 1):
+            cur = self.reaction(cap, cur, math.exp(-self.lam * self.tau))
+            cur = self.diffusion(False, cur)
+        return self.reaction(cap, cur, math.exp(-self.lam * self.tau / 2.))
+
+
+def sdirk2_stability(z, kappa=SDIRK2_KAPPA):
+    """R(z) = 1 + z b^T (I - z A_bt)^{-1} 1 of the SDIRK-2 Butcher tableau
+    A_bt = [[kappa, 0], [1 - kappa, kappa]], b = (1 - kappa, kappa): the factor
+    one step multiplies a mode of y' = (z / h) y by."""
+    Abt = np.array([[kappa, 0.0], [1.0 - kappa, kappa]])
+    b = np.array([1.0 - kappa, kappa])
+    return 1.0 + z * b @ np.linalg.solve(np.eye(2) - z * Abt, np.ones(2))

@@ -7,6 +7,8 @@ otherwise within the tolerance written in each test (1e-12 relative to the
 row's infinity norm for assembled matrices, as the element couplings across
 right angles are cancellation-prone near-zeros).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -957,3 +959,175 @@ def test_robin_problem_solution_on_gpu(ctx, degree):
     assert np.abs(x.numpy() - xi).max() < 1e-10
     for hh in (x, rhs, vf, vh, plan, fe, E, g):
         hh.close()
+
+
+# --------------------------------------------- Fisher/KPP Strang time stepping
+def _fkpp_gpu(ctx, m, c=1.2, h=1.0):
+    from lehrfempp_amd import fisher_kpp
+    g = capi.Mesh(ctx, m.xy, m.cells)
+    E = capi.Edges(ctx, g, 1, m.edges)
+    ef = fisher_kpp.boundary_edge_flags(m.nedges, m.cell_e)
+    assert np.array_equal(ef, m.boundary_flags()[1])
+    A, M = fisher_kpp.assemble_galerkin_matrices(ctx, g, E, ef, capi.coeff(capi.COEFF_CONST, c),
+                                                 capi.coeff(capi.COEFF_CONST, h))
+    return g, E, A, M
+
+
+def _scipy(csr_triplet, n):
+    import scipy.sparse as sp
+    rp, col, val = csr_triplet
+    return sp.csr_matrix((val, col, rp), shape=(n, n))
+
+
+def test_fisher_kpp_matrices_and_csr_combinations(ctx):
+    # assembleGalerkinMatrices on the device equals the oracle bit for bit;
+    # clone copies, axpby combines value by value on the shared pattern
+    m = L.tp_triag_mesh(11)
+    g, E, A, M = _fkpp_gpu(ctx, m, c=1.2, h=0.7)
+    (ra, ca, va), (rm, cm, vm) = L.fisher_kpp_matrices(m, 1.2, (L.G_CONST, 0.7, None))
+    rpA, cA, vA = A.download()
+    rpM, cM, vM = M.download()
+    assert np.array_equal(rpA, ra) and np.array_equal(cA, ca) and np.array_equal(vA, va)
+    assert np.array_equal(rpM, rm) and np.array_equal(cM, cm) and np.array_equal(vM, vm)
+    Z = capi.csr_axpby(ctx, 1.0, M, 0.37, A)
+    rz, cz, vz = Z.download()
+    assert np.array_equal(rz, rpM) and np.array_equal(cz, cM)
+    assert np.array_equal(vz, 1.0 * vM + 0.37 * vA)
+    Cl = capi.csr_clone(ctx, A)
+    assert all(np.array_equal(a, b) for a, b in zip(Cl.download(), (rpA, cA, vA)))
+    # same size and nonzero count, different pattern
+    n = 5
+    X = capi.coo_to_csr(ctx, np.arange(n), np.arange(n), np.ones(n), n, n)
+    cols = np.arange(n)
+    cols[0] = 1
+    Y = capi.coo_to_csr(ctx, np.arange(n), cols, np.ones(n), n, n)
+    with pytest.raises(capi.LfgpuError):
+        capi.csr_axpby(ctx, 1.0, X, 1.0, Y)
+    with pytest.raises(capi.LfgpuError):
+        capi.csr_axpby(ctx, 1.0, X, 1.0, A)
+    for h in (X, Y, Cl, Z, A, M, E, g):
+        h.close()
+
+
+def test_strang_diffusion_operator_matches_oracle(ctx):
+    # diffusionEvolutionOperator: first (tau/2) and later (tau) calls against
+    # the oracle's SuperLU stages; PCG to 1e-13, tolerance 1e-11 of max|out|
+    from lehrfempp_amd import fisher_kpp
+    m = L.tp_triag_mesh(20)
+    g, E, A, M = _fkpp_gpu(ctx, m)
+    T, steps, lam = 1.0, 10, 2.1
+    ss = fisher_kpp.StrangSplit(ctx, A, M, T, steps, lam, rtol=1e-13)
+    (ra, ca, va), (rm, cm, vm) = L.fisher_kpp_matrices(m, 1.2, (L.G_CONST, 1.0, None))
+    ref = L.StrangSplit(_scipy((ra, ca, va), m.nv), _scipy((rm, cm, vm), m.nv), T, steps, lam)
+    mu = np.random.default_rng(3).random(m.nv)
+    vmu = capi.Vec(ctx, mu)
+    for first in (True, False, True):
+        out = ss.diffusionEvolutionOperator(first, vmu)
+        info = ss.last_info
+        assert info.converged and info.solves == 2 and info.diffusion_steps == 1
+        want = ref.diffusion(first, mu)
+        got = out.numpy()
+        assert np.abs(got - want).max() < 1e-11 * np.abs(want).max(), first
+        out.close()
+    ss.close()
+    for h in (vmu, A, M, E, g):
+        h.close()
+
+
+@pytest.mark.parametrize("steps", [1, 2, 12])
+def test_strang_evolution_matches_oracle(ctx, steps):
+    # Evolution with the reaction (lambda = 2.1, carrying capacity K varying
+    # in space) from a point source as in convergence_meshwidth_main.cc
+    from lehrfempp_amd import fisher_kpp
+    m = L.tp_triag_mesh(24)
+    g, E, A, M = _fkpp_gpu(ctx, m)
+    T, lam = 1.0, 2.1
+    K = 0.8 + 0.4 * m.xy[:, 0]
+    u0 = np.zeros(m.nv)
+    u0[13] = 0.3
+    u0[300] = 0.5
+    ss = fisher_kpp.StrangSplit(ctx, A, M, T, steps, lam, rtol=1e-13)
+    vK, vu = capi.Vec(ctx, K), capi.Vec(ctx, u0)
+    out = ss.Evolution(vK, vu)
+    info = ss.last_info
+    nd = 1 + max(steps - 2, 0)
+    assert info.converged and info.diffusion_steps == nd and info.solves == 2 * nd
+    (ra, ca, va), (rm, cm, vm) = L.fisher_kpp_matrices(m, 1.2, (L.G_CONST, 1.0, None))
+    ref = L.StrangSplit(_scipy((ra, ca, va), m.nv), _scipy((rm, cm, vm), m.nv), T, steps, lam)
+    want = ref.evolution(K, u0)
+    got = out.numpy()
+    assert np.abs(got - want).max() < 1e-10 * np.abs(want).max()
+    # in-place: out aliases mu
+    capi.copy(ctx, vu, out)
+    ss._s.evolution(vK, out, out)
+    assert np.abs(out.numpy() - want).max() < 1e-10 * np.abs(want).max()
+    ss.close()
+    for h in (out, vK, vu, A, M, E, g):
+        h.close()
+
+
+def test_strang_reaction_only_is_bitexact(ctx):
+    # c = 0, h = 0: A is all (explicit) zeros, the diffusion steps are exact
+    # identities and the fused reaction reproduces the oracle bit for bit;
+    # both equal the logistic flow over t = (m-2) tau + tau/2
+    from lehrfempp_amd import fisher_kpp
+    m = L.tp_triag_mesh(16)
+    g, E, A, M = _fkpp_gpu(ctx, m, c=0.0, h=0.0)
+    T, steps, lam = 1.3, 7, 2.1
+    rng = np.random.default_rng(8)
+    K = 0.5 + rng.random(m.nv)
+    u0 = 0.01 + rng.random(m.nv)
+    ss = fisher_kpp.StrangSplit(ctx, A, M, T, steps, lam)
+    vK, vu = capi.Vec(ctx, K), capi.Vec(ctx, u0)
+    got = ss.Evolution(vK, vu)
+    assert ss.last_info.iters == 0
+    (ra, ca, va), (rm, cm, vm) = L.fisher_kpp_matrices(m, 0.0, (L.G_CONST, 0.0, None))
+    assert not va.any()
+    ref = L.StrangSplit(_scipy((ra, ca, va), m.nv), _scipy((rm, cm, vm), m.nv), T, steps, lam)
+    want = ref.evolution(K, u0)
+    assert np.array_equal(got.numpy(), want)
+    tau = T / steps
+    flow = K / (1 + (K / u0 - 1) * math.exp(-lam * ((steps - 2) * tau + tau / 2)))
+    assert np.abs(want - flow).max() < 1e-14
+    ss.close()
+    for h in (got, vK, vu, A, M, E, g):
+        h.close()
+
+
+def test_strang_conserves_mass_on_a_large_mesh(ctx):
+    # pure Neumann diffusion (h = 0) without reaction (lambda = 0) keeps
+    # 1^T M u: the drift per stage is bounded by the solve's residual
+    from lehrfempp_amd import fisher_kpp
+    n = 256
+    m = L.tp_triag_mesh(n)
+    g, E, A, M = _fkpp_gpu(ctx, m, c=1.2, h=0.0)
+    rng = np.random.default_rng(4)
+    u0 = rng.random(m.nv)
+    ss = fisher_kpp.StrangSplit(ctx, A, M, 0.5, 8, 0.0, rtol=1e-12)
+    vK, vu = capi.Vec(ctx, np.ones(m.nv)), capi.Vec(ctx, u0)
+    out = ss.Evolution(vK, vu)
+    assert ss.last_info.converged
+    Ms = _scipy(M.download(), m.nv)
+    u = out.numpy()
+    m0 = (Ms @ u0).sum()
+    assert abs((Ms @ u).sum() - m0) < 1e-10 * m0
+    assert np.abs(u - u0).max() > 1e-2
+    ss.close()
+    for h in (out, vK, vu, A, M, E, g):
+        h.close()
+
+
+def test_strang_invalid_arguments(ctx):
+    m = L.tp_triag_mesh(4)
+    g, E, A, M = _fkpp_gpu(ctx, m)
+    with pytest.raises(capi.LfgpuError):
+        capi.Strang(ctx, A, M, 1.0, 0, 1.0)
+    ss = capi.Strang(ctx, A, M, 1.0, 3, 1.0)
+    small, ok = capi.Vec(ctx, 3), capi.Vec(ctx, m.nv)
+    with pytest.raises(capi.LfgpuError):
+        ss.evolution(small, ok, ok)
+    with pytest.raises(capi.LfgpuError):
+        ss.diffusion(True, ok, small)
+    ss.close()
+    for h in (small, ok, A, M, E, g):
+        h.close()

@@ -562,3 +562,109 @@ def test_robin_problem_reproduces_linear_solution(degree):
     x = L.band_cholesky_solve(rp, col, val, b)
     xi = _interpolant(m, degree, u)
     assert np.abs(x - xi).max() < 1e-11, np.abs(x - xi).max()
+
+
+# -- Fisher/KPP Strang splitting ------------------------------------------------
+def _fkpp_scipy(n, c=1.2, h=1.0):
+    import scipy.sparse as sp
+    m = L.tp_triag_mesh(n)
+    (rA, cA, vA), (rM, cM, vM) = L.fisher_kpp_matrices(m, c, (L.G_CONST, h, None))
+    A = sp.csr_matrix((vA, cA, rA), shape=(m.nv, m.nv))
+    M = sp.csr_matrix((vM, cM, rM), shape=(m.nv, m.nv))
+    return m, A, M
+
+
+def test_fisher_kpp_matrices_structure():
+    # A: P1 stiffness rows sum to zero, so A 1 is the Robin edge mass
+    # applied to ones = h * (edge lengths shared by the node) on boundary
+    # nodes; M 1 sums to the area; both symmetric on one pattern
+    m, A, M = _fkpp_scipy(6, c=1.2, h=0.5)
+    nf, _ = m.boundary_flags()
+    a1 = A @ np.ones(m.nv)
+    assert np.abs(a1[nf == 0]).max() < 1e-13
+    corner = np.array([0, 6, 42, 48])
+    expect = np.where(nf == 1, 0.5 * (1 / 6), 0.0)
+    expect[corner] = 0.5 * (1 / 6)  # two half-edges of length 1/6 each
+    assert np.allclose(a1, expect, atol=1e-14)
+    assert abs((M @ np.ones(m.nv)).sum() - 1.0) < 1e-14
+    assert abs(A - A.T).max() < 1e-15 and abs(M - M.T).max() < 1e-15
+    assert np.array_equal(A.indptr, M.indptr) and np.array_equal(A.indices, M.indices)
+
+
+def test_strang_diffusion_on_generalised_eigenvectors():
+    # A v = mu M v: one diffusion step multiplies v by the SDIRK-2 stability
+    # function R(-h mu), h = tau / 2 on the first call and tau afterwards;
+    # R is computed from the Butcher tableau, independently of the stepper
+    import scipy.linalg as sl
+    _, A, M = _fkpp_scipy(5, c=1.2, h=1.0)
+    T, nsteps = 0.7, 9
+    ss = L.StrangSplit(A, M, T, nsteps, 2.1)
+    w, V = sl.eigh(A.toarray(), M.toarray())
+    tau = T / nsteps
+    for k in (0, 3, len(w) // 2, len(w) - 1):
+        v = V[:, k]
+        for first, h in ((True, 0.5 * tau), (False, tau)):
+            got = ss.diffusion(first, v)
+            want = L.sdirk2_stability(-h * w[k]) * v
+            assert np.abs(got - want).max() < 1e-11 * np.abs(v).max(), (k, first)
+
+
+def test_sdirk2_stability_known_values():
+    # R(0) = 1; second order: R(z) = 1 + z + z^2/2 + O(z^3); L-stable: R -> 0
+    k = L.SDIRK2_KAPPA
+    assert abs(k - (1 - 1 / math.sqrt(2))) < 3e-16
+    assert L.sdirk2_stability(0.0) == 1.0
+    z = 1e-3
+    assert abs(L.sdirk2_stability(z) - (1 + z + z * z / 2)) < 1e-9
+    assert abs(L.sdirk2_stability(-1e8)) < 1e-7
+
+
+@pytest.mark.parametrize("nsteps", [1, 2, 3, 10])
+def test_strang_reaction_only_is_logistic_flow(nsteps):
+    # with A = 0 the diffusion steps are the identity and the splitting
+    # composes exact logistic flows: u(t) = K / (1 + (K/u0 - 1) e^{-lambda t})
+    # over t = max(m - 2, 0) tau + tau / 2 (the reference's loop runs
+    # i = 1 .. m-2)
+    import scipy.sparse as sp
+    _, A, M = _fkpp_scipy(4)
+    Z = sp.csr_matrix((np.zeros_like(A.data), A.indices, A.indptr), shape=A.shape)
+    T, lam = 1.3, 2.1
+    ss = L.StrangSplit(Z, M, T, nsteps, lam)
+    rng = np.random.default_rng(5)
+    K = 0.5 + rng.random(A.shape[0])
+    u0 = 0.01 + rng.random(A.shape[0])
+    got = ss.evolution(K, u0)
+    tau = T / nsteps
+    t = max(nsteps - 2, 0) * tau + tau / 2
+    want = K / (1 + (K / u0 - 1) * math.exp(-lam * t))
+    assert np.abs(got - want).max() < 1e-14
+
+
+def test_strang_conserves_mass_without_reaction_and_robin_term():
+    # pure Neumann diffusion (h = 0), lambda = 0: 1^T M u is invariant since
+    # 1^T A = 0, and the reaction with lambda = 0 is the identity
+    _, A, M = _fkpp_scipy(6, c=1.2, h=0.0)
+    ss = L.StrangSplit(A, M, 1.0, 12, 0.0)
+    rng = np.random.default_rng(1)
+    u0 = rng.random(A.shape[0])
+    u = ss.evolution(np.ones(A.shape[0]), u0)
+    assert abs((M @ u).sum() - (M @ u0).sum()) < 1e-13
+    assert np.abs(u - u0).max() > 1e-3  # it did diffuse
+
+
+def test_strang_diffusion_is_second_order_in_time():
+    # lambda = 0: evolution = exp(-M^{-1} A t) u0 with t = tau/2 + (m-2) tau,
+    # up to the SDIRK-2 error O(tau^2)
+    import scipy.linalg as sl
+    _, A, M = _fkpp_scipy(5, c=1.2, h=1.0)
+    Ad, Md = A.toarray(), M.toarray()
+    u0 = np.random.default_rng(2).random(A.shape[0])
+    errs = []
+    for nsteps in (20, 40):
+        T = 0.5
+        tau = T / nsteps
+        t = tau / 2 + (nsteps - 2) * tau
+        exact = sl.expm(-np.linalg.solve(Md, Ad) * t) @ u0
+        got = L.StrangSplit(A, M, T, nsteps, 0.0).evolution(np.ones(A.shape[0]), u0)
+        errs.append(np.abs(got - exact).max())
+    assert 3.3 < errs[0] / errs[1] < 4.7, errs
