@@ -518,7 +518,9 @@ def run_fkpp(args, capi, ctx):
     n, m_steps, T, lam = args.fkpp_n, args.fkpp_steps, 1.0, 2.1
     mesh, edges, A, M, u0 = _fkpp_setup(capi, ctx, n)
     nv = u0.size
-    ss = fisher_kpp.StrangSplit(ctx, A, M, T, m_steps, lam, rtol=1e-10)
+    mode = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
+            "mixed_matrix": capi.PCG_MIXED_MATRIX}[args.fkpp_mode]
+    ss = fisher_kpp.StrangSplit(ctx, A, M, T, m_steps, lam, rtol=1e-10, mode=mode)
     vK, vu = capi.Vec(ctx, np.ones(nv)), capi.Vec(ctx, u0)
     out = ss.Evolution(vK, vu)  # warm-up: builds the solver storage of A, S1, S2
     out.close()
@@ -543,6 +545,7 @@ def run_fkpp(args, capi, ctx):
                                   "diffusion + exact logistic reaction, stage solves by "
                                   "Jacobi-PCG to 1e-10" % (n, n, m_steps),
                       "dofs": nv, "nnz": int(A.info()[2])},
+           "pcg_mode": args.fkpp_mode,
            "evolution_s": wall, "device_ms": dev_ms, "diffusion_steps": nd,
            "solves": solves, "iters_per_solve": iters / max(solves, 1),
            "max_iters": max_it, "converged": bool(conv),
@@ -640,7 +643,8 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--fkpp-n", type=int, default=1024)
     ap.add_argument("--fkpp-steps", type=int, default=100)
-    ap.add_argument("--fkpp-sample-n", type=int, default=128)
+    ap.add_argument("--fkpp-sample-n", type=int, default=256)
+    ap.add_argument("--fkpp-mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path; "

@@ -384,7 +384,8 @@ typedef struct lfgpu_strang_info {
   double ms;           /* device time of the call                 */
 } lfgpu_strang_info;
 /* StrangSplit ctor: T the final time, m the number of time steps (>= 1),
- * lambda the growth factor; opts the PCG settings of every solve. */
+ * lambda the growth factor; opts the PCG settings of every solve
+ * (check_every <= 0: adapted per stage to the previous solve's count). */
 int lfgpu_strang_create(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_csr *M,
                         double T, int m, double lambda,
                         const lfgpu_pcg_opts *opts, lfgpu_strang **out);

@@ -501,7 +501,7 @@ class Strang:
     """Strang-splitting time stepper (lfgpu_strang_*)."""
 
     def __init__(self, ctx, A: Csr, M: Csr, T, m, lam, rtol=1e-10, maxit=100000,
-                 mode=PCG_FP64, check_every=16):
+                 mode=PCG_FP64, check_every=0):
         self.ctx = ctx
         opts = PcgOpts(rtol, int(maxit), mode, check_every, 0.0, 0, 0)
         self.h = _P()

@@ -42,6 +42,10 @@ struct lfgpu_strang {
   // device work vectors, n each: y, b2, kt1, kt2, cur
   double *work = nullptr;
   bool warm = false;  // kt2 holds a previous stage
+  // iterations of the last solve of each stage: the next solve of that stage
+  // checks for convergence about 12 times over a similar count (fewer host
+  // round trips on long solves, little overshoot on short ones)
+  int last_iters[2] = {0, 0};
   // timing events of the stepper's own calls (the solves record the
   // context's pair)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -74,6 +78,18 @@ __global__ void k_copy_vec(const double *x, double *y, int64_t n) {
   if (i < n) y[i] = x[i];
 }
 
+// options of the next solve of stage q: the caller's, with the check
+// interval adapted unless the caller fixed one
+lfgpu_pcg_opts stage_opts(const lfgpu_strang *st, int q) {
+  lfgpu_pcg_opts o = st->opts;
+  if (o.check_every <= 0) {
+    int c = st->last_iters[q] > 0 ? st->last_iters[q] / 12 : 16;
+    c = std::max(8, std::min(128, c));
+    o.check_every = c + (c & 1);
+  }
+  return o;
+}
+
 void acc_info(lfgpu_strang_info *info, const lfgpu_pcg_info &pi) {
   info->solves += 1;
   info->iters += pi.iters;
@@ -104,9 +120,11 @@ int diffusion_step(lfgpu_strang *st, bool firstcall, const double *mu, double *o
   LFG_HIP(hipGetLastError());
   lfgpu_vec vb{ctx, y, n}, vx{ctx, kt1, n};
   lfgpu_pcg_info pi{};
-  int rc = lfgpu_pcg(ctx, S, &vb, &vx, &st->opts, &pi);
+  lfgpu_pcg_opts o1 = stage_opts(st, 0);
+  int rc = lfgpu_pcg(ctx, S, &vb, &vx, &o1, &pi);
   if (rc != LFGPU_OK && rc != LFGPU_ENOCONV) return rc;
   acc_info(info, pi);
+  st->last_iters[0] = pi.iters;
   if (rc == LFGPU_ENOCONV) return rc;
   // b2 = y - s1 A kt1 (b2 holds A kt1 first)
   LFG_TRY(spmv_async(ctx, st->A, kt1, b2));
@@ -114,9 +132,11 @@ int diffusion_step(lfgpu_strang *st, bool firstcall, const double *mu, double *o
   hipLaunchKernelGGL(k_copy_vec, dim3(G), dim3(256), 0, s, kt1, kt2, n);
   LFG_HIP(hipGetLastError());
   lfgpu_vec vb2{ctx, b2, n}, vx2{ctx, kt2, n};
-  rc = lfgpu_pcg(ctx, S, &vb2, &vx2, &st->opts, &pi);
+  lfgpu_pcg_opts o2 = stage_opts(st, 1);
+  rc = lfgpu_pcg(ctx, S, &vb2, &vx2, &o2, &pi);
   if (rc != LFGPU_OK && rc != LFGPU_ENOCONV) return rc;
   acc_info(info, pi);
+  st->last_iters[1] = pi.iters;
   if (rc == LFGPU_ENOCONV) return rc;
   st->warm = true;
   hipLaunchKernelGGL(k_stage_update, dim3(G), dim3(256), 0, s, mu, kt1, kt2, s1, s2,

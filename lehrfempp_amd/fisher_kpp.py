@@ -51,7 +51,7 @@ class StrangSplit:
     """FisherKPP::StrangSplit on the device (lfgpu_strang_*)."""
 
     def __init__(self, ctx, A: capi.Csr, M: capi.Csr, T, m, lam, rtol=1e-10,
-                 mode=capi.PCG_FP64, check_every=16):
+                 mode=capi.PCG_FP64, check_every=0):
         self.ctx = ctx
         self.n = A.info()[0]
         self._s = capi.Strang(ctx, A, M, T, m, lam, rtol=rtol, mode=mode,
