@@ -479,6 +479,8 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
             "worst_err_vs_ref": float(max(errs))}
         log("batched %s: %.1f ms/step (pcg %.1f ms), iters %d..%d, worst err %.2e"
             % (name, 1e3 * wall, per_mode[name]["pcg_ms"], min(it), max(it), max(errs)))
+    plain = None if args.no_plain else plain_fp64_per_problem(args, capi, ctx, sizes, g, coff,
+                                                               voff, x_ref)
     sel = per_mode[args.batch_mode]
     out = {"metric": "batched P1 assemble + PCG, Melem/s", "value": sel["melem_s"],
            "unit": "Melem/s", "pcg_mode": args.batch_mode,
@@ -489,9 +491,56 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
            "speedup_vs_fp64": {k: per_mode["fp64"]["ms_per_step"] / v["ms_per_step"]
                                for k, v in per_mode.items()},
            "tuned_rows_per_workgroup": summ.refinements}
+    if plain is not None:
+        out["plain_fp64_per_problem"] = plain
+        out["speedup_vs_plain_fp64"] = {k: plain["ms_per_step"] / v["ms_per_step"]
+                                        for k, v in per_mode.items()}
     for hh in (batch, b, x, theta, plan, fe, mesh):
         hh.close()
     return out
+
+
+def plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff, x_ref):
+    """The same problems through the plain FP64 path: one plan, one assembly
+    and one lfgpu_pcg call per problem, in sequence."""
+    probs = []
+    for k, n in enumerate(sizes):
+        xy, cells, _ = capi.tp_triag_mesh(int(n))
+        j, i = np.divmod(np.arange(xy.shape[0]), n + 1)
+        fl = ((i == 0) | (i == n) | (j == 0) | (j == n)).astype(np.uint8)
+        mesh = capi.Mesh(ctx, xy, cells)
+        plan = capi.Plan(ctx, mesh, fl)
+        th = capi.Vec(ctx, g[coff[k]:coff[k + 1]])
+        probs.append((mesh, plan, th, capi.Vec(ctx, xy.shape[0]), capi.Vec(ctx, xy.shape[0])))
+    fe = capi.FE(ctx, 1)
+    one, zero = capi.coeff(capi.COEFF_CONST, 1.0), capi.coeff(capi.COEFF_CONST, 0.0)
+
+    def step():
+        its = []
+        for mesh, plan, th, b, x in probs:
+            plan.assemble(fe, capi.coeff(capi.COEFF_EXPCELL, 1.0, th), zero, one, b)
+            x.fill(0.0)
+            its.append(capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, maxit=MAXIT).iters)
+        return its
+
+    step()  # builds every solver mirror
+    times = []
+    for _ in range(max(1, args.steps)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        its = step()
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+    errs = [rel_err(pr[4].numpy(), x_ref[voff[k]:voff[k + 1]]) for k, pr in enumerate(probs)]
+    wall = float(np.median(times))
+    log("plain fp64 per problem: %.1f ms/step, iters %d..%d, worst err %.2e"
+        % (1e3 * wall, min(its), max(its), max(errs)))
+    for pr in probs:
+        for hh in pr[::-1]:
+            hh.close()
+    fe.close()
+    return {"ms_per_step": 1e3 * wall, "calls_per_step": 2 * len(sizes),
+            "iters_sum": int(sum(its)), "worst_err_vs_ref": float(max(errs))}
 
 
 def _fkpp_setup(capi, ctx, n, c=1.2, h=1.0):
@@ -639,6 +688,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="p3: assembly only")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched: skip the per-problem plain FP64 comparison")
     ap.add_argument("--fd-dirs", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--fkpp-n", type=int, default=1024)
