@@ -553,15 +553,28 @@ __device__ __forceinline__ void store_partials_k(double *part, double (&v)[K], c
     for (int k = 0; k < K; ++k) part[slots[k] * kMaxGrid + blockIdx.x] = v[k];
 }
 
+// Every load of a thread's partials is issued before the first addition
+// (one L2 round trip at the start of the sweep instead of one per group of
+// SB partials); the additions keep the order g = t, t + SB, ...
 template <int SB, int K>
 __device__ __forceinline__ void reduce_partials_k(const double *part, int ngroups,
                                                   const int (&slots)[K], double (&out)[K],
                                                   double *sh) {
+  constexpr int U = (kMaxGrid + SB - 1) / SB;
+  double v[K][U];
 #pragma unroll
-  for (int k = 0; k < K; ++k) out[k] = 0.0;
-  for (int g = threadIdx.x; g < ngroups; g += SB)
+  for (int u = 0; u < U; ++u) {
+    const int g = threadIdx.x + u * SB;
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[k] += part[slots[k] * kMaxGrid + g];
+    for (int k = 0; k < K; ++k) v[k][u] = g < ngroups ? part[slots[k] * kMaxGrid + g] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    out[k] = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (threadIdx.x + u * SB < ngroups) out[k] += v[k][u];
+  }
   block_sum_k<SB, K>(out, sh);
 }
 
