@@ -329,6 +329,14 @@ int lfgpu_batch_destroy(lfgpu_batch *bt);
 int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec *b,
                     lfgpu_vec *x, const lfgpu_pcg_opts *opts,
                     lfgpu_pcg_info *infos, lfgpu_pcg_info *summary);
+/* Matrix storage the batched sweeps use, chosen on the first solve (-1
+ * before it): whole-matrix DIA, per-problem DIA (every problem banded, e.g.
+ * structured meshes of different sizes: diagonals with per-problem offsets,
+ * no column indices) or sliced ELL.  All give the same iterates. */
+#define LFGPU_STORAGE_ELL 0
+#define LFGPU_STORAGE_DIA 1
+#define LFGPU_STORAGE_BLOCK_DIA 2
+int lfgpu_batch_storage(const lfgpu_batch *bt, int *storage);
 
 /* ---- error norms ---------------------------------------------------------
  * Squared L2 and H1-seminorm errors of the finite element function with DOF

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "liblfgpu.so")
 OK, EINVAL, EDEGENERATE, ENOCONV, EOOM, EHIP, ENORULE, ERANGE = range(8)
 COEFF_CONST, COEFF_CELL, COEFF_QP, COEFF_SINSIN, COEFF_EXPCELL = range(5)
 PCG_FP64, PCG_MIXED, PCG_MIXED_MATRIX = 0, 1, 2
+STORAGE_ELL, STORAGE_DIA, STORAGE_BLOCK_DIA = 0, 1, 2
 
 _P = C.c_void_p
 _d = C.POINTER(C.c_double)
@@ -104,6 +105,7 @@ SIGNATURES = {
     "lfgpu_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo)]),
     "lfgpu_batch_create": (C.c_int, [_P, _P, C.c_int, _i32, C.POINTER(_P)]),
     "lfgpu_batch_destroy": (C.c_int, [_P]),
+    "lfgpu_batch_storage": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "lfgpu_batch_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo),
                                   C.POINTER(PcgInfo)]),
     "lfgpu_error_norms": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_double),
@@ -455,6 +457,12 @@ class Batch:
         if rc != OK and (rc != ENOCONV or raise_on_noconv):
             check(rc)
         return summ, list(infos)
+
+    def storage(self):
+        """lfgpu_batch_storage: STORAGE_ELL / _DIA / _BLOCK_DIA (-1 before a solve)."""
+        v = C.c_int()
+        check(lib().lfgpu_batch_storage(self.h, C.byref(v)))
+        return v.value
 
     def close(self):
         if self.h:

@@ -38,6 +38,9 @@ constexpr int kMaxDiag = 8;
 constexpr int kMaxGrid = 2048;  // workgroups of the solver sweeps
 constexpr int kPartSlots = 8;   // partial-sum arrays of kMaxGrid doubles
 constexpr int FMT_ELL = 0, FMT_DIA = 1;
+// batches only: DIA storage whose offsets are chosen per problem (a block
+// diagonal matrix of structured meshes of different sizes)
+constexpr int FMT_BDIA = 2;
 
 }  // namespace
 
@@ -1691,7 +1694,24 @@ struct BArgs {
   const BGroup *groups;
   const BProb *probs;
   BState *st;
+  const int32_t *boff;  // FMT_BDIA: [nprob x kMaxDiag] offsets
 };
+
+// The matrix view of a workgroup of problem prob: FMT_BDIA takes that
+// problem's diagonal offsets and is then applied as DIA.
+template <int FMT>
+__device__ __forceinline__ MatView batch_view(const MatView &m0, const int32_t *boff, int prob) {
+  MatView m = m0;
+  if constexpr (FMT == FMT_BDIA) {
+#pragma unroll
+    for (int d = 0; d < kMaxDiag; ++d) m.off[d] = boff[prob * kMaxDiag + d];
+  }
+  return m;
+}
+template <int FMT>
+constexpr int applied_fmt() {
+  return FMT == FMT_BDIA ? FMT_DIA : FMT;
+}
 
 __device__ __forceinline__ double *bslot(const BArgs &a, int k) { return a.part + (int64_t)k * a.ngroups; }
 
@@ -1732,11 +1752,12 @@ template <int FMT>
 __global__ __launch_bounds__(256) void k_bcg_init(BArgs a, const double *__restrict__ b) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
+  const MatView m = batch_view<FMT>(a.m, a.boff, G.prob);
   double bb = 0.0, rr = 0.0, rz = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
     const double xi = a.x[i];
     const double bi = b[i];
-    const double ri = bi - apply64<FMT>(a.m, a.x, i, xi);
+    const double ri = bi - apply64<applied_fmt<FMT>()>(m, a.x, i, xi);
     a.r[i] = ri;
     const double zi = a.dinv[i] * ri;
     a.p[i] = zi;
@@ -1806,14 +1827,15 @@ __global__ __launch_bounds__(256) void k_bcg_spmv(BArgs a, int par) {
     st->iter += 1;
     if (st->iter >= st->maxit) st->done = 1;
   }
+  const MatView m = batch_view<FMT>(a.m, a.boff, G.prob);
   double pq = 0.0, qdr = 0.0, qdq = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-    const double pi = a.p[i];
-    const double qi = apply64<FMT>(a.m, a.p, i, pi);
+    const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
+    const double qi = apply64<applied_fmt<FMT>(), true>(m, a.p, i, pi);
     a.q[i] = qi;
-    const double dq = a.dinv[i] * qi;
+    const double dq = di * qi;
     pq += pi * qi;
-    qdr += dq * a.r[i];
+    qdr += dq * ri;
     qdq += dq * qi;
   }
   double v[3] = {pq, qdr, qdq};
@@ -1897,6 +1919,7 @@ struct BMArgs {
   BMState *st;
   double delta2;
   int max_interval;
+  const int32_t *boff;  // FMT_BDIA: [nprob x kMaxDiag] offsets
 };
 
 template <class VT>
@@ -1909,10 +1932,11 @@ template <int FMT, class VT>
 __global__ __launch_bounds__(256) void k_bmix_init(BMArgs<VT> a) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
+  const MatView m = batch_view<FMT>(a.m, a.boff, G.prob);
   double bb = 0.0, rr = 0.0, rz = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
     const double bi = a.b[i];
-    const double ri = bi - apply64<FMT>(a.m, a.x, i, a.x[i]);
+    const double ri = bi - apply64<applied_fmt<FMT>()>(m, a.x, i, a.x[i]);
     a.r64[i] = ri;
     const VT rv = (VT)ri;
     a.r[i] = rv;
@@ -1990,15 +2014,17 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
     st->iterp[par] = refine ? itp : itp + 1;
     if (!refine) st->iter = itp + 1;
   }
+  const MatView m = batch_view<FMT>(a.m, a.boff, G.prob);
+  constexpr int AF = applied_fmt<FMT>();
   if (!refine) {
     double pq = 0.0, qdr = 0.0, qdq = 0.0;
     for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-      const VT pi = a.p[i];
-      const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
+      const VT pi = a.p[i], mi = (VT)a.dinv32[i], ri = a.r[i];
+      const VT qi = apply_mixed<AF>(m, a.p, i, pi);
       a.q[i] = qi;
-      const double dq = (double)((VT)a.dinv32[i] * qi);
+      const double dq = (double)(mi * qi);
       pq += (double)pi * (double)qi;
-      qdr += dq * (double)a.r[i];
+      qdr += dq * (double)ri;
       qdq += dq * (double)qi;
     }
     double v[3] = {pq, qdr, qdq};
@@ -2008,10 +2034,10 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
     double rt = 0.0;
     const VT *__restrict__ xa = a.xa;
     for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-      const double ah = apply64<FMT>(a.m, a.x, i, a.x[i]);
+      const double ah = apply64<AF>(m, a.x, i, a.x[i]);
       const double aa =
-          apply64_fn<FMT>(a.m, i, (double)xa[i], [&](int32_t j) { return (double)xa[j]; });
-      const double al = apply64<FMT>(a.m, a.xlo, i, a.xlo[i]);
+          apply64_fn<AF>(m, i, (double)xa[i], [&](int32_t j) { return (double)xa[j]; });
+      const double al = apply64<AF>(m, a.xlo, i, a.xlo[i]);
       const double ri = ((a.b[i] - ah) - aa) - al;
       a.r64[i] = ri;
       rt += ri * ri;
@@ -2131,9 +2157,204 @@ struct lfgpu_batch {
   double graph_delta2 = 0.0;
   int graph_max_interval = 0;
   double tune_ms[4] = {0, 0, 0, 0};
+  // storage used by the batched sweeps: the matrix's own mirror (DIA or
+  // ELL) or per-problem DIA (FMT_BDIA); -1 until the first solve decides
+  int fmt = -1;
+  int bndiag = 0;
+  int32_t *d_row_off = nullptr;  // [nprob + 1]
+  int32_t *boff = nullptr;       // [nprob x kMaxDiag]
+  double *bval = nullptr;        // [bndiag x n]
+  float *bval32 = nullptr;       // [bndiag x n]
+  uint64_t bbuilt = ~0ull;       // matrix value version of bval / bval32
+  bool bbuilt32 = false;
 };
 
 namespace {
+
+// ---- per-problem DIA storage of a batch ----------------------------------
+// problem of row r: the last p with row_off[p] <= r
+__device__ __forceinline__ int batch_prob_of(const int32_t *row_off, int nprob, int32_t r) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (row_off[mid] <= r) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// For each problem: the ascending union of the off-diagonal offsets of (up
+// to) 64 consecutive rows from its middle row on (count -1 if more than
+// kMaxDiag; 0 for an empty problem), and its off-diagonal count.  Rows near
+// a boundary miss some offsets; the union over a few rows recovers the
+// interior stencil.
+__global__ void k_bdia_mid(const int32_t *row_ptr, const int32_t *col, const int32_t *row_off,
+                           int nprob, int32_t *out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  int32_t *o = out + p * (kMaxDiag + 2);
+  const int32_t a = row_off[p], b = row_off[p + 1];
+  o[kMaxDiag + 1] = (row_ptr[b] - row_ptr[a]) - (b - a);
+  if (b <= a) {
+    o[0] = 0;
+    return;
+  }
+  int32_t set[kMaxDiag];
+  int k = 0;
+  const int32_t rows = b - a, scan = rows < 64 ? rows : 64;
+  for (int32_t t = 0; t < scan; ++t) {
+    const int32_t r = a + (rows / 2 + t) % rows;
+    for (int32_t q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
+      if (col[q] == r) continue;
+      const int32_t v = col[q] - r;
+      int j = 0;
+      while (j < k && set[j] < v) ++j;
+      if (j < k && set[j] == v) continue;
+      if (k == kMaxDiag) {
+        o[0] = -1;
+        return;
+      }
+      for (int i = k; i > j; --i) set[i] = set[i - 1];
+      set[j] = v;
+      ++k;
+    }
+  }
+  for (int i = 0; i < k; ++i) o[1 + i] = set[i];
+  o[0] = k;
+}
+
+// flag |= 1 if an off-diagonal entry's offset is missing from its
+// problem's set
+__global__ void k_bdia_check(const int32_t *row_ptr, const int32_t *col, int32_t n,
+                             const int32_t *row_off, int nprob, const int32_t *boff, int nd,
+                             int *flag) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int p = batch_prob_of(row_off, nprob, r);
+  const int32_t *off = boff + p * kMaxDiag;
+  for (int32_t q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
+    const int32_t c = col[q];
+    if (c == r) continue;
+    bool found = false;
+    for (int d = 0; d < nd; ++d) found = found || off[d] == c - r;
+    if (!found) atomicOr(flag, 1);
+  }
+}
+
+// values of row r in its problem's diagonal slots (zeros elsewhere)
+__global__ void k_bdia_fill(const int32_t *row_ptr, const int32_t *col, const double *val,
+                            int32_t n, const int32_t *row_off, int nprob, const int32_t *boff,
+                            int nd, double *bval, float *bval32) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int p = batch_prob_of(row_off, nprob, r);
+  const int32_t *off = boff + p * kMaxDiag;
+  for (int d = 0; d < nd; ++d) {
+    bval[(int64_t)d * n + r] = 0.0;
+    if (bval32) bval32[(int64_t)d * n + r] = 0.0f;
+  }
+  for (int32_t q = row_ptr[r]; q < row_ptr[r + 1]; ++q) {
+    const int32_t c = col[q];
+    if (c == r) continue;
+    int d = 0;
+    while (d < nd - 1 && off[d] != c - r) ++d;
+    bval[(int64_t)d * n + r] = val[q];
+    if (bval32) bval32[(int64_t)d * n + r] = (float)val[q];
+  }
+}
+
+// Chooses the batch's storage on its first solve: the matrix mirror when it
+// is DIA already, per-problem DIA when every problem is a banded matrix
+// whose offsets near its middle row cover all of its rows (and whole
+// diagonals waste at most half again the batch's off-diagonal count), ELL
+// otherwise.
+int batch_choose_format(lfgpu_batch *bt) {
+  lfgpu_csr *A = bt->A;
+  bt->fmt = A->cache->fmt;
+  if (bt->fmt == FMT_DIA) return LFGPU_OK;
+  hipStream_t s = bt->ctx->stream;
+  const int np = bt->nprob;
+  const int32_t n = A->nrows;
+  LFG_TRY(dalloc(&bt->d_row_off, (size_t)np + 1, "batch row offsets"));
+  LFG_HIP(hipMemcpyAsync(bt->d_row_off, bt->row_off.data(), ((size_t)np + 1) * 4,
+                         hipMemcpyHostToDevice, s));
+  int32_t *mid = nullptr;
+  LFG_TRY(dalloc(&mid, (size_t)np * (kMaxDiag + 2), "batch middle rows"));
+  hipLaunchKernelGGL(k_bdia_mid, dim3(grid_for(np, 256)), dim3(256), 0, s, A->row_ptr, A->col,
+                     bt->d_row_off, np, mid);
+  std::vector<int32_t> h((size_t)np * (kMaxDiag + 2));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), mid, h.size() * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(mid);
+  if (e != hipSuccess) return hip_fail(e, "batch storage choice");
+  int nd = 0;
+  double stored = 0.0, offdiag = 0.0;
+  for (int p = 0; p < np; ++p) {
+    const int32_t *o = &h[(size_t)p * (kMaxDiag + 2)];
+    if (o[0] < 0) return LFGPU_OK;
+    stored += (double)o[0] * (bt->row_off[p + 1] - bt->row_off[p]);
+    offdiag += o[kMaxDiag + 1];
+    nd = std::max(nd, (int)o[0]);
+  }
+  if (nd == 0 || stored > 1.5 * offdiag) return LFGPU_OK;
+  // offsets padded with 0: the diagonal, which holds no off-diagonal value
+  std::vector<int32_t> off((size_t)np * kMaxDiag, 0);
+  for (int p = 0; p < np; ++p)
+    for (int d = 0; d < h[(size_t)p * (kMaxDiag + 2)]; ++d)
+      off[(size_t)p * kMaxDiag + d] = h[(size_t)p * (kMaxDiag + 2) + 1 + d];
+  LFG_TRY(dalloc(&bt->boff, off.size(), "batch diagonal offsets"));
+  LFG_HIP(hipMemcpyAsync(bt->boff, off.data(), off.size() * 4, hipMemcpyHostToDevice, s));
+  int *flag = nullptr;
+  LFG_TRY(dalloc(&flag, 1, "batch DIA flag"));
+  e = hipMemsetAsync(flag, 0, 4, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_bdia_check, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr,
+                       A->col, n, bt->d_row_off, np, bt->boff, nd, flag);
+    e = hipGetLastError();
+  }
+  int hflag = 1;
+  if (e == hipSuccess) e = hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(flag);
+  if (e != hipSuccess) return hip_fail(e, "batch DIA check");
+  if (hflag != 0) return LFGPU_OK;
+  LFG_TRY(dalloc(&bt->bval, (size_t)nd * n, "batch DIA values"));
+  bt->bndiag = nd;
+  bt->fmt = FMT_BDIA;
+  return LFGPU_OK;
+}
+
+// (Re)fills the per-problem DIA values after the matrix values changed.
+int batch_fill(lfgpu_batch *bt, bool need32) {
+  if (bt->fmt != FMT_BDIA) return LFGPU_OK;
+  lfgpu_csr *A = bt->A;
+  const int32_t n = A->nrows;
+  if (need32 && !bt->bval32) {
+    LFG_TRY(dalloc(&bt->bval32, (size_t)bt->bndiag * n, "batch DIA values FP32"));
+    bt->bbuilt = ~0ull;
+  }
+  const bool with32 = bt->bval32 != nullptr;
+  if (bt->bbuilt == A->val_version && bt->bbuilt32 == with32) return LFGPU_OK;
+  hipLaunchKernelGGL(k_bdia_fill, dim3(grid_for(n, 256)), dim3(256), 0, bt->ctx->stream,
+                     A->row_ptr, A->col, A->val, n, bt->d_row_off, bt->nprob, bt->boff,
+                     bt->bndiag, bt->bval, bt->bval32);
+  LFG_HIP(hipGetLastError());
+  bt->bbuilt = A->val_version;
+  bt->bbuilt32 = with32;
+  return LFGPU_OK;
+}
+
+// the matrix view of the batched sweeps
+MatView batch_matview(const lfgpu_batch *bt) {
+  MatView m = view_of(bt->A->cache, bt->A->nrows);
+  if (bt->fmt == FMT_BDIA) {
+    m.ndiag = bt->bndiag;
+    m.val = bt->bval;
+    m.val32 = bt->bval32;
+  }
+  return m;
+}
 
 void batch_release_graph(lfgpu_batch *bt) {
   if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
@@ -2175,8 +2396,8 @@ int batch_layout(lfgpu_batch *bt, int R) {
 
 BArgs batch_args(lfgpu_batch *bt, double *x) {
   lfgpu_solver_cache *c = bt->A->cache;
-  return BArgs{view_of(c, bt->A->nrows), c->dinv, x, c->r, c->p, c->q, bt->part, bt->ngroups,
-               bt->groups, bt->probs, bt->st};
+  return BArgs{batch_matview(bt), c->dinv, x, c->r, c->p, c->q, bt->part, bt->ngroups,
+               bt->groups, bt->probs, bt->st, bt->boff};
 }
 
 int batch_init(lfgpu_batch *bt, const double *b, double *x, double rtol, int maxit) {
@@ -2189,8 +2410,10 @@ int batch_init(lfgpu_batch *bt, const double *b, double *x, double rtol, int max
   LFG_HIP(hipMemcpyAsync(bt->st, bt->h_st, bt->nprob * sizeof(BState), hipMemcpyHostToDevice, s));
   LFG_HIP(hipStreamSynchronize(s));
   const BArgs a = batch_args(bt, x);
-  if (bt->A->cache->fmt == FMT_DIA)
+  if (bt->fmt == FMT_DIA)
     hipLaunchKernelGGL(k_bcg_init<FMT_DIA>, dim3(a.ngroups), dim3(256), 0, s, a, b);
+  else if (bt->fmt == FMT_BDIA)
+    hipLaunchKernelGGL(k_bcg_init<FMT_BDIA>, dim3(a.ngroups), dim3(256), 0, s, a, b);
   else
     hipLaunchKernelGGL(k_bcg_init<FMT_ELL>, dim3(a.ngroups), dim3(256), 0, s, a, b);
   hipLaunchKernelGGL(k_bcg_init_finish, dim3(bt->nprob), dim3(256), 0, s, a);
@@ -2200,8 +2423,10 @@ int batch_init(lfgpu_batch *bt, const double *b, double *x, double rtol, int max
 }
 
 void batch_chunk(lfgpu_batch *bt, const BArgs &a, int count) {
-  if (bt->A->cache->fmt == FMT_DIA)
+  if (bt->fmt == FMT_DIA)
     launch_bcg_chunk<FMT_DIA>(bt->ctx->stream, a, count);
+  else if (bt->fmt == FMT_BDIA)
+    launch_bcg_chunk<FMT_BDIA>(bt->ctx->stream, a, count);
   else
     launch_bcg_chunk<FMT_ELL>(bt->ctx->stream, a, count);
 }
@@ -2252,7 +2477,8 @@ BMArgs<VT> batch_mixed_args(lfgpu_batch *bt, const double *b, double *x, double 
                             int max_interval) {
   lfgpu_solver_cache *c = bt->A->cache;
   BMArgs<VT> a;
-  a.m = view_of(c, bt->A->nrows);
+  a.m = batch_matview(bt);
+  a.boff = bt->boff;
   a.dinv32 = c->dinv32;
   a.dinv = c->dinv;
   a.x = x;
@@ -2282,8 +2508,10 @@ BMArgs<VT> batch_mixed_args(lfgpu_batch *bt, const double *b, double *x, double 
 
 template <class VT>
 void batch_mixed_chunk(lfgpu_batch *bt, const BMArgs<VT> &a, int count) {
-  if (bt->A->cache->fmt == FMT_DIA)
+  if (bt->fmt == FMT_DIA)
     launch_bmix_chunk<FMT_DIA, VT>(bt->ctx->stream, a, count);
+  else if (bt->fmt == FMT_BDIA)
+    launch_bmix_chunk<FMT_BDIA, VT>(bt->ctx->stream, a, count);
   else
     launch_bmix_chunk<FMT_ELL, VT>(bt->ctx->stream, a, count);
 }
@@ -2306,8 +2534,10 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
                          s));
   const BMArgs<VT> a = batch_mixed_args<VT>(bt, b, x, delta2, max_interval);
   const dim3 G(a.ngroups), B(256);
-  if (bt->A->cache->fmt == FMT_DIA)
+  if (bt->fmt == FMT_DIA)
     hipLaunchKernelGGL((k_bmix_init<FMT_DIA, VT>), G, B, 0, s, a);
+  else if (bt->fmt == FMT_BDIA)
+    hipLaunchKernelGGL((k_bmix_init<FMT_BDIA, VT>), G, B, 0, s, a);
   else
     hipLaunchKernelGGL((k_bmix_init<FMT_ELL, VT>), G, B, 0, s, a);
   hipLaunchKernelGGL(k_bmix_init_finish<VT>, dim3(bt->nprob), B, 0, s, a);
@@ -2607,6 +2837,15 @@ extern "C" int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
   return LFGPU_OK;
 }
 
+extern "C" int lfgpu_batch_storage(const lfgpu_batch *bt, int *storage) {
+  LFG_REQUIRE(bt && storage, LFGPU_EINVAL, "NULL argument to lfgpu_batch_storage");
+  *storage = bt->fmt < 0 ? -1
+             : bt->fmt == FMT_BDIA ? LFGPU_STORAGE_BLOCK_DIA
+             : bt->fmt == FMT_DIA  ? LFGPU_STORAGE_DIA
+                                   : LFGPU_STORAGE_ELL;
+  return LFGPU_OK;
+}
+
 extern "C" int lfgpu_batch_destroy(lfgpu_batch *bt) {
   if (!bt) return LFGPU_OK;
   if (bt->ctx) (void)hipStreamSynchronize(bt->ctx->stream);
@@ -2616,6 +2855,10 @@ extern "C" int lfgpu_batch_destroy(lfgpu_batch *bt) {
   dfree(bt->part);
   dfree(bt->st);
   dfree(bt->mst);
+  dfree(bt->d_row_off);
+  dfree(bt->boff);
+  dfree(bt->bval);
+  dfree(bt->bval32);
   if (bt->h_st) (void)hipHostFree(bt->h_st);
   if (bt->h_mst) (void)hipHostFree(bt->h_mst);
   delete bt;
@@ -2635,6 +2878,11 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
   const bool mixed = opts->mode != LFGPU_PCG_FP64;
   LFG_TRY(ensure_cache(ctx, bt->A, mixed));
   hipStream_t s = ctx->stream;
+  if (bt->fmt < 0) LFG_TRY(batch_choose_format(bt));
+  const float *before32 = bt->bval32;
+  LFG_TRY(batch_fill(bt, mixed));
+  // a newly allocated FP32 array changes the captured kernel arguments
+  if (bt->bval32 != before32) batch_release_graph(bt);
   if (bt->rows_per_group == 0) LFG_TRY(batch_tune(bt, b->d));
   if (mixed && !bt->mst) {
     LFG_TRY(dalloc(&bt->mst, (size_t)bt->nprob, "batch mixed states"));

@@ -526,6 +526,73 @@ def test_batched_mixed_pcg_matches_direct_solves(ctx, mode):
         h.close()
 
 
+def _batch_system(ctx, sizes, extra=None, seed=23):
+    """Block-diagonal P1 system of tp meshes of the given sizes, optionally
+    followed by one unstructured mesh; returns handles and offsets."""
+    xy, cells, flags, voff, coff = capi.batched_tp_mesh(sizes)
+    if extra is not None:
+        nv0 = xy.shape[0]
+        xy = np.vstack([xy, extra.xy])
+        cells = np.vstack([cells, extra.cells + nv0]).astype(np.int32)
+        flags = np.concatenate([flags, extra.dirichlet_flags(1)])
+        voff = np.append(voff, xy.shape[0])
+        coff = np.append(coff, cells.shape[0])
+    alpha = np.exp(np.random.default_rng(seed).standard_normal(cells.shape[0]))
+    g = capi.Mesh(ctx, xy, cells)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, flags)
+    va = capi.Vec(ctx, alpha)
+    b = capi.Vec(ctx, g.ndofs)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 0.5),
+                  capi.coeff(capi.COEFF_CONST, 1.0), b)
+    return g, fe, plan, va, b, voff
+
+
+@pytest.mark.parametrize("mode", [capi.PCG_FP64, capi.PCG_MIXED, capi.PCG_MIXED_MATRIX])
+def test_batched_block_dia_storage_gives_the_ell_iterates(ctx, mode):
+    # structured problems of different sizes (odd and even n, whose middle
+    # rows lie on the boundary or inside) use per-problem DIA storage; one
+    # unstructured problem more forces sliced ELL.  Both sum every row in
+    # column order, so the structured problems' solutions and iteration
+    # counts agree bit for bit.
+    sizes = [9, 17, 5, 31, 12, 24, 3, 40, 8]
+    A = _batch_system(ctx, sizes)
+    B = _batch_system(ctx, sizes, extra=delaunay_mesh(60, 3))
+    nshared = int(A[5][-1])
+    xs, its = [], []
+    for g, fe, plan, va, b, voff in (A, B):
+        batch = capi.Batch(ctx, plan.csr, voff)
+        assert batch.storage() == -1
+        x = capi.Vec(ctx, g.ndofs)
+        for _ in range(2):  # the second call replays the captured graph
+            x.fill(0.0)
+            summ, infos = batch.pcg(b, x, rtol=1e-10, mode=mode)
+            assert summ.converged
+        xs.append(x.numpy()[:nshared])
+        its.append([i.iters for i in infos[:len(sizes)]])
+        xs_storage = batch.storage()
+        its[-1].append(xs_storage)
+        batch.close()
+        x.close()
+    assert its[0][-1] == capi.STORAGE_BLOCK_DIA and its[1][-1] == capi.STORAGE_ELL
+    assert its[0][:-1] == its[1][:-1]
+    assert np.array_equal(xs[0], xs[1])
+    # and the solutions solve the systems
+    g, fe, plan, va, b, voff = A
+    rp, col, val = plan.csr.download()
+    bh = b.numpy()
+    for k in range(len(sizes)):
+        lo, hi = int(voff[k]), int(voff[k + 1])
+        sub = slice(int(rp[lo]), int(rp[hi]))
+        rpk = np.ascontiguousarray(rp[lo:hi + 1] - rp[lo], np.int32)
+        colk = np.ascontiguousarray(col[sub] - lo, np.int32)
+        xd = L.band_cholesky_solve(rpk, colk, np.ascontiguousarray(val[sub]), bh[lo:hi])
+        assert np.linalg.norm(xs[0][lo:hi] - xd) <= 1e-8 * np.linalg.norm(xd), k
+    for sysh in (A, B):
+        for h in sysh[4::-1]:
+            h.close()
+
+
 # ---------------------------------------------- adjoint gradient end to end
 def test_adjoint_gradient_against_finite_differences(ctx):
     # J(theta) = 1/2 (u - u_obs)^T M (u - u_obs), A(exp(theta)) u = f;
