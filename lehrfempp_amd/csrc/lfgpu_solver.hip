@@ -196,12 +196,26 @@ __device__ __forceinline__ double apply64(const MatView &m, const double *__rest
 // (A x)_r from FP32 storage with FP64 arithmetic: a product of two FP32
 // numbers is exact in FP64 and the row is summed in FP64, so the only FP32
 // rounding of the result is the final conversion.
-template <int FMT>
+template <int FMT, bool HOIST = false>
 __device__ __forceinline__ float apply32(const MatView &m, const float *__restrict__ x, int32_t r,
                                          float xr) {
   const double xd = (double)xr;
   double acc = 0.0;
-  if constexpr (FMT == FMT_DIA) {
+  if constexpr (FMT == FMT_DIA && HOIST) {
+    float v[kMaxDiag], xc[kMaxDiag];
+#pragma unroll
+    for (int d = 0; d < kMaxDiag; ++d) {
+      if (d < m.ndiag) {
+        int32_t c = r + m.off[d];
+        c = c < 0 ? 0 : (c >= m.n ? m.n - 1 : c);
+        v[d] = m.val32[(int64_t)d * m.n + r];
+        xc[d] = x[c];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < kMaxDiag; ++d)
+      if (d < m.ndiag) acc += (double)v[d] * ((double)xc[d] - xd);
+  } else if constexpr (FMT == FMT_DIA) {
 #pragma unroll
     for (int d = 0; d < kMaxDiag; ++d) {
       if (d < m.ndiag) {
@@ -1177,15 +1191,30 @@ __global__ __launch_bounds__(256) void k_mix_setup(MixArgs<VT> a) {
 }
 
 // (A x)_r from FP32 storage for either vector precision
-template <int FMT>
+template <int FMT, bool HOIST = false>
 __device__ __forceinline__ float apply_mixed(const MatView &m, const float *x, int32_t r, float xr) {
-  return apply32<FMT>(m, x, r, xr);
+  return apply32<FMT, HOIST>(m, x, r, xr);
 }
-template <int FMT>
+template <int FMT, bool HOIST = false>
 __device__ __forceinline__ double apply_mixed(const MatView &m, const double *__restrict__ x,
                                               int32_t r, double xr) {
   double acc = 0.0;
-  if constexpr (FMT == FMT_DIA) {
+  if constexpr (FMT == FMT_DIA && HOIST) {
+    float v[kMaxDiag];
+    double xc[kMaxDiag];
+#pragma unroll
+    for (int d = 0; d < kMaxDiag; ++d) {
+      if (d < m.ndiag) {
+        int32_t c = r + m.off[d];
+        c = c < 0 ? 0 : (c >= m.n ? m.n - 1 : c);
+        v[d] = m.val32[(int64_t)d * m.n + r];
+        xc[d] = x[c];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < kMaxDiag; ++d)
+      if (d < m.ndiag) acc += (double)v[d] * (xc[d] - xr);
+  } else if constexpr (FMT == FMT_DIA) {
 #pragma unroll
     for (int d = 0; d < kMaxDiag; ++d) {
       if (d < m.ndiag) {
@@ -1258,12 +1287,12 @@ __global__ __launch_bounds__(SB) void k_mx_a(MixArgs<VT> a, int par) {
   if (!refine) {
     double pq = 0.0, qdr = 0.0, qdq = 0.0;
     for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
-      const VT pi = a.p[i];
-      const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
+      const VT pi = a.p[i], mi = (VT)a.dinv32[i], ri = a.r[i];
+      const VT qi = apply_mixed<FMT, true>(a.m, a.p, i, pi);
       a.q[i] = qi;
-      const double dq = (double)((VT)a.dinv32[i] * qi);
+      const double dq = (double)(mi * qi);
       pq += (double)pi * (double)qi;
-      qdr += dq * (double)a.r[i];
+      qdr += dq * (double)ri;
       qdq += dq * (double)qi;
     }
     double v[3] = {pq, qdr, qdq};
@@ -2020,7 +2049,7 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
     double pq = 0.0, qdr = 0.0, qdq = 0.0;
     for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
       const VT pi = a.p[i], mi = (VT)a.dinv32[i], ri = a.r[i];
-      const VT qi = apply_mixed<AF>(m, a.p, i, pi);
+      const VT qi = apply_mixed<AF, true>(m, a.p, i, pi);
       a.q[i] = qi;
       const double dq = (double)(mi * qi);
       pq += (double)pi * (double)qi;
