@@ -177,6 +177,19 @@ __device__ __forceinline__ double apply64(const MatView &m, const double *__rest
     const double *vp = m.val + o + l;
     const int32_t *cp = m.s_col + o + l;
     int32_t k = 0;
+    for (; k + 8 <= w; k += 8) {
+      int32_t c[8];
+      double vv[8], xx[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        c[u] = cp[64 * (k + u)];
+        vv[u] = vp[64 * (k + u)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xx[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += vv[u] * (xx[u] - xr);
+    }
     for (; k + 4 <= w; k += 4) {
       const int32_t c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)],
                     c3 = cp[64 * (k + 3)];
@@ -231,6 +244,19 @@ __device__ __forceinline__ float apply32(const MatView &m, const float *__restri
     const float *vp = m.val32 + o + l;
     const int32_t *cp = m.s_col + o + l;
     int32_t k = 0;
+    for (; k + 8 <= w; k += 8) {
+      int32_t c[8];
+      float vv[8], xx[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        c[u] = cp[64 * (k + u)];
+        vv[u] = vp[64 * (k + u)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xx[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += (double)vv[u] * ((double)xx[u] - xd);
+    }
     for (; k + 4 <= w; k += 4) {  // four independent gathers in flight
       const int32_t c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)],
                     c3 = cp[64 * (k + 3)];
@@ -1230,6 +1256,20 @@ __device__ __forceinline__ double apply_mixed(const MatView &m, const double *__
     const float *vp = m.val32 + o + l;
     const int32_t *cp = m.s_col + o + l;
     int32_t k = 0;
+    for (; k + 8 <= w; k += 8) {
+      int32_t c[8];
+      float vv[8];
+      double xx[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        c[u] = cp[64 * (k + u)];
+        vv[u] = vp[64 * (k + u)];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xx[u] = x[c[u]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += (double)vv[u] * (xx[u] - xr);
+    }
     for (; k + 4 <= w; k += 4) {  // four independent gathers in flight
       const int32_t c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)],
                     c3 = cp[64 * (k + 3)];
