@@ -446,11 +446,15 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
     batch = capi.Batch(ctx, plan.csr, voff)
     # reference: every problem to rtol 1e-13 (FP32 matrix, FP64 vectors and
     # true-residual stopping test)
-    x.fill(0.0)
-    batch.pcg(b, x, rtol=1e-13, maxit=MAXIT, mode=capi.PCG_MIXED_MATRIX)
-    x_ref = x.numpy()
+    x_ref = None
+    if not args.no_ref:
+        x.fill(0.0)
+        batch.pcg(b, x, rtol=1e-13, maxit=MAXIT, mode=capi.PCG_MIXED_MATRIX)
+        x_ref = x.numpy()
     modes = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
              "mixed_matrix": capi.PCG_MIXED_MATRIX}
+    wanted = args.batch_modes.split(",")
+    modes = {k: v for k, v in modes.items() if k in wanted}
     per_mode = {}
     for name, md in modes.items():
         x.fill(0.0)
@@ -465,8 +469,8 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
             ctx.sync()
             times.append((time.perf_counter() - t0, asm_ms, summ.ms))
         xs = x.numpy()
-        errs = [rel_err(xs[voff[k]:voff[k + 1]], x_ref[voff[k]:voff[k + 1]])
-                for k in range(nprob)]
+        errs = ([rel_err(xs[voff[k]:voff[k + 1]], x_ref[voff[k]:voff[k + 1]])
+                 for k in range(nprob)] if x_ref is not None else [float("nan")])
         it = [i.iters for i in infos]
         wall = float(np.median([t[0] for t in times]))
         per_mode[name] = {
@@ -479,17 +483,19 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
             "worst_err_vs_ref": float(max(errs))}
         log("batched %s: %.1f ms/step (pcg %.1f ms), iters %d..%d, worst err %.2e"
             % (name, 1e3 * wall, per_mode[name]["pcg_ms"], min(it), max(it), max(errs)))
-    plain = None if args.no_plain else plain_fp64_per_problem(args, capi, ctx, sizes, g, coff,
-                                                               voff, x_ref)
-    sel = per_mode[args.batch_mode]
+    plain = (None if args.no_plain or x_ref is None else
+             plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff, x_ref))
+    sel = per_mode[args.batch_mode if args.batch_mode in per_mode else next(iter(per_mode))]
     out = {"metric": "batched P1 assemble + PCG, Melem/s", "value": sel["melem_s"],
            "unit": "Melem/s", "pcg_mode": args.batch_mode,
            "config": {"workload": "%d P1 problems, sides log-uniform in [32,512], "
                                   "a=exp(N(0,1)), f=1" % nprob,
                       "cells": int(ne), "dofs": int(nv), "nnz": int(plan.csr.info()[2])},
            "ms_per_step": sel["ms_per_step"], "modes": per_mode,
-           "speedup_vs_fp64": {k: per_mode["fp64"]["ms_per_step"] / v["ms_per_step"]
-                               for k, v in per_mode.items()},
+           "storage": {capi.STORAGE_ELL: "sliced ELL", capi.STORAGE_DIA: "DIA",
+                       capi.STORAGE_BLOCK_DIA: "per-problem DIA"}.get(batch.storage()),
+           "speedup_vs_fp64": ({k: per_mode["fp64"]["ms_per_step"] / v["ms_per_step"]
+                                for k, v in per_mode.items()} if "fp64" in per_mode else None),
            "tuned_rows_per_workgroup": summ.refinements}
     if plain is not None:
         out["plain_fp64_per_problem"] = plain
@@ -567,25 +573,40 @@ def run_fkpp(args, capi, ctx):
     n, m_steps, T, lam = args.fkpp_n, args.fkpp_steps, 1.0, 2.1
     mesh, edges, A, M, u0 = _fkpp_setup(capi, ctx, n)
     nv = u0.size
-    mode = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
-            "mixed_matrix": capi.PCG_MIXED_MATRIX}[args.fkpp_mode]
-    ss = fisher_kpp.StrangSplit(ctx, A, M, T, m_steps, lam, rtol=1e-10, mode=mode)
+    codes = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
+             "mixed_matrix": capi.PCG_MIXED_MATRIX}
     vK, vu = capi.Vec(ctx, np.ones(nv)), capi.Vec(ctx, u0)
-    out = ss.Evolution(vK, vu)  # warm-up: builds the solver storage of A, S1, S2
-    out.close()
-    runs = []
-    for _ in range(max(1, args.steps)):
-        ctx.sync()
-        t0 = time.perf_counter()
-        out = ss.Evolution(vK, vu)
-        ctx.sync()
-        wall = time.perf_counter() - t0
-        info = ss.last_info
-        runs.append((wall, info.ms, info.iters, info.max_iters, info.solves, info.converged))
-        u = out.numpy()
+    per_mode, results = {}, {}
+    wanted = args.fkpp_modes.split(",")
+    for name in [args.fkpp_mode] + [k for k in codes if k != args.fkpp_mode and k in wanted]:
+        ss = fisher_kpp.StrangSplit(ctx, A, M, T, m_steps, lam, rtol=1e-10, mode=codes[name])
+        out = ss.Evolution(vK, vu)  # warm-up: builds the solver storage of A, S1, S2
         out.close()
-    wall = float(np.median([r[0] for r in runs]))
-    _, dev_ms, iters, max_it, solves, conv = runs[-1]
+        runs = []
+        for _ in range(max(1, args.steps)):
+            ctx.sync()
+            t0 = time.perf_counter()
+            out = ss.Evolution(vK, vu)
+            ctx.sync()
+            wall = time.perf_counter() - t0
+            info = ss.last_info
+            runs.append((wall, info.ms, info.iters, info.max_iters, info.solves, info.converged))
+            results[name] = out.numpy()
+            out.close()
+        ss.close()
+        w = float(np.median([r[0] for r in runs]))
+        per_mode[name] = {"ms_per_step": 1e3 * w / (runs[-1][4] // 2),
+                          "iters_per_solve": runs[-1][2] / max(runs[-1][4], 1),
+                          "converged": bool(runs[-1][5])}
+        if name == args.fkpp_mode:
+            wall = w
+            _, dev_ms, iters, max_it, solves, conv = runs[-1]
+        log("fkpp %s: %.2f ms/step" % (name, per_mode[name]["ms_per_step"]))
+    ref = results[args.fkpp_mode]
+    for name in per_mode:
+        per_mode[name]["max_rel_diff_vs_" + args.fkpp_mode] = float(
+            np.abs(results[name] - ref).max() / np.abs(ref).max())
+    u = ref
     nd = solves // 2
     res = {"metric": "Fisher/KPP Strang splitting, ms per time step", "unit": "ms",
            "value": 1e3 * wall / nd, "higher_is_better": False,
@@ -598,10 +619,9 @@ def run_fkpp(args, capi, ctx):
            "evolution_s": wall, "device_ms": dev_ms, "diffusion_steps": nd,
            "solves": solves, "iters_per_solve": iters / max(solves, 1),
            "max_iters": max_it, "converged": bool(conv),
-           "mass_final": float(u.sum() / nv)}
+           "mass_final": float(u.sum() / nv), "modes": per_mode}
     log("fkpp %d^2: %.2f ms/step, %.1f iterations/solve (max %d)"
         % (n, res["value"], res["iters_per_solve"], max_it))
-    ss.close()
     for hh in (vK, vu, A, M, edges, mesh):
         hh.close()
     if not args.no_cpu_baseline:
@@ -684,18 +704,24 @@ def main():
                     default="p1")
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
     ap.add_argument("--batch-mode", choices=["fp64", "mixed", "mixed_matrix"], default="mixed")
+    ap.add_argument("--batch-modes", default="fp64,mixed,mixed_matrix",
+                    help="batched: comma-separated precision modes to run")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="p3: assembly only")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched: skip the per-problem plain FP64 comparison")
+    ap.add_argument("--no-ref", action="store_true",
+                    help="batched: skip the 1e-13 reference solve (one graph capture per mode)")
     ap.add_argument("--fd-dirs", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--fkpp-n", type=int, default=1024)
     ap.add_argument("--fkpp-steps", type=int, default=100)
     ap.add_argument("--fkpp-sample-n", type=int, default=256)
     ap.add_argument("--fkpp-mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
+    ap.add_argument("--fkpp-modes", default="fp64,mixed,mixed_matrix",
+                    help="fkpp: comma-separated precision modes to run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path; "

@@ -795,6 +795,30 @@ __global__ __launch_bounds__(SB) void k_cg_update(CgArgs a, int par) {
 constexpr int kRB = 256;    // rows per workgroup of the row sweeps
 constexpr int kRedB = 1024; // threads of the reduction kernels
 
+// This thread's share of the K per-workgroup partial sums (part[k nwg + g],
+// g = t, t + kRedB, ...), added in that order; four groups' loads are issued
+// before their additions.
+template <int K>
+__device__ __forceinline__ void sum_partials(const double *part, int nwg, double (&t)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) t[k] = 0.0;
+  int g = threadIdx.x;
+  for (; g + 3 * kRedB < nwg; g += 4 * kRedB) {
+    double v[K][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k][u] = part[(int64_t)k * nwg + g + u * kRedB];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < K; ++k) t[k] += v[k][u];
+  }
+  for (; g < nwg; g += kRedB)
+#pragma unroll
+    for (int k = 0; k < K; ++k) t[k] += part[(int64_t)k * nwg + g];
+}
+
 struct RowArgs {
   MatView m;
   const double *__restrict__ dinv;
@@ -817,11 +841,7 @@ __device__ __forceinline__ void row_partials(const RowArgs &a, double (&v)[K], d
 
 template <int K>
 __device__ __forceinline__ void row_totals(const RowArgs &a, double (&t)[K], double *sh) {
-#pragma unroll
-  for (int k = 0; k < K; ++k) t[k] = 0.0;
-  for (int g = threadIdx.x; g < a.nwg; g += kRedB)
-#pragma unroll
-    for (int k = 0; k < K; ++k) t[k] += a.part[(int64_t)k * a.nwg + g];
+  sum_partials<K>(a.part, a.nwg, t);
   block_sum_k<kRedB, K>(t, sh);
 }
 
@@ -1451,11 +1471,7 @@ __device__ __forceinline__ void mxr_partials(const MixArgs<VT> &a, double (&v)[K
 
 template <class VT, int K>
 __device__ __forceinline__ void mxr_totals(const MixArgs<VT> &a, double (&t)[K], double *sh) {
-#pragma unroll
-  for (int k = 0; k < K; ++k) t[k] = 0.0;
-  for (int g = threadIdx.x; g < a.nwg; g += kRedB)
-#pragma unroll
-    for (int k = 0; k < K; ++k) t[k] += a.rpart[(int64_t)k * a.nwg + g];
+  sum_partials<K>(a.rpart, a.nwg, t);
   block_sum_k<kRedB, K>(t, sh);
 }
 
