@@ -81,6 +81,11 @@ struct lfgpu_solver_cache {
   float *rowsum32 = nullptr;
   uint64_t built_version = ~0ull;
   bool have32 = false;
+  // DIA with a mirror-symmetric offset list whose lower-diagonal values
+  // equal the mirrored upper-diagonal values bit for bit: the FP64 SpMV
+  // sweep of the PCG then reads the upper diagonals only (measured 7 %
+  // faster at 2048^2; the FP32 sweeps measured slower that way)
+  bool sym64 = false;
   // work vectors
   double *r = nullptr, *p = nullptr, *q = nullptr;
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *xa32 = nullptr;
@@ -143,7 +148,11 @@ MatView view_of(const lfgpu_solver_cache *c, int32_t n) {
 // row before the first use (one memory round trip instead of one per
 // diagonal; the sum keeps the diagonal order).  It costs registers, so it is
 // used where it measured faster: the FP64 SpMV and the FP64 PCG SpMV sweep.
-template <int FMT, bool HOIST = false>
+// SYM (DIA, hoisted): the lower diagonal d (offset < 0) is read as the
+// mirrored upper diagonal ndiag-1-d of row c = r + off[d], which holds the
+// same value when the matrix is symmetric bit for bit; half the diagonals'
+// bytes, the same sum.
+template <int FMT, bool HOIST = false, bool SYM = false>
 __device__ __forceinline__ double apply64(const MatView &m, const double *__restrict__ x,
                                           int32_t r, double xr) {
   double acc = 0.0;
@@ -153,9 +162,15 @@ __device__ __forceinline__ double apply64(const MatView &m, const double *__rest
     for (int d = 0; d < kMaxDiag; ++d) {
       if (d < m.ndiag) {
         int32_t c = r + m.off[d];
-        c = c < 0 ? 0 : (c >= m.n ? m.n - 1 : c);
-        v[d] = m.val[(int64_t)d * m.n + r];
-        xc[d] = x[c];
+        if (SYM && 2 * d < m.ndiag) {
+          const bool in = c >= 0;
+          v[d] = in ? m.val[(int64_t)(m.ndiag - 1 - d) * m.n + c] : 0.0;
+          xc[d] = x[in ? c : 0];
+        } else {
+          c = c < 0 ? 0 : (c >= m.n ? m.n - 1 : c);
+          v[d] = m.val[(int64_t)d * m.n + r];
+          xc[d] = x[c];
+        }
       }
     }
 #pragma unroll
@@ -380,6 +395,50 @@ __global__ void k_fill_vals(const int32_t *row_ptr, const int32_t *col, const do
   if (dinv32) dinv32[r] = (float)d;
 }
 
+// flag = 1 if a lower-diagonal value differs, bit for bit, from the
+// mirrored upper-diagonal value (the offsets are ascending and
+// mirror-symmetric: off[d] = -off[nd-1-d])
+__global__ void k_dia_sym(const double *val, int32_t n, int nd, DiagSet ds, int *flag) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  bool bad = false;
+  for (int d = 0; 2 * d < nd; ++d) {
+    const int32_t c = r + ds.off[d];
+    const double a = val[(int64_t)d * n + r];
+    const double b = c >= 0 ? val[(int64_t)(nd - 1 - d) * n + c] : 0.0;
+    bad = bad || __double_as_longlong(a) != __double_as_longlong(b);
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+// Sets the cache's sym64 flag for its current values.
+int dia_symmetry(lfgpu_ctx *ctx, lfgpu_solver_cache *c, int32_t n) {
+  c->sym64 = false;
+  if (c->fmt != FMT_DIA || c->ndiag % 2 != 0) return LFGPU_OK;
+  for (int d = 0; d < c->ndiag; ++d)
+    if (c->off[d] != -c->off[c->ndiag - 1 - d] || (2 * d < c->ndiag && c->off[d] >= 0))
+      return LFGPU_OK;
+  DiagSet ds{};
+  ds.ndiag = c->ndiag;
+  for (int d = 0; d < kMaxDiag; ++d) ds.off[d] = c->off[d];
+  hipStream_t s = ctx->stream;
+  int *flag = nullptr;
+  LFG_TRY(dalloc(&flag, 1, "symmetry flag"));
+  hipError_t e = hipMemsetAsync(flag, 0, 4, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_dia_sym, dim3(grid_for(n, 256)), dim3(256), 0, s, c->val, n, c->ndiag,
+                       ds, flag);
+    e = hipGetLastError();
+  }
+  int h = 1;
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(flag);
+  if (e != hipSuccess) return hip_fail(e, "DIA symmetry check");
+  c->sym64 = h == 0;
+  return LFGPU_OK;
+}
+
 // DIA when the off-diagonal offsets of a middle row cover every row and
 // storing whole diagonals wastes at most 25 %.
 int detect_dia(lfgpu_ctx *ctx, const lfgpu_csr *A, DiagSet *ds) {
@@ -488,6 +547,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32) {
                        c->have32 ? c->rowsum32 : nullptr, c->dinv,
                        c->have32 ? c->dinv32 : nullptr);
     LFG_HIP(hipGetLastError());
+    LFG_TRY(dia_symmetry(ctx, c, n));
     c->built_version = A->val_version;
   }
   return LFGPU_OK;
@@ -704,7 +764,7 @@ __global__ void k_zero_if_bb0(double *x, int32_t n, const PcgState *st) {
 // FP64 iteration k, sweep 1 (par = k & 1): Eigen's convergence test on r_k
 // (||r_k||^2 left by the previous sweep 2), then q = A p with the partials
 // p.q, q.D^-1 r and q.D^-1 q.
-template <int FMT, int SB>
+template <int FMT, int SB, bool SYM = false>
 __global__ __launch_bounds__(SB) void k_cg_spmv(CgArgs a, int par) {
   __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
@@ -734,7 +794,7 @@ __global__ __launch_bounds__(SB) void k_cg_spmv(CgArgs a, int par) {
     // loads before the store: the compiler keeps stores and later loads in
     // program order
     const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
-    const double qi = apply64<FMT, true>(a.m, a.p, i, pi);
+    const double qi = apply64<FMT, true, SYM>(a.m, a.p, i, pi);
     a.q[i] = qi;
     const double dq = di * qi;
     pq += pi * qi;
@@ -1683,11 +1743,14 @@ void launch_mxr_chunk(hipStream_t s, const MixArgs<VT> &m, int count) {
 }
 
 template <int FMT, int SB>
-void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count) {
+void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count, bool sym) {
   const dim3 G(sw.ngroups), B(SB);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    hipLaunchKernelGGL((k_cg_spmv<FMT, SB>), G, B, 0, s, a, par);
+    if (FMT == FMT_DIA && sym)
+      hipLaunchKernelGGL((k_cg_spmv<FMT, SB, true>), G, B, 0, s, a, par);
+    else
+      hipLaunchKernelGGL((k_cg_spmv<FMT, SB>), G, B, 0, s, a, par);
     hipLaunchKernelGGL(k_cg_update<SB>, G, B, 0, s, a, par);
   }
 }
@@ -2883,9 +2946,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   } else {
     for (;;) {
       if (c->fmt == FMT_DIA)
-        launch_cg_chunk<FMT_DIA, kSB64>(s, sw, a, check);
+        launch_cg_chunk<FMT_DIA, kSB64>(s, sw, a, check, c->sym64);
       else
-        launch_cg_chunk<FMT_ELL, kSBMix>(s, sw, a, check);
+        launch_cg_chunk<FMT_ELL, kSBMix>(s, sw, a, check, false);
       LFG_HIP(hipGetLastError());
       LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
       LFG_HIP(hipStreamSynchronize(s));
