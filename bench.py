@@ -110,6 +110,24 @@ def pmc_traffic(kernel, name="pmc_traffic_p1_2048.md"):
     return int(m.group(1)) if m else None
 
 
+def roofline_entry(sb, spmv_ms, achieved):
+    """Roofline of the FP64 SpMV: achieved = the CSR algorithmic bytes
+    (12 nnz + 4 (N+1) + 16 N) / measured time.  The DIA mirror stores no
+    indices, so the kernel can beat that figure's rate; hbm_gbs is the
+    counted HBM traffic per launch over the same time."""
+    traffic = pmc_traffic("k_spmv_f64")
+    out = {"bound": "hbm", "kernel": "k_spmv_f64 (FP64 SpMV, DIA storage)",
+           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "algorithmic_bytes": sb, "avg_us": 1e3 * spmv_ms}
+    if traffic:
+        out["hbm_gbs"] = traffic / (spmv_ms * 1e-3) / 1e9
+        out["hbm_frac"] = out["hbm_gbs"] / HBM_PEAK_GBS
+        out["note"] = ("achieved uses the CSR algorithmic bytes; the DIA storage reads "
+                       "fewer (traffic = counted HBM bytes per launch, hbm_gbs its rate)")
+    return out
+
+
 def true_relres(capi, ctx, A, x, b):
     y = capi.Vec(ctx, b.n)
     capi.spmv(ctx, A, x, y)
@@ -328,10 +346,7 @@ def run_p1(args, capi, ctx):
         "grad_cost_per_solve": None if adj is None else adj["grad_cost_per_solve"],
         "adjoint": adj,
         "plan_build_s": P.plan_s,
-        "roofline": {"bound": "hbm", "kernel": "k_spmv_f64 (FP64 SpMV, DIA storage)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic("k_spmv_f64"),
-                     "algorithmic_bytes": sb, "avg_us": 1e3 * spmv_ms},
+        "roofline": roofline_entry(sb, spmv_ms, achieved),
         "pcg_iteration": {
             "algorithmic_bytes": sb + 80 * P.nv,
             "us": 1e3 * float(np.median(pcg_ms)) / max(1, iters[-1]),
