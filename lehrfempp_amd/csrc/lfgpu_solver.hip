@@ -120,6 +120,7 @@ namespace {
 // ---------------------------------------------------------------------------
 struct MatView {
   int32_t n;
+  int32_t lo;  // first row of the block (a batch problem's rows; else 0)
   int ndiag;
   int off[kMaxDiag];
   const int32_t *__restrict__ slice_off;
@@ -133,6 +134,7 @@ struct MatView {
 MatView view_of(const lfgpu_solver_cache *c, int32_t n) {
   MatView m;
   m.n = n;
+  m.lo = 0;
   m.ndiag = c->ndiag;
   for (int d = 0; d < kMaxDiag; ++d) m.off[d] = c->off[d];
   m.slice_off = c->slice_off;
@@ -149,9 +151,9 @@ MatView view_of(const lfgpu_solver_cache *c, int32_t n) {
 // diagonal; the sum keeps the diagonal order).  It costs registers, so it is
 // used where it measured faster: the FP64 SpMV and the FP64 PCG SpMV sweep.
 // SYM (DIA, hoisted): the lower diagonal d (offset < 0) is read as the
-// mirrored upper diagonal ndiag-1-d of row c = r + off[d], which holds the
-// same value when the matrix is symmetric bit for bit; half the diagonals'
-// bytes, the same sum.
+// mirrored upper diagonal ndiag-1-d of row c = r + off[d] (zero when c lies
+// before the block's first row), which holds the same value when the matrix
+// is symmetric bit for bit; half the diagonals' bytes, the same sum.
 template <int FMT, bool HOIST = false, bool SYM = false>
 __device__ __forceinline__ double apply64(const MatView &m, const double *__restrict__ x,
                                           int32_t r, double xr) {
@@ -163,9 +165,9 @@ __device__ __forceinline__ double apply64(const MatView &m, const double *__rest
       if (d < m.ndiag) {
         int32_t c = r + m.off[d];
         if (SYM && 2 * d < m.ndiag) {
-          const bool in = c >= 0;
+          const bool in = c >= m.lo;
           v[d] = in ? m.val[(int64_t)(m.ndiag - 1 - d) * m.n + c] : 0.0;
-          xc[d] = x[in ? c : 0];
+          xc[d] = x[in ? c : m.lo];
         } else {
           c = c < 0 ? 0 : (c >= m.n ? m.n - 1 : c);
           v[d] = m.val[(int64_t)d * m.n + r];
@@ -1843,16 +1845,19 @@ struct BArgs {
   const BProb *probs;
   BState *st;
   const int32_t *boff;  // FMT_BDIA: [nprob x kMaxDiag] offsets
+  const int32_t *prow;  // FMT_BDIA: [nprob + 1] first rows
 };
 
 // The matrix view of a workgroup of problem prob: FMT_BDIA takes that
 // problem's diagonal offsets and is then applied as DIA.
 template <int FMT>
-__device__ __forceinline__ MatView batch_view(const MatView &m0, const int32_t *boff, int prob) {
+__device__ __forceinline__ MatView batch_view(const MatView &m0, const int32_t *boff,
+                                              const int32_t *prow, int prob) {
   MatView m = m0;
   if constexpr (FMT == FMT_BDIA) {
 #pragma unroll
     for (int d = 0; d < kMaxDiag; ++d) m.off[d] = boff[prob * kMaxDiag + d];
+    m.lo = prow[prob];
   }
   return m;
 }
@@ -1900,7 +1905,7 @@ template <int FMT>
 __global__ __launch_bounds__(256) void k_bcg_init(BArgs a, const double *__restrict__ b) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
-  const MatView m = batch_view<FMT>(a.m, a.boff, G.prob);
+  const MatView m = batch_view<FMT>(a.m, a.boff, a.prow, G.prob);
   double bb = 0.0, rr = 0.0, rz = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
     const double xi = a.x[i];
@@ -1952,7 +1957,7 @@ __global__ void k_bcg_zero_if_bb0(BArgs a) {
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) a.x[i] = 0.0;
 }
 
-template <int FMT>
+template <int FMT, bool SYM = false>
 __global__ __launch_bounds__(256) void k_bcg_spmv(BArgs a, int par) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
@@ -1975,11 +1980,11 @@ __global__ __launch_bounds__(256) void k_bcg_spmv(BArgs a, int par) {
     st->iter += 1;
     if (st->iter >= st->maxit) st->done = 1;
   }
-  const MatView m = batch_view<FMT>(a.m, a.boff, G.prob);
+  const MatView m = batch_view<FMT>(a.m, a.boff, a.prow, G.prob);
   double pq = 0.0, qdr = 0.0, qdq = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
     const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
-    const double qi = apply64<applied_fmt<FMT>(), true>(m, a.p, i, pi);
+    const double qi = apply64<applied_fmt<FMT>(), true, SYM>(m, a.p, i, pi);
     a.q[i] = qi;
     const double dq = di * qi;
     pq += pi * qi;
@@ -2022,10 +2027,13 @@ __global__ __launch_bounds__(256) void k_bcg_update(BArgs a, int par) {
 }
 
 template <int FMT>
-void launch_bcg_chunk(hipStream_t s, const BArgs &a, int count) {
+void launch_bcg_chunk(hipStream_t s, const BArgs &a, int count, bool sym) {
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    hipLaunchKernelGGL(k_bcg_spmv<FMT>, dim3(a.ngroups), dim3(256), 0, s, a, par);
+    if (FMT == FMT_BDIA && sym)
+      hipLaunchKernelGGL((k_bcg_spmv<FMT, true>), dim3(a.ngroups), dim3(256), 0, s, a, par);
+    else
+      hipLaunchKernelGGL(k_bcg_spmv<FMT>, dim3(a.ngroups), dim3(256), 0, s, a, par);
     hipLaunchKernelGGL(k_bcg_update, dim3(a.ngroups), dim3(256), 0, s, a, par);
   }
 }
@@ -2068,6 +2076,7 @@ struct BMArgs {
   double delta2;
   int max_interval;
   const int32_t *boff;  // FMT_BDIA: [nprob x kMaxDiag] offsets
+  const int32_t *prow;  // FMT_BDIA: [nprob + 1] first rows
 };
 
 template <class VT>
@@ -2080,7 +2089,7 @@ template <int FMT, class VT>
 __global__ __launch_bounds__(256) void k_bmix_init(BMArgs<VT> a) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
-  const MatView m = batch_view<FMT>(a.m, a.boff, G.prob);
+  const MatView m = batch_view<FMT>(a.m, a.boff, a.prow, G.prob);
   double bb = 0.0, rr = 0.0, rz = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
     const double bi = a.b[i];
@@ -2162,7 +2171,7 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
     st->iterp[par] = refine ? itp : itp + 1;
     if (!refine) st->iter = itp + 1;
   }
-  const MatView m = batch_view<FMT>(a.m, a.boff, G.prob);
+  const MatView m = batch_view<FMT>(a.m, a.boff, a.prow, G.prob);
   constexpr int AF = applied_fmt<FMT>();
   if (!refine) {
     double pq = 0.0, qdr = 0.0, qdq = 0.0;
@@ -2315,6 +2324,9 @@ struct lfgpu_batch {
   float *bval32 = nullptr;       // [bndiag x n]
   uint64_t bbuilt = ~0ull;       // matrix value version of bval / bval32
   bool bbuilt32 = false;
+  // every problem has the same mirror-symmetric offsets and its FP64 values
+  // are symmetric bit for bit: the FP64 SpMV sweep reads upper diagonals only
+  bool bsymoff = false, bsym = false;
 };
 
 namespace {
@@ -2410,6 +2422,25 @@ __global__ void k_bdia_fill(const int32_t *row_ptr, const int32_t *col, const do
   }
 }
 
+// flag = 1 if a lower-diagonal value of row r differs, bit for bit, from the
+// mirrored upper-diagonal value of row r + off (offsets of r's problem,
+// ascending and mirror-symmetric)
+__global__ void k_bdia_sym(const double *bval, int32_t n, const int32_t *row_off, int nprob,
+                           const int32_t *boff, int nd, int *flag) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int p = batch_prob_of(row_off, nprob, r);
+  const int32_t *off = boff + p * kMaxDiag;
+  bool bad = false;
+  for (int d = 0; 2 * d < nd; ++d) {
+    const int32_t c = r + off[d];
+    const double a = bval[(int64_t)d * n + r];
+    const double b = c >= row_off[p] ? bval[(int64_t)(nd - 1 - d) * n + c] : 0.0;
+    bad = bad || __double_as_longlong(a) != __double_as_longlong(b);
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
 // Chooses the batch's storage on its first solve: the matrix mirror when it
 // is DIA already, per-problem DIA when every problem is a banded matrix
 // whose offsets near its middle row cover all of its rows (and whole
@@ -2470,6 +2501,14 @@ int batch_choose_format(lfgpu_batch *bt) {
   LFG_TRY(dalloc(&bt->bval, (size_t)nd * n, "batch DIA values"));
   bt->bndiag = nd;
   bt->fmt = FMT_BDIA;
+  bool symoff = nd % 2 == 0;
+  for (int p = 0; p < np && symoff; ++p) {
+    const int32_t *o = &off[(size_t)p * kMaxDiag];
+    if (h[(size_t)p * (kMaxDiag + 2)] != nd) symoff = false;
+    for (int d = 0; d < nd && symoff; ++d)
+      symoff = o[d] == -o[nd - 1 - d] && (2 * d >= nd || o[d] < 0);
+  }
+  bt->bsymoff = symoff;
   return LFGPU_OK;
 }
 
@@ -2484,10 +2523,28 @@ int batch_fill(lfgpu_batch *bt, bool need32) {
   }
   const bool with32 = bt->bval32 != nullptr;
   if (bt->bbuilt == A->val_version && bt->bbuilt32 == with32) return LFGPU_OK;
-  hipLaunchKernelGGL(k_bdia_fill, dim3(grid_for(n, 256)), dim3(256), 0, bt->ctx->stream,
-                     A->row_ptr, A->col, A->val, n, bt->d_row_off, bt->nprob, bt->boff,
-                     bt->bndiag, bt->bval, bt->bval32);
+  hipStream_t s = bt->ctx->stream;
+  hipLaunchKernelGGL(k_bdia_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col,
+                     A->val, n, bt->d_row_off, bt->nprob, bt->boff, bt->bndiag, bt->bval,
+                     bt->bval32);
   LFG_HIP(hipGetLastError());
+  bt->bsym = false;
+  if (bt->bsymoff) {
+    int *flag = nullptr;
+    LFG_TRY(dalloc(&flag, 1, "batch symmetry flag"));
+    hipError_t e = hipMemsetAsync(flag, 0, 4, s);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_bdia_sym, dim3(grid_for(n, 256)), dim3(256), 0, s, bt->bval, n,
+                         bt->d_row_off, bt->nprob, bt->boff, bt->bndiag, flag);
+      e = hipGetLastError();
+    }
+    int h = 1;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(flag);
+    if (e != hipSuccess) return hip_fail(e, "batch symmetry check");
+    bt->bsym = h == 0;
+  }
   bt->bbuilt = A->val_version;
   bt->bbuilt32 = with32;
   return LFGPU_OK;
@@ -2545,7 +2602,7 @@ int batch_layout(lfgpu_batch *bt, int R) {
 BArgs batch_args(lfgpu_batch *bt, double *x) {
   lfgpu_solver_cache *c = bt->A->cache;
   return BArgs{batch_matview(bt), c->dinv, x, c->r, c->p, c->q, bt->part, bt->ngroups,
-               bt->groups, bt->probs, bt->st, bt->boff};
+               bt->groups, bt->probs, bt->st, bt->boff, bt->d_row_off};
 }
 
 int batch_init(lfgpu_batch *bt, const double *b, double *x, double rtol, int maxit) {
@@ -2572,11 +2629,11 @@ int batch_init(lfgpu_batch *bt, const double *b, double *x, double rtol, int max
 
 void batch_chunk(lfgpu_batch *bt, const BArgs &a, int count) {
   if (bt->fmt == FMT_DIA)
-    launch_bcg_chunk<FMT_DIA>(bt->ctx->stream, a, count);
+    launch_bcg_chunk<FMT_DIA>(bt->ctx->stream, a, count, false);
   else if (bt->fmt == FMT_BDIA)
-    launch_bcg_chunk<FMT_BDIA>(bt->ctx->stream, a, count);
+    launch_bcg_chunk<FMT_BDIA>(bt->ctx->stream, a, count, bt->bsym);
   else
-    launch_bcg_chunk<FMT_ELL>(bt->ctx->stream, a, count);
+    launch_bcg_chunk<FMT_ELL>(bt->ctx->stream, a, count, false);
 }
 
 // Autotuner: times a few iterations for each candidate workgroup tile (on a
@@ -2627,6 +2684,7 @@ BMArgs<VT> batch_mixed_args(lfgpu_batch *bt, const double *b, double *x, double 
   BMArgs<VT> a;
   a.m = batch_matview(bt);
   a.boff = bt->boff;
+  a.prow = bt->d_row_off;
   a.dinv32 = c->dinv32;
   a.dinv = c->dinv;
   a.x = x;
