@@ -603,9 +603,9 @@ def run_fkpp(args, capi, ctx):
             t0 = time.perf_counter()
             out = ss.Evolution(vK, vu)
             ctx.sync()
-            wall = time.perf_counter() - t0
+            t_run = time.perf_counter() - t0
             info = ss.last_info
-            runs.append((wall, info.ms, info.iters, info.max_iters, info.solves, info.converged))
+            runs.append((t_run, info.ms, info.iters, info.max_iters, info.solves, info.converged))
             results[name] = out.numpy()
             out.close()
         ss.close()
