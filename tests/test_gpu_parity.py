@@ -526,6 +526,50 @@ def test_batched_mixed_pcg_matches_direct_solves(ctx, mode):
         h.close()
 
 
+def _bitwise_symmetric(rp, col, val):
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp)).astype(np.int64)
+    key = rows * n + col
+    order = np.argsort(key)
+    pos = np.searchsorted(key[order], col.astype(np.int64) * n + rows)
+    return bool(np.array_equal(val[order][pos].view(np.int64), val.view(np.int64)))
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.2])
+def test_pcg_fp64_on_symmetric_and_asymmetric_dia_values(ctx, jitter):
+    # a structured P1 matrix is symmetric bit for bit, so the FP64 sweep reads
+    # only its upper diagonals; a jittered mesh gives the same DIA pattern
+    # with values symmetric only up to rounding, which takes the full read.
+    # Both must match the direct solve and Eigen's iteration count.
+    n = 48
+    m = jittered_mesh(n, amount=jitter, seed=5) if jitter else L.tp_triag_mesh(n)
+    a = np.exp(np.random.default_rng(6).standard_normal(m.ne))
+    g = gpu_mesh(ctx, m, 1)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, m.dirichlet_flags(1))
+    va = capi.Vec(ctx, a)
+    b = capi.Vec(ctx, g.ndofs)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 1.0),
+                  capi.coeff(capi.COEFF_SINSIN, 1.0), b)
+    rp, col, val = plan.csr.download()
+    assert _bitwise_symmetric(rp, col, val) == (jitter == 0.0)
+    x = capi.Vec(ctx, g.ndofs)
+    info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10)
+    assert info.converged
+    bh = b.numpy()
+    xd = L.band_cholesky_solve(rp, col, val, bh)
+    assert np.linalg.norm(x.numpy() - xd) <= 1e-8 * np.linalg.norm(xd)
+    _, ito, _ = L.pcg(rp, col, val, bh, tol=1e-10)
+    assert abs(info.iters - ito) <= 2, (info.iters, ito)
+    # y = A x through the same storage equals the CSR product
+    y = capi.Vec(ctx, g.ndofs)
+    capi.spmv(ctx, plan.csr, x, y)
+    yo = L.csr_spmv(rp, col, val, x.numpy())
+    assert np.abs(y.numpy() - yo).max() <= 1e-13 * np.abs(yo).max()
+    for h in (y, x, b, va, plan, fe, g):
+        h.close()
+
+
 def _batch_system(ctx, sizes, extra=None, seed=23):
     """Block-diagonal P1 system of tp meshes of the given sizes, optionally
     followed by one unstructured mesh; returns handles and offsets."""
