@@ -405,10 +405,13 @@ int lfgpu_strang_diffusion(lfgpu_ctx *ctx, lfgpu_strang *s, int firstcall,
                            const lfgpu_vec *mu, lfgpu_vec *out,
                            lfgpu_strang_info *info);
 /* Evolution (strangsplitting.cc:53-94): half a diffusion step, then
- * (m - 2) times { reaction over tau, diffusion over tau }, then a reaction
- * over tau / 2, where the reaction is the exact logistic flow
+ * max(m - 2, 0) times { reaction over tau, diffusion over tau } (the
+ * reference's loop bounds), then a reaction over tau / 2, where the
+ * reaction is the exact logistic flow
  *   u <- cap / (1 + (cap / u - 1) exp(-lambda t))
- * with carrying capacity cap [n].  out may alias mu. */
+ * with carrying capacity cap [n].  out may alias mu.  A stage solve that
+ * misses rtol within opts->maxit returns LFGPU_ENOCONV at once (info holds
+ * the counts so far, out is not written). */
 int lfgpu_strang_evolution(lfgpu_ctx *ctx, lfgpu_strang *s,
                            const lfgpu_vec *cap, const lfgpu_vec *mu,
                            lfgpu_vec *out, lfgpu_strang_info *info);
