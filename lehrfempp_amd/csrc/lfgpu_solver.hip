@@ -832,11 +832,15 @@ __global__ __launch_bounds__(SB) void k_cg_update(CgArgs a, int par) {
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
   for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
-    // all loads of the row before its stores
-    const double pi = a.p[i], xi = a.x[i], ri0 = a.r[i], qi = a.q[i], di = a.dinv[i];
+    // all loads of the row before its stores; x and q are not read again
+    // before the next write, so they bypass the caches and leave room for
+    // p, r and D^-1, which the next sweep reads
+    const double pi = a.p[i], ri0 = a.r[i], di = a.dinv[i];
+    const double xi = __builtin_nontemporal_load(&a.x[i]);
+    const double qi = __builtin_nontemporal_load(&a.q[i]);
     const double ri = ri0 - alpha * qi;
     const double zi = di * ri;
-    a.x[i] = xi + alpha * pi;
+    __builtin_nontemporal_store(xi + alpha * pi, &a.x[i]);
     a.r[i] = ri;
     a.p[i] = zi + beta * pi;
     rr += ri * ri;
@@ -999,10 +1003,12 @@ __global__ __launch_bounds__(kRB) void k_cgr_update(RowArgs a) {
   if (i < a.m.n) {
     // all loads of the row before its stores (stores and later loads stay
     // in program order)
-    const double pi = a.p[i], xi = a.x[i], r0 = a.r[i], qi = a.q[i], di = a.dinv[i];
+    const double pi = a.p[i], r0 = a.r[i], di = a.dinv[i];
+    const double xi = __builtin_nontemporal_load(&a.x[i]);
+    const double qi = __builtin_nontemporal_load(&a.q[i]);
     const double ri = r0 - alpha * qi;
     const double zi = di * ri;
-    a.x[i] = xi + alpha * pi;
+    __builtin_nontemporal_store(xi + alpha * pi, &a.x[i]);
     a.r[i] = ri;
     a.p[i] = zi + beta * pi;
     v[0] = ri * ri;
@@ -1458,6 +1464,8 @@ __global__ __launch_bounds__(SB) void k_mx_b(MixArgs<VT> a, int par) {
     const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
     const VT fa = (VT)alpha, fb = (VT)(rho_next / rho);
     for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
+      // plain loads here: the non-temporal form measured 2% slower for
+      // this chunked sweep (faster in the batched and row sweeps)
       const VT pi = a.p[i], xai = a.xa[i], r0 = a.r[i], qi = a.q[i];
       const VT mi = (VT)a.dinv32[i];
       const VT ri = r0 - fa * qi;
@@ -1675,11 +1683,13 @@ __global__ __launch_bounds__(kRB) void k_mxr_b(MixArgs<VT> a) {
   if (!st->refine[0]) {
     const VT fa = (VT)st->alpha_s, fb = (VT)st->beta_s;
     if (i < a.m.n) {
-      const VT pi = a.p[i], xai = a.xa[i], r0 = a.r[i], qi = a.q[i];
+      const VT pi = a.p[i], r0 = a.r[i];
+      const VT xai = __builtin_nontemporal_load(&a.xa[i]);
+      const VT qi = __builtin_nontemporal_load(&a.q[i]);
       const VT mi = (VT)a.dinv32[i];
       const VT ri = r0 - fa * qi;
       const VT zi = mi * ri;
-      a.xa[i] = xai + fa * pi;
+      __builtin_nontemporal_store(xai + fa * pi, &a.xa[i]);
       a.r[i] = ri;
       a.p[i] = zi + fb * pi;
       v[0] = (double)ri * (double)ri;
@@ -2015,10 +2025,12 @@ __global__ __launch_bounds__(256) void k_bcg_update(BArgs a, int par) {
   const double beta = rho_next / rho;
   double rr = 0.0, rz = 0.0;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-    const double pi = a.p[i], xi = a.x[i], r0 = a.r[i], qi = a.q[i], di = a.dinv[i];
+    const double pi = a.p[i], r0 = a.r[i], di = a.dinv[i];
+    const double xi = __builtin_nontemporal_load(&a.x[i]);
+    const double qi = __builtin_nontemporal_load(&a.q[i]);
     const double ri = r0 - alpha * qi;
     const double zi = di * ri;
-    a.x[i] = xi + alpha * pi;
+    __builtin_nontemporal_store(xi + alpha * pi, &a.x[i]);
     a.r[i] = ri;
     a.p[i] = zi + beta * pi;
     rr += ri * ri;
@@ -2225,11 +2237,13 @@ __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
     const double rho_next = rho - 2.0 * alpha * qdr + alpha * alpha * qdq;
     const VT fa = (VT)alpha, fb = (VT)(rho_next / rho);
     for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-      const VT pi = a.p[i], xai = a.xa[i], r0 = a.r[i], qi = a.q[i];
+      const VT pi = a.p[i], r0 = a.r[i];
+      const VT xai = __builtin_nontemporal_load(&a.xa[i]);
+      const VT qi = __builtin_nontemporal_load(&a.q[i]);
       const VT mi = (VT)a.dinv32[i];
       const VT ri = r0 - fa * qi;
       const VT zi = mi * ri;
-      a.xa[i] = xai + fa * pi;
+      __builtin_nontemporal_store(xai + fa * pi, &a.xa[i]);
       a.r[i] = ri;
       a.p[i] = zi + fb * pi;
       rr += (double)ri * (double)ri;
