@@ -20,6 +20,9 @@ Other configurations (each prints its own JSON line):
                     by an adjoint solve, checked against central differences
   --config fkpp     Fisher/KPP Strang splitting (SDIRK-2 diffusion + exact
                     logistic reaction) on a 1024 x 1024 mesh, 100 time steps
+  --config strip    one GPU's share of the default workload split into 8
+                    strips (2048 x 256 cells): assembly, SpMV and PCG
+                    iteration times next to 1/8 of the full-size ones
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config p1]
 """
@@ -709,13 +712,89 @@ def run_traffic(args, capi, ctx):
     return out
 
 
+def run_strip(args, capi, ctx, strips=8, iters=2000):
+    """One GPU's share of the default workload when it is split into `strips`
+    horizontal strips: an interior 2048 x 256-cell strip of the unit square
+    (y in [3/8, 4/8]) with the same coefficient law, Dirichlet rows only
+    where the strip meets the domain boundary (x = 0, x = 1).  Times the
+    assembly, the FP64 SpMV and a fixed number of PCG iterations (FP64 with
+    two sweeps and with a single reduction, FP32-Krylov, FP32-matrix), each
+    next to 1/strips of the full-size figure
+    measured in the same process.  No halo exchange and no all-reduce are
+    included: this is the local compute of one shard."""
+    n, ny = args.n, args.n // strips
+    k = strips // 2 - 1
+    xy, cells, _ = capi.tp_triag_mesh(n, ny, (0.0, k / strips, 1.0, (k + 1) / strips))
+    i = np.arange(xy.shape[0]) % (n + 1)
+    flags = ((i == 0) | (i == n)).astype(np.uint8)
+    g = np.random.default_rng(args.seed).standard_normal(cells.shape[0])
+    mesh, fe = capi.Mesh(ctx, xy, cells), capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, mesh, flags)
+    theta = capi.Vec(ctx, g)
+    alpha = capi.coeff(capi.COEFF_EXPCELL, 1.0, theta)
+    gamma = capi.coeff(capi.COEFF_CONST, 1.0)
+    f = capi.coeff(capi.COEFF_SINSIN, 1.0)
+    nv, nnz = xy.shape[0], plan.csr.info()[2]
+
+    def measure(pl, b, x, nrows, nnz_, alpha_):
+        pl.assemble(fe, alpha_, gamma, f, b)
+        asm = float(np.median([pl.assemble(fe, alpha_, gamma, f, b) for _ in range(10)]))
+        xs, ys = capi.Vec(ctx, np.random.default_rng(1).standard_normal(nrows)), capi.Vec(ctx, nrows)
+        spmv_us = 1e3 * capi.spmv_timed(ctx, pl.csr, xs, ys, args.spmv_reps)
+        xs.close()
+        ys.close()
+        it = {}
+        for name, md, single in (("two_sweep", capi.PCG_FP64, False),
+                                 ("single_reduction", capi.PCG_FP64, True),
+                                 ("mixed", capi.PCG_MIXED, False),
+                                 ("mixed_matrix", capi.PCG_MIXED_MATRIX, False)):
+            for _ in range(2):  # the first run also builds the solver's storage
+                x.fill(0.0)
+                inf = capi.pcg(ctx, pl.csr, b, x, rtol=1e-30, maxit=iters, check_every=128,
+                               mode=md, raise_on_noconv=False, single_reduction=single)
+            it[name] = 1e3 * inf.ms / max(1, inf.iters)
+        return {"assembly_ms": asm, "spmv_us": spmv_us,
+                "spmv_gbs_algorithmic": spmv_bytes(nnz_, nrows) / (spmv_us * 1e-6) / 1e9,
+                "pcg_iteration_us": it}
+
+    b, x = capi.Vec(ctx, nv), capi.Vec(ctx, nv)
+    strip = measure(plan, b, x, nv, nnz, alpha)
+    for h in (x, b, plan):
+        h.close()
+    log("strip %dx%d: assembly %.3f ms, spmv %.2f us, iteration (us) %s"
+        % (n, ny, strip["assembly_ms"], strip["spmv_us"],
+           ", ".join("%s %.2f" % kv for kv in strip["pcg_iteration_us"].items())))
+    P = Problem(capi, ctx, n, args.seed)
+    full = measure(P.plan, P.b, P.x, P.nv, P.nnz, P.alpha)
+    P.close()
+    log("full %dx%d: assembly %.3f ms, spmv %.2f us, iteration (us) %s"
+        % (n, n, full["assembly_ms"], full["spmv_us"],
+           ", ".join("%s %.2f" % kv for kv in full["pcg_iteration_us"].items())))
+    for h in (theta, fe, mesh):
+        h.close()
+    share = {"assembly_ms": full["assembly_ms"] / strips, "spmv_us": full["spmv_us"] / strips,
+             "pcg_iteration_us": {kk: v / strips for kk, v in full["pcg_iteration_us"].items()}}
+    return {"metric": "one shard's local compute (assembly, SpMV, PCG iteration)",
+            "config": {"workload": "interior %dx%d-cell strip of the %dx%d P1 reaction-diffusion "
+                                   "problem (1/%d of it), a=exp(N(0,1)) per cell, "
+                                   "f=sin(pi x)sin(pi y)" % (n, ny, n, n, strips),
+                       "cells": int(cells.shape[0]), "dofs": int(nv), "nnz": int(nnz)},
+            "data": "synthetic", "strip": strip, "full": full, "full_over_strips": share,
+            "strip_vs_share": {"assembly": strip["assembly_ms"] / share["assembly_ms"],
+                               "spmv": strip["spmv_us"] / share["spmv_us"],
+                               "pcg_iteration": {kk: strip["pcg_iteration_us"][kk]
+                                                 / share["pcg_iteration_us"][kk]
+                                                 for kk in share["pcg_iteration_us"]}}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint", "traffic", "fkpp"],
+    ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint", "traffic", "fkpp",
+                                         "strip"],
                     default="p1")
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
     ap.add_argument("--batch-mode", choices=["fp64", "mixed", "mixed_matrix"], default="mixed")
@@ -755,6 +834,8 @@ def main():
         out = run_traffic(args, capi, ctx)
     elif args.config == "fkpp":
         out = run_fkpp(args, capi, ctx)
+    elif args.config == "strip":
+        out = run_strip(args, capi, ctx)
     else:
         out = adjoint_run(capi, ctx, n=1024, fd_dirs=args.fd_dirs)
         out = {"metric": "adjoint gradient cost / forward solve", "config": {
