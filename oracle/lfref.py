@@ -308,6 +308,41 @@ def pcg(rp, col, val, b, tol=1e-10, maxit=100000, x0=None):
     return x, it, err.value
 
 
+_OMP = None
+
+
+def _omp_lib():
+    """oracle/liblfref_omp.so: the all-cores PCG, loaded only when timed."""
+    global _OMP
+    if _OMP is None:
+        path = os.path.join(_HERE, "liblfref_omp.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        L.lfref_pcg_omp.restype = C.c_int
+        L.lfref_pcg_omp.argtypes = [C.c_int32, _i32, _i32, _d, _d, _d, C.c_double, C.c_int,
+                                    C.POINTER(C.c_double)]
+        L.lfref_omp_threads.restype = C.c_int
+        L.lfref_omp_threads.argtypes = []
+        _OMP = L
+    return _OMP
+
+
+def omp_threads():
+    return int(_omp_lib().lfref_omp_threads())
+
+
+def pcg_omp(rp, col, val, b, tol=1e-10, maxit=100000, x0=None):
+    """pcg on all cores (OpenMP loops and reductions; different summation
+    order, so the last bits of the iterates differ from pcg)."""
+    n = len(rp) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    err = C.c_double(0.0)
+    it = _omp_lib().lfref_pcg_omp(n, rp, col, val, np.ascontiguousarray(b, np.float64), x,
+                                  tol, maxit, C.byref(err))
+    return x, it, err.value
+
+
 def cpu_path_assemble(mesh: TriMesh, degree, cell_dofs, ndofs, alpha_cell, gamma_const,
                       f_kind, f_const, fixed):
     """Reference-faithful single-thread path: Eval -> triplets -> fix -> CSR."""

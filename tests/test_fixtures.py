@@ -90,6 +90,19 @@ def test_c1_solution_against_an_independent_solver_and_the_pde(c1):
     assert abs(x[32 * 65 + 32] - 0.07367135328) < 2e-4
 
 
+def test_all_cores_pcg_baseline_solves_c1(c1):
+    """The OpenMP PCG timed as the all-cores CPU baseline solves the same
+    system: same iteration count as the serial oracle loop within one or
+    two (reductions in another order), solution within kappa * tol of the
+    direct one."""
+    rp, col, val, b = c1["row_ptr"], c1["col"], c1["val"], c1["rhs"]
+    x1, it1, e1 = L.pcg(rp, col, val, b, tol=1e-12)
+    x2, it2, e2 = L.pcg_omp(rp, col, val, b, tol=1e-12)
+    assert L.omp_threads() >= 1
+    assert abs(it1 - it2) <= 2 and e2 < 1e-12
+    assert np.linalg.norm(x2 - c1["x"]) <= 1e-8 * np.linalg.norm(c1["x"])
+
+
 # ----------------------------------------------------------- C2/C3 cells
 def test_cell_fixture_inputs_are_the_benchmark_meshes(cells):
     n = 2048
