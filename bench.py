@@ -20,6 +20,8 @@ Other configurations (each prints its own JSON line):
                     by an adjoint solve, checked against central differences
   --config fkpp     Fisher/KPP Strang splitting (SDIRK-2 diffusion + exact
                     logistic reaction) on a 1024 x 1024 mesh, 100 time steps
+  --config c1       the 64 x 64 P1 Poisson problem: absolute assembly, SpMV
+                    and PCG times (launch-latency bound)
   --config strip    one GPU's share of the default workload split into 8
                     strips (2048 x 256 cells): assembly, SpMV and PCG
                     iteration times next to 1/8 of the full-size ones
@@ -307,6 +309,9 @@ def run_p1(args, capi, ctx):
     x_ref = P.x.numpy()
     for name in modes:
         per_mode[name]["err_vs_ref"] = rel_err(per_mode[name].pop("x"), x_ref)
+
+    # assembly rate: median over the timed steps and ten more launches
+    asm_ms = asm_ms + [P.assemble() for _ in range(10)]
 
     # the two-phase assembly (matrix-core element matrices, then the gather)
     P.plan.assemble(P.fe, P.alpha, P.gamma, P.f, P.b, mfma=True)
@@ -702,6 +707,39 @@ def fkpp_cpu_sample(capi, ctx, n, m_steps):
             "max_rel_diff_gpu_vs_cpu": float(np.abs(ug - uc).max() / np.abs(uc).max())}
 
 
+def run_c1(args, capi, ctx, n=64, reps=20):
+    """The 64 x 64 P1 Poisson problem (alpha = 1, gamma = 0, f = 1, zero
+    Dirichlet data) of homDir_linfe_demo.cc:114-175.  At 4,225 DOFs every
+    kernel is launch-latency bound, so only absolute times are reported:
+    assembly, one FP64 SpMV and the Jacobi-PCG solve to 1e-10 in each mode
+    (medians over `reps`)."""
+    P = Problem(capi, ctx, n, args.seed, gamma=0.0, f_sinsin=False,
+                log_alpha=np.zeros(2 * n * n))
+    asm = [P.assemble() for _ in range(reps)]
+    xs, ys = capi.Vec(ctx, np.random.default_rng(1).standard_normal(P.nv)), capi.Vec(ctx, P.nv)
+    spmv_us = 1e3 * capi.spmv_timed(ctx, P.plan.csr, xs, ys, 200)
+    xs.close()
+    ys.close()
+    modes = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED, "mixed_matrix": capi.PCG_MIXED_MATRIX}
+    solves = {}
+    for name, md in modes.items():
+        runs = [P.solve(md) for _ in range(5)]
+        solves[name] = {"ms": float(np.median([r.ms for r in runs])), "iters": runs[-1].iters,
+                        "relres": runs[-1].relres,
+                        "true_relres": true_relres(capi, ctx, P.plan.csr, P.x, P.b)}
+    log("c1 %dx%d: assembly %.1f us, spmv %.2f us, pcg %s" % (
+        n, n, 1e3 * float(np.median(asm)), spmv_us,
+        ", ".join("%s %.2f ms (%d it)" % (k, v["ms"], v["iters"]) for k, v in solves.items())))
+    out = {"metric": "C1 absolute times (launch-latency bound)",
+           "config": {"workload": "P1 Poisson on a %dx%d structured triangle mesh, a=1, f=1, "
+                                  "zero Dirichlet" % (n, n),
+                      "cells": P.ne, "dofs": P.nv, "nnz": P.nnz},
+           "data": "synthetic", "assembly_us": 1e3 * float(np.median(asm)), "spmv_us": spmv_us,
+           "pcg": solves}
+    P.close()
+    return out
+
+
 def run_traffic(args, capi, ctx):
     """A short fixed sequence for HBM byte counters (rocprofv3 --pmc): the
     FP64 SpMV of the 2048x2048 system, a few PCG iterations and a plain copy
@@ -810,7 +848,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint", "traffic", "fkpp",
-                                         "strip"],
+                                         "strip", "c1"],
                     default="p1")
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
     ap.add_argument("--batch-mode", choices=["fp64", "mixed", "mixed_matrix"], default="mixed")
@@ -852,6 +890,8 @@ def main():
         out = run_fkpp(args, capi, ctx)
     elif args.config == "strip":
         out = run_strip(args, capi, ctx)
+    elif args.config == "c1":
+        out = run_c1(args, capi, ctx)
     else:
         out = adjoint_run(capi, ctx, n=1024, fd_dirs=args.fd_dirs)
         out = {"metric": "adjoint gradient cost / forward solve", "config": {

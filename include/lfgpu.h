@@ -296,6 +296,12 @@ typedef struct lfgpu_pcg_opts {
    * preconditioned residual on the fly; 0 = two sweeps with Eigen's own
    * recurrences (default). */
   int single_reduction;
+  /* FP64 two-sweep and row-sweep iterations: 0 = replay each chunk of
+   * check_every iterations from a captured hipGraph below 2^20 rows, where
+   * the solve is bound by the launch rate (default); 1 = always; -1 = never
+   * (plain launches).  The iterates are the same either way.  Up to two
+   * captured chunks are kept per matrix, keyed by the solution vector. */
+  int graph;
 } lfgpu_pcg_opts;
 typedef struct lfgpu_pcg_info {
   int iters;       /* Eigen's iteration count                     */

@@ -35,7 +35,8 @@ class Coeff(C.Structure):
 class PcgOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("maxit", C.c_int), ("mode", C.c_int),
                 ("check_every", C.c_int), ("refine_delta", C.c_double),
-                ("refine_max_interval", C.c_int), ("single_reduction", C.c_int)]
+                ("refine_max_interval", C.c_int), ("single_reduction", C.c_int),
+                ("graph", C.c_int)]
 
 
 class PcgInfo(C.Structure):
@@ -425,10 +426,14 @@ def spmv_timed(ctx, A: Csr, x: Vec, y: Vec, reps: int):
 
 
 def pcg(ctx, A: Csr, b: Vec, x: Vec, rtol=1e-10, maxit=None, mode=PCG_FP64, check_every=64,
-        raise_on_noconv=True, refine_delta=0.0, refine_max_interval=0, single_reduction=False):
+        raise_on_noconv=True, refine_delta=0.0, refine_max_interval=0, single_reduction=False,
+        graph=0):
+    """lfgpu_pcg.  graph: 0 = replay captured FP64 iteration chunks below 2^20
+    rows, 1 = always, -1 = never."""
     n = A.info()[0]
     opts = PcgOpts(rtol, int(2 * n if maxit is None else maxit), mode, check_every,
-                   float(refine_delta), int(refine_max_interval), int(bool(single_reduction)))
+                   float(refine_delta), int(refine_max_interval), int(bool(single_reduction)),
+                   int(graph))
     info = PcgInfo()
     rc = lib().lfgpu_pcg(ctx.h, A.h, b.h, x.h, C.byref(opts), C.byref(info))
     if rc != OK and (rc != ENOCONV or raise_on_noconv):

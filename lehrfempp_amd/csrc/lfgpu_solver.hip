@@ -96,12 +96,19 @@ struct lfgpu_solver_cache {
   double *cg1 = nullptr;   // single-reduction PCG: r, w, s by parity [6 n]
   double *rpart = nullptr; // row-sweep PCG partials [3 x ceil(n / 256)]
   PcgState *st = nullptr;
+  // captured chunks of FP64 iterations, two slots (opaque: the slot type
+  // holds the kernels' argument blocks, defined with them below)
+  void *cg_graphs = nullptr;
+  int cg_graph_next = 0;
 };
 
 namespace lfgpu {
+void cg_graphs_destroy(void *slots);
+
 void solver_cache_free(lfgpu_csr *A) {
   lfgpu_solver_cache *c = A->cache;
   if (!c) return;
+  if (c->cg_graphs) cg_graphs_destroy(c->cg_graphs);
   for (void *p : {(void *)c->slice_off, (void *)c->s_col, (void *)c->val, (void *)c->val32,
                   (void *)c->dinv, (void *)c->dinv32, (void *)c->rowsum, (void *)c->rowsum32,
                   (void *)c->r, (void *)c->p, (void *)c->q, (void *)c->r32, (void *)c->p32,
@@ -1770,6 +1777,104 @@ void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Captured chunks of FP64 iterations.  Below ~1M rows a sweep takes a few
+// microseconds and the solve is bound by the host's launch rate (64x64 P1:
+// ~15 us per iteration for two sweeps of ~3 us); a chunk of `count`
+// iterations is therefore captured into a graph once and replayed.  A slot is
+// reused when the captured argument blocks equal the next call's field by
+// field (the solution vector is the only argument that changes between
+// calls on one matrix); two slots serve callers that alternate between two
+// solution vectors (the time stepper's stages).
+// ---------------------------------------------------------------------------
+enum { CG_CHUNK_DIA = 0, CG_CHUNK_ELL = 1, CG_ROWS_ELL = 2 };
+
+struct CgGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int kind = -1, count = 0;
+  bool sym = false;
+  CgArgs ca{};
+  RowArgs ra{};
+};
+
+bool same_view(const MatView &x, const MatView &y) {
+  if (x.n != y.n || x.lo != y.lo || x.ndiag != y.ndiag || x.slice_off != y.slice_off ||
+      x.s_col != y.s_col || x.val != y.val || x.val32 != y.val32 || x.rowsum != y.rowsum ||
+      x.rowsum32 != y.rowsum32)
+    return false;
+  for (int d = 0; d < kMaxDiag; ++d)
+    if (x.off[d] != y.off[d]) return false;
+  return true;
+}
+
+bool same_args(const CgArgs &x, const CgArgs &y) {
+  return same_view(x.m, y.m) && x.dinv == y.dinv && x.x == y.x && x.r == y.r && x.p == y.p &&
+         x.q == y.q && x.part == y.part && x.st == y.st && x.sw.n == y.sw.n &&
+         x.sw.rows_per_group == y.sw.rows_per_group && x.sw.ngroups == y.sw.ngroups;
+}
+
+bool same_args(const RowArgs &x, const RowArgs &y) {
+  return same_view(x.m, y.m) && x.dinv == y.dinv && x.x == y.x && x.r == y.r && x.p == y.p &&
+         x.q == y.q && x.part == y.part && x.nwg == y.nwg && x.st == y.st;
+}
+
+void launch_fp64_chunk(hipStream_t s, int kind, const CgArgs &ca, const RowArgs &ra, int count,
+                       bool sym) {
+  if (kind == CG_CHUNK_DIA)
+    launch_cg_chunk<FMT_DIA, kSB64>(s, ca.sw, ca, count, sym);
+  else if (kind == CG_CHUNK_ELL)
+    launch_cg_chunk<FMT_ELL, kSBMix>(s, ca.sw, ca, count, false);
+  else
+    launch_cgr_chunk<FMT_ELL>(s, ra, count);
+}
+
+// One chunk of FP64 iterations, eagerly or from a (re)captured graph.
+int run_fp64_chunk(lfgpu_solver_cache *c, hipStream_t s, int kind, const CgArgs &ca,
+                   const RowArgs &ra, int count, bool sym, bool use_graph) {
+  if (!use_graph) {
+    launch_fp64_chunk(s, kind, ca, ra, count, sym);
+    LFG_HIP(hipGetLastError());
+    return LFGPU_OK;
+  }
+  if (!c->cg_graphs) c->cg_graphs = new CgGraph[2];
+  CgGraph *slots = static_cast<CgGraph *>(c->cg_graphs);
+  CgGraph *g = nullptr;
+  for (int k = 0; k < 2 && !g; ++k) {
+    CgGraph &t = slots[k];
+    if (t.exec && t.kind == kind && t.count == count && t.sym == sym &&
+        (kind == CG_ROWS_ELL ? same_args(t.ra, ra) : same_args(t.ca, ca)))
+      g = &t;
+  }
+  if (!g) {
+    g = &slots[c->cg_graph_next];
+    c->cg_graph_next ^= 1;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    *g = CgGraph{};
+    LFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launch_fp64_chunk(s, kind, ca, ra, count, sym);
+    hipError_t e = hipStreamEndCapture(s, &g->graph);
+    if (e != hipSuccess) {
+      g->graph = nullptr;
+      return hip_fail(e, "hipStreamEndCapture (PCG chunk)");
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g->graph);
+      *g = CgGraph{};
+      return hip_fail(e, "hipGraphInstantiate (PCG chunk)");
+    }
+    g->kind = kind;
+    g->count = count;
+    g->sym = sym;
+    g->ca = ca;
+    g->ra = ra;
+  }
+  LFG_HIP(hipGraphLaunch(g->exec, s));
+  return LFGPU_OK;
+}
+
 template <int FMT, class VT>
 void launch_mix_chunk(hipStream_t s, const Sweep &sw, const MixArgs<VT> &m, int count) {
   const dim3 G(sw.ngroups), B(kSBMix);
@@ -2906,6 +3011,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   // with the iteration count
   int check = opts->check_every > 0 ? opts->check_every : 64;
   check += check & 1;
+  // FP64 two-sweep and row-sweep iterations replay a captured graph of each
+  // chunk below 2^20 rows (launch-bound sizes), or as the caller asks
+  const bool use_graph = opts->graph > 0 || (opts->graph == 0 && n < (1 << 20));
 
   if (single && !c->cg1)
     LFG_TRY(dalloc(&c->cg1, (size_t)6 * n, "single-reduction PCG vectors"));
@@ -2980,8 +3088,7 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
     hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, c->st);
     LFG_HIP(hipGetLastError());
     for (;;) {
-      launch_cgr_chunk<FMT_ELL>(s, ra, check);
-      LFG_HIP(hipGetLastError());
+      LFG_TRY(run_fp64_chunk(c, s, CG_ROWS_ELL, CgArgs{}, ra, check, false, use_graph));
       LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
       LFG_HIP(hipStreamSynchronize(s));
       if (hs->done) break;
@@ -3026,10 +3133,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   } else {
     for (;;) {
       if (c->fmt == FMT_DIA)
-        launch_cg_chunk<FMT_DIA, kSB64>(s, sw, a, check, c->sym64);
+        LFG_TRY(run_fp64_chunk(c, s, CG_CHUNK_DIA, a, RowArgs{}, check, c->sym64, use_graph));
       else
-        launch_cg_chunk<FMT_ELL, kSBMix>(s, sw, a, check, false);
-      LFG_HIP(hipGetLastError());
+        LFG_TRY(run_fp64_chunk(c, s, CG_CHUNK_ELL, a, RowArgs{}, check, false, use_graph));
       LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
       LFG_HIP(hipStreamSynchronize(s));
       if (hs->done) break;
@@ -3193,3 +3299,14 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
   }
   return LFGPU_OK;
 }
+
+namespace lfgpu {
+void cg_graphs_destroy(void *p) {
+  CgGraph *slots = static_cast<CgGraph *>(p);
+  for (int k = 0; k < 2; ++k) {
+    if (slots[k].exec) (void)hipGraphExecDestroy(slots[k].exec);
+    if (slots[k].graph) (void)hipGraphDestroy(slots[k].graph);
+  }
+  delete[] slots;
+}
+}  // namespace lfgpu

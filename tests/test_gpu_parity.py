@@ -444,6 +444,43 @@ def test_pcg_single_reduction_matches_eigen_iterates(ctx, degree):
         h.close()
 
 
+@pytest.mark.parametrize("degree", [1, 3])
+def test_pcg_graph_replay_matches_plain_launches(ctx, degree):
+    # FP64 iterations replayed from captured chunks (graph=1) give the
+    # iterates of plain launches (graph=-1) bit for bit on every replay, with
+    # other device work and synchronisations between replays and three
+    # solution vectors rotating through the two capture slots; DIA sweeps at
+    # P1, sliced-ELL row sweeps at P3
+    if degree == 1:
+        m, g, fe, plan, va, b = _c1_system(
+            ctx, 48, np.exp(np.random.default_rng(5).standard_normal(2 * 48 * 48)), 1.0)
+        nd = m.nv
+    else:
+        m, nd, g, fe, plan, va, b = _p3_system(ctx, 9)
+    x0 = capi.Vec(ctx, nd)
+    x0.fill(0.0)
+    ref = capi.pcg(ctx, plan.csr, b, x0, rtol=1e-10, check_every=8, graph=-1)
+    x_ref = x0.numpy()
+    assert ref.converged
+    xs = [capi.Vec(ctx, nd) for _ in range(3)]
+    y = capi.Vec(ctx, nd)
+    for rep in range(3):
+        # slot hits (0, 1 alternating) and evictions (2)
+        for x in (xs[0], xs[1], xs[0], xs[1], xs[2], xs[0]):
+            x.fill(0.0)
+            info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, check_every=8, graph=1)
+            assert info.converged and info.iters == ref.iters, (rep, info.iters, ref.iters)
+            assert np.array_equal(x.numpy(), x_ref), rep
+            capi.spmv(ctx, plan.csr, x, y)  # eager work between replays
+            ctx.sync()
+        # a plain solve in between leaves the captured chunks valid
+        x0.fill(0.0)
+        capi.pcg(ctx, plan.csr, b, x0, rtol=1e-10, check_every=8, graph=-1)
+        assert np.array_equal(x0.numpy(), x_ref)
+    for h in xs + [y, x0, va, b, plan, fe, g]:
+        h.close()
+
+
 # ------------------------------------------------------------ batched solves
 def test_batched_pcg_matches_per_problem_solves(ctx):
     sizes = [9, 17, 5, 31, 12, 24, 40, 8]
