@@ -807,7 +807,9 @@ __global__ __launch_bounds__(SB) void k_cg_spmv(CgArgs a, int par) {
     // program order
     const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
     const double qi = apply64<FMT, true, SYM>(a.m, a.p, i, pi);
-    a.q[i] = qi;
+    // q goes past the caches (read once, by the update sweep): 2% faster at
+    // 2048^2; the same store measured slower in the mixed and row sweeps
+    __builtin_nontemporal_store(qi, &a.q[i]);
     const double dq = di * qi;
     pq += pi * qi;
     qdr += dq * ri;
@@ -2103,7 +2105,7 @@ __global__ __launch_bounds__(256) void k_bcg_spmv(BArgs a, int par) {
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
     const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
     const double qi = apply64<applied_fmt<FMT>(), true, SYM, true>(m, a.p, i, pi);
-    a.q[i] = qi;
+    __builtin_nontemporal_store(qi, &a.q[i]);  // as in k_cg_spmv
     const double dq = di * qi;
     pq += pi * qi;
     qdr += dq * ri;
