@@ -103,6 +103,14 @@ def test_all_cores_pcg_baseline_solves_c1(c1):
     assert np.linalg.norm(x2 - c1["x"]) <= 1e-8 * np.linalg.norm(c1["x"])
 
 
+def test_autograd_binding_imports_without_a_gpu():
+    """The checking binding is a torch.autograd.Function over the C ABI."""
+    import torch
+    from lehrfempp_amd import autograd
+    assert issubclass(autograd._AdjointJ, torch.autograd.Function)
+    assert callable(autograd.AdjointObjective)
+
+
 # ----------------------------------------------------------- C2/C3 cells
 def test_cell_fixture_inputs_are_the_benchmark_meshes(cells):
     n = 2048
