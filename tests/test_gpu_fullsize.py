@@ -131,3 +131,70 @@ def test_c3_full_size_row_sums(ctx):
         assert abs(b.numpy().sum() - 4 / np.pi ** 2) <= 4.0 / n
     for hh in (b, theta, plan, fe, mesh):
         hh.close()
+
+
+def test_c4_full_size_batch(ctx):
+    """The 256 problems of the batched workload (sides log-uniform in
+    [32, 512]): every problem converges in every mode, and each problem's
+    solutions agree across modes to 1e-8 (rtol 1e-10 with kappa ~ 1e5)."""
+    rng = np.random.default_rng(4)
+    sizes = np.rint(np.exp(rng.uniform(np.log(32), np.log(512), 256))).astype(int)
+    xy, cells, flags, voff, coff = capi.batched_tp_mesh(sizes)
+    assert int(voff[-1]) == int(((sizes + 1) ** 2).sum()) == 14_647_286
+    g = np.concatenate([np.random.default_rng(5 + k).standard_normal(int(coff[k + 1] - coff[k]))
+                        for k in range(256)])
+    mesh, fe = capi.Mesh(ctx, xy, cells), capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, mesh, flags)
+    theta = capi.Vec(ctx, g)
+    b, x = capi.Vec(ctx, xy.shape[0]), capi.Vec(ctx, xy.shape[0])
+    plan.assemble(fe, capi.coeff(capi.COEFF_EXPCELL, 1.0, theta), capi.coeff(capi.COEFF_CONST, 0.0),
+                  capi.coeff(capi.COEFF_CONST, 1.0), b)
+    batch = capi.Batch(ctx, plan.csr, voff)
+    sols = {}
+    for name, md in (("fp64", capi.PCG_FP64), ("mixed", capi.PCG_MIXED),
+                     ("mixed_matrix", capi.PCG_MIXED_MATRIX)):
+        x.fill(0.0)
+        summ, infos = batch.pcg(b, x, rtol=1e-10, maxit=60000, mode=md)
+        assert summ.converged and all(i.converged for i in infos), name
+        sols[name] = x.numpy()
+    for name in ("mixed", "mixed_matrix"):
+        for k in range(256):
+            a, r = sols[name][voff[k]:voff[k + 1]], sols["fp64"][voff[k]:voff[k + 1]]
+            assert np.linalg.norm(a - r) <= 1e-8 * np.linalg.norm(r), (name, k)
+    for h in (batch, x, b, theta, plan, fe, mesh):
+        h.close()
+
+
+def test_c5_full_size_adjoint_gradient(ctx):
+    """dJ/dtheta on the 1024 x 1024 mesh (theta, theta* ~ N(0,1), u_obs =
+    u(theta*) + 1 % noise on interior DOFs) through the autograd objective,
+    against central differences along the gradient and along a mix of the
+    gradient and a random direction.  (Along a purely random unit direction
+    in 2M dimensions the derivative is ~|grad J| / 1400, about 4e-13 here,
+    and the solves' noise in J divided by 2 eps limits the agreement to
+    ~1e-3.)"""
+    from lehrfempp_amd.autograd import AdjointObjective
+    n = 1024
+    xy, cells, _ = capi.tp_triag_mesh(n)
+    nv, ne = xy.shape[0], cells.shape[0]
+    j, i = np.divmod(np.arange(nv), n + 1)
+    flags = ((i == 0) | (i == n) | (j == 0) | (j == n)).astype(np.uint8)
+    theta = np.random.default_rng(1).standard_normal(ne)
+    theta_star = np.random.default_rng(2).standard_normal(ne)
+    obj = AdjointObjective(ctx, xy, cells, flags, np.zeros(nv), rtol=1e-13)
+    _, ustar = obj.value(theta_star)
+    obj.close()
+    noise = np.random.default_rng(3).standard_normal(nv) * (1 - flags)
+    obj = AdjointObjective(ctx, xy, cells, flags, ustar + 0.01 * np.abs(ustar).max() * noise,
+                           rtol=1e-13)
+    J0, _ = obj.value(theta)
+    g = obj.gradient(theta)
+    rng = np.random.default_rng(7)
+    eps = 1e-5
+    gn = g / np.linalg.norm(g)
+    r = rng.standard_normal(ne)
+    r /= np.linalg.norm(r)
+    for dv in (gn, (gn + r) / np.linalg.norm(gn + r)):
+        fd = (obj.value(theta + eps * dv)[0] - obj.value(theta - eps * dv)[0]) / (2 * eps)
+        assert abs(fd - g @ dv) <= 1e-5 * abs(fd), (fd, g @ dv)
+    obj.close()
