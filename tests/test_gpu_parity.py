@@ -477,6 +477,17 @@ def test_pcg_graph_replay_matches_plain_launches(ctx, degree):
         x0.fill(0.0)
         capi.pcg(ctx, plan.csr, b, x0, rtol=1e-10, check_every=8, graph=-1)
         assert np.array_equal(x0.numpy(), x_ref)
+    # new values assembled into the same matrix (same storage): the chunk
+    # captured for xs[0] is reused and must read them
+    va.upload(np.exp(np.random.default_rng(11).standard_normal(m.ne)))
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 1.0),
+                  capi.coeff(capi.COEFF_CONST if degree == 1 else capi.COEFF_SINSIN, 1.0), b)
+    x0.fill(0.0)
+    ref2 = capi.pcg(ctx, plan.csr, b, x0, rtol=1e-10, check_every=8, graph=-1)
+    xs[0].fill(0.0)
+    info = capi.pcg(ctx, plan.csr, b, xs[0], rtol=1e-10, check_every=8, graph=1)
+    assert info.iters == ref2.iters and np.array_equal(xs[0].numpy(), x0.numpy())
+    assert not np.array_equal(x0.numpy(), x_ref)
     for h in xs + [y, x0, va, b, plan, fe, g]:
         h.close()
 
