@@ -1789,7 +1789,7 @@ void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count,
 // calls on one matrix); two slots serve callers that alternate between two
 // solution vectors (the time stepper's stages).
 // ---------------------------------------------------------------------------
-enum { CG_CHUNK_DIA = 0, CG_CHUNK_ELL = 1, CG_ROWS_ELL = 2 };
+enum { CG_CHUNK_DIA = 0, CG_CHUNK_ELL = 1, CG_ROWS_ELL = 2, CG_CHUNK_DIA_WIDE = 3 };
 
 struct CgGraph {
   hipGraph_t graph = nullptr;
@@ -1825,6 +1825,8 @@ void launch_fp64_chunk(hipStream_t s, int kind, const CgArgs &ca, const RowArgs 
                        bool sym) {
   if (kind == CG_CHUNK_DIA)
     launch_cg_chunk<FMT_DIA, kSB64>(s, ca.sw, ca, count, sym);
+  else if (kind == CG_CHUNK_DIA_WIDE)
+    launch_cg_chunk<FMT_DIA, kSBMix>(s, ca.sw, ca, count, sym);
   else if (kind == CG_CHUNK_ELL)
     launch_cg_chunk<FMT_ELL, kSBMix>(s, ca.sw, ca, count, false);
   else
@@ -2998,9 +3000,12 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   const int32_t n = A->nrows;
   const bool single = opts->mode == LFGPU_PCG_FP64 && opts->single_reduction;
   // the set-up sweep runs on the workgroup shape of the iteration that
-  // follows: 512 threads for the mixed sweeps and for FP64 on sliced ELL,
-  // 256 for FP64 on DIA (and the single-reduction variant)
-  const int sb = (mixed || (c->fmt == FMT_ELL && !single)) ? kSBMix : kSB64;
+  // follows: 512 threads for the mixed sweeps, for FP64 on sliced ELL and for
+  // FP64 on DIA from 2^21 rows (2048^2: 929.7 vs 945.8 ms; at 1024^2 and
+  // below the 256-thread split is 9-35% faster), else 256 (FP64 on DIA
+  // below 2^21 rows, and the single-reduction variant)
+  const bool wide = !mixed && !single && c->fmt == FMT_DIA && n >= (1 << 21);
+  const int sb = (mixed || wide || (c->fmt == FMT_ELL && !single)) ? kSBMix : kSB64;
   // the chunked mixed sweeps take at least 4 rows per thread: below ~4M
   // rows the natural split (up to kMaxGrid workgroups) spends more on the
   // per-workgroup prologue (partial sums, reliable-update decision) than it
@@ -3135,7 +3140,8 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   } else {
     for (;;) {
       if (c->fmt == FMT_DIA)
-        LFG_TRY(run_fp64_chunk(c, s, CG_CHUNK_DIA, a, RowArgs{}, check, c->sym64, use_graph));
+        LFG_TRY(run_fp64_chunk(c, s, wide ? CG_CHUNK_DIA_WIDE : CG_CHUNK_DIA, a, RowArgs{}, check,
+                               c->sym64, use_graph));
       else
         LFG_TRY(run_fp64_chunk(c, s, CG_CHUNK_ELL, a, RowArgs{}, check, false, use_graph));
       LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
