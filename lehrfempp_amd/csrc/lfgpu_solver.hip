@@ -3010,8 +3010,16 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   // rows the natural split (up to kMaxGrid workgroups) spends more on the
   // per-workgroup prologue (partial sums, reliable-update decision) than it
   // gains in parallelism (1024^2: 29.5 -> 23.7 us per FP32-Krylov
-  // iteration; the FP64 sweeps are fastest with the natural split)
-  const Sweep sw = make_sweep(n, sb, mixed ? 4 : 1);
+  // iteration; the FP64 sweeps are fastest with the natural split).  The
+  // FP32-Krylov sweeps take 8 from 2^20 rows and 9 from 2^21 rows (1024^2:
+  // 22.0 vs 23.7 us; 2048^2: 719 vs 728 ms, the same iterates), measured
+  // values: the trend is not monotonic (8 measured 65 vs 60 us at 2048^2)
+  int rpt = 1;
+  if (opts->mode == LFGPU_PCG_MIXED)
+    rpt = n >= (1 << 21) ? 9 : (n >= (1 << 20) ? 8 : 4);
+  else if (mixed)
+    rpt = 4;
+  const Sweep sw = make_sweep(n, sb, rpt);
   const dim3 G(sw.ngroups), BF(256);
   const unsigned gflat = grid_for(n, 256);
   // an even number of iterations per check keeps the launch parity aligned
