@@ -753,6 +753,7 @@ __global__ __launch_bounds__(SB) void k_cg_init_finish(CgArgs a) {
     st->refine[0] = st->refine[1] = 0;
     st->converged = 0;
     st->done = 0;
+    st->stop_pending = 0;
     st->refinements = 0;
     st->last_refine = 0;
     const double thr = st->thr * bb;  // thr holds tol^2 on entry
@@ -1409,7 +1410,15 @@ template <int FMT, class VT, int SB>
 __global__ __launch_bounds__(SB) void k_mx_a(MixArgs<VT> a, int par) {
   __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
-  if (st->done) return;
+  // both flags loaded before either branch (they sit on different lines)
+  const int done = st->done, pending = st->stop_pending;
+  if (done) return;
+  // a stop decided by the previous sweep takes effect here, after every
+  // workgroup of that sweep has folded its rows (all workgroups return)
+  if (pending) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+    return;
+  }
   double t[1];
   const int sl_in[1] = {X_RR + (par ^ 1)};
   reduce_partials_k<SB, 1>(a.part, a.sw.ngroups, sl_in, t, sh);
@@ -1498,7 +1507,9 @@ __global__ __launch_bounds__(SB) void k_mx_b(MixArgs<VT> a, int par) {
       st->refinements += 1;
       st->last_refine = its;
       if (stop) {
-        st->done = 1;
+        // not `done`: the other workgroups of this sweep test `done` on
+        // entry and would skip folding their rows
+        st->stop_pending = 1;
         st->converged = conv ? 1 : 0;
       } else {
         st->rr_ref = rt;
@@ -1903,7 +1914,8 @@ int run_mixed(hipStream_t s, const lfgpu_solver_cache *c, const Sweep &sw, const
     LFG_HIP(hipGetLastError());
     LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     LFG_HIP(hipStreamSynchronize(s));
-    if (hs->done) break;
+    // a pending stop: every workgroup has folded its rows already
+    if (hs->done || hs->stop_pending) break;
   }
   hipLaunchKernelGGL(k_mix_final<VT>, dim3(gflat), dim3(256), 0, s, m);
   LFG_HIP(hipGetLastError());
@@ -2174,6 +2186,7 @@ struct BMState {
   int iter, done, converged, maxit;
   int iterp[2], refine[2];
   int last_refine, refinements;
+  int stop_pending;  // a stop decided in k_bmx_b, applied by the next k_bmx_a
 };
 
 // partial slots: the X_* slots of the single solve, plus b.b
@@ -2253,6 +2266,7 @@ __global__ __launch_bounds__(256) void k_bmix_init_finish(BMArgs<VT> a) {
     st->refine[0] = st->refine[1] = 0;
     st->converged = 0;
     st->done = 0;
+    st->stop_pending = 0;
     st->refinements = 0;
     st->last_refine = 0;
     const double thr = st->thr * bb;  // thr holds tol^2 on entry
@@ -2283,9 +2297,15 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   BMState *st = a.st + G.prob;
-  if (st->done) return;
+  const int done = st->done, pending = st->stop_pending;
+  if (done) return;
   const BProb P = a.probs[G.prob];
   const bool writer = blockIdx.x == (unsigned)G.gb && threadIdx.x == 0;
+  // a stop decided by the previous sweep takes effect here (see k_mx_a)
+  if (pending) {
+    if (writer) st->done = 1;
+    return;
+  }
   const double rr = seg_reduce(mslot(a, X_RR + (par ^ 1)), P.gb, P.ge, sh);
   const int itp = st->iterp[par ^ 1];
   const bool refine = rr < st->thr || rr < a.delta2 * st->rr_ref ||
@@ -2368,7 +2388,7 @@ __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
       st->refinements += 1;
       st->last_refine = its;
       if (stop) {
-        st->done = 1;
+        st->stop_pending = 1;  // as in k_mx_b
         st->converged = conv ? 1 : 0;
       } else {
         st->rr_ref = rt;
@@ -2899,7 +2919,8 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
                            hipMemcpyDeviceToHost, s));
     LFG_HIP(hipStreamSynchronize(s));
     bool all = true;
-    for (int p = 0; p < bt->nprob && all; ++p) all = bt->h_mst[p].done != 0;
+    for (int p = 0; p < bt->nprob && all; ++p)
+      all = bt->h_mst[p].done != 0 || bt->h_mst[p].stop_pending != 0;
     if (all) break;
   }
   hipLaunchKernelGGL(k_bmix_final<VT>, G, B, 0, s, a);

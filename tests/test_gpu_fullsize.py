@@ -161,6 +161,11 @@ def test_c4_full_size_batch(ctx):
         for k in range(256):
             a, r = sols[name][voff[k]:voff[k + 1]], sols["fp64"][voff[k]:voff[k + 1]]
             assert np.linalg.norm(a - r) <= 1e-8 * np.linalg.norm(r), (name, k)
+    # the mixed solves are bitwise repeatable: a stop found by one workgroup
+    # must not make the problem's other workgroups skip their final fold
+    x.fill(0.0)
+    batch.pcg(b, x, rtol=1e-10, maxit=60000, mode=capi.PCG_MIXED)
+    assert np.array_equal(x.numpy(), sols["mixed"])
     for h in (batch, x, b, theta, plan, fe, mesh):
         h.close()
 
