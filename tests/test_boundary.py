@@ -29,6 +29,36 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(capi.SIGNATURES)
 
 
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """Every struct the ctypes binding mirrors has the C compiler's size and
+    field offsets (a program built by gcc against include/lfgpu.h prints
+    them), so option and info blocks cross the ABI unchanged."""
+    import ctypes
+    import subprocess
+    mirrors = {"lfgpu_coeff": capi.Coeff, "lfgpu_pcg_opts": capi.PcgOpts,
+               "lfgpu_pcg_info": capi.PcgInfo, "lfgpu_strang_info": capi.StrangInfo}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "lfgpu.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append('  printf("%s sizeof %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('  printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n"):
+        if ln:
+            cname, key, val = ln.split()
+            got[(cname, key)] = int(val)
+    for cname, py in mirrors.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
 def test_error_path_without_gpu_work():
     # invalid arguments are rejected before any device call
     with pytest.raises(capi.LfgpuError) as e:
