@@ -753,7 +753,6 @@ __global__ __launch_bounds__(SB) void k_cg_init_finish(CgArgs a) {
     st->refine[0] = st->refine[1] = 0;
     st->converged = 0;
     st->done = 0;
-    st->stop_pending = 0;
     st->refinements = 0;
     st->last_refine = 0;
     const double thr = st->thr * bb;  // thr holds tol^2 on entry
@@ -1410,12 +1409,12 @@ template <int FMT, class VT, int SB>
 __global__ __launch_bounds__(SB) void k_mx_a(MixArgs<VT> a, int par) {
   __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
-  // both flags loaded before either branch (they sit on different lines)
-  const int done = st->done, pending = st->stop_pending;
-  if (done) return;
-  // a stop decided by the previous sweep takes effect here, after every
-  // workgroup of that sweep has folded its rows (all workgroups return)
-  if (pending) {
+  // done: 0 running, 1 stopped, 2 stop decided by the previous sweep; it
+  // takes effect here, after every workgroup of that sweep has folded its
+  // rows (all workgroups return either way)
+  const int done = st->done;
+  if (done == 1) return;
+  if (done == 2) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
     return;
   }
@@ -1469,7 +1468,7 @@ template <class VT, int SB>
 __global__ __launch_bounds__(SB) void k_mx_b(MixArgs<VT> a, int par) {
   __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
-  if (st->done) return;
+  if (st->done == 1) return;  // 2 is set by this sweep's own writer
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
   const int32_t r1 = min(r0 + a.sw.rows_per_group, a.sw.n);
   double rr = 0.0, rz = 0.0;
@@ -1507,9 +1506,9 @@ __global__ __launch_bounds__(SB) void k_mx_b(MixArgs<VT> a, int par) {
       st->refinements += 1;
       st->last_refine = its;
       if (stop) {
-        // not `done`: the other workgroups of this sweep test `done` on
-        // entry and would skip folding their rows
-        st->stop_pending = 1;
+        // 2, not 1: the other workgroups of this sweep return on entry only
+        // for 1, so they still fold their rows
+        st->done = 2;
         st->converged = conv ? 1 : 0;
       } else {
         st->rr_ref = rt;
@@ -1914,8 +1913,7 @@ int run_mixed(hipStream_t s, const lfgpu_solver_cache *c, const Sweep &sw, const
     LFG_HIP(hipGetLastError());
     LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     LFG_HIP(hipStreamSynchronize(s));
-    // a pending stop: every workgroup has folded its rows already
-    if (hs->done || hs->stop_pending) break;
+    if (hs->done) break;  // 1, or 2: every workgroup has folded its rows
   }
   hipLaunchKernelGGL(k_mix_final<VT>, dim3(gflat), dim3(256), 0, s, m);
   LFG_HIP(hipGetLastError());
@@ -2186,7 +2184,6 @@ struct BMState {
   int iter, done, converged, maxit;
   int iterp[2], refine[2];
   int last_refine, refinements;
-  int stop_pending;  // a stop decided in k_bmx_b, applied by the next k_bmx_a
 };
 
 // partial slots: the X_* slots of the single solve, plus b.b
@@ -2266,7 +2263,6 @@ __global__ __launch_bounds__(256) void k_bmix_init_finish(BMArgs<VT> a) {
     st->refine[0] = st->refine[1] = 0;
     st->converged = 0;
     st->done = 0;
-    st->stop_pending = 0;
     st->refinements = 0;
     st->last_refine = 0;
     const double thr = st->thr * bb;  // thr holds tol^2 on entry
@@ -2297,12 +2293,11 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   BMState *st = a.st + G.prob;
-  const int done = st->done, pending = st->stop_pending;
-  if (done) return;
+  const int done = st->done;  // 0, 1 or 2 as in k_mx_a
+  if (done == 1) return;
   const BProb P = a.probs[G.prob];
   const bool writer = blockIdx.x == (unsigned)G.gb && threadIdx.x == 0;
-  // a stop decided by the previous sweep takes effect here (see k_mx_a)
-  if (pending) {
+  if (done == 2) {
     if (writer) st->done = 1;
     return;
   }
@@ -2352,7 +2347,7 @@ __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
   BMState *st = a.st + G.prob;
-  if (st->done) return;
+  if (st->done == 1) return;  // 2 is set by this sweep's own writer
   const BProb P = a.probs[G.prob];
   const bool writer = blockIdx.x == (unsigned)G.gb && threadIdx.x == 0;
   double rr = 0.0, rz = 0.0;
@@ -2388,7 +2383,7 @@ __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
       st->refinements += 1;
       st->last_refine = its;
       if (stop) {
-        st->stop_pending = 1;  // as in k_mx_b
+        st->done = 2;  // as in k_mx_b
         st->converged = conv ? 1 : 0;
       } else {
         st->rr_ref = rt;
@@ -2919,8 +2914,7 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
                            hipMemcpyDeviceToHost, s));
     LFG_HIP(hipStreamSynchronize(s));
     bool all = true;
-    for (int p = 0; p < bt->nprob && all; ++p)
-      all = bt->h_mst[p].done != 0 || bt->h_mst[p].stop_pending != 0;
+    for (int p = 0; p < bt->nprob && all; ++p) all = bt->h_mst[p].done != 0;
     if (all) break;
   }
   hipLaunchKernelGGL(k_bmix_final<VT>, G, B, 0, s, a);
