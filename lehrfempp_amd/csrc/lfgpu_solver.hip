@@ -807,9 +807,11 @@ __global__ __launch_bounds__(SB) void k_cg_spmv(CgArgs a, int par) {
     // program order
     const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
     const double qi = apply64<FMT, true, SYM>(a.m, a.p, i, pi);
-    // q goes past the caches (read once, by the update sweep): 2% faster at
-    // 2048^2; the same store measured slower in the mixed and row sweeps
-    __builtin_nontemporal_store(qi, &a.q[i]);
+    // in the 512-thread sweeps (2^21 rows and more) q goes past the caches:
+    // it is read once, by the update sweep (2% faster at 2048^2); the same
+    // store measured slower in the mixed and row sweeps
+    if constexpr (SB == 512) __builtin_nontemporal_store(qi, &a.q[i]);
+    else a.q[i] = qi;
     const double dq = di * qi;
     pq += pi * qi;
     qdr += dq * ri;
@@ -845,11 +847,22 @@ __global__ __launch_bounds__(SB) void k_cg_update(CgArgs a, int par) {
     // before the next write, so they bypass the caches and leave room for
     // p, r and D^-1, which the next sweep reads
     const double pi = a.p[i], ri0 = a.r[i], di = a.dinv[i];
-    const double xi = __builtin_nontemporal_load(&a.x[i]);
-    const double qi = __builtin_nontemporal_load(&a.q[i]);
+    // the 512-thread sweeps run the systems of 2^21 rows and more, whose
+    // per-iteration working set exceeds the last-level cache: there x and q
+    // bypass it (2048^2: 964 -> 947 ms); at 1024^2 the working set fits and
+    // plain accesses measured 3% faster
+    double xi, qi;
+    if constexpr (SB == 512) {
+      xi = __builtin_nontemporal_load(&a.x[i]);
+      qi = __builtin_nontemporal_load(&a.q[i]);
+    } else {
+      xi = a.x[i];
+      qi = a.q[i];
+    }
     const double ri = ri0 - alpha * qi;
     const double zi = di * ri;
-    __builtin_nontemporal_store(xi + alpha * pi, &a.x[i]);
+    if constexpr (SB == 512) __builtin_nontemporal_store(xi + alpha * pi, &a.x[i]);
+    else a.x[i] = xi + alpha * pi;
     a.r[i] = ri;
     a.p[i] = zi + beta * pi;
     rr += ri * ri;
