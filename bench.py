@@ -10,6 +10,17 @@ the cached pattern + solve from x = 0.  Mesh, coefficient field and assembly
 plan are device resident before the timed region.  value = cells assembled
 and solved per second over the timed steps (Melem/s).
 
+The default line also carries, as extra keys, the P3 workload (c3) and the
+batched workload (c4) below, each precision mode of every solve, the
+roofline of the dominant kernels (every sweep launch of 200 PCG iterations
+timed by its own hipEvent pair), and, after all device work has finished,
+the oracle checks at full size: the oracle (CPU restatement of the
+reference pipeline) builds its own mesh, assembles the same problem in the
+reference's cell order, compares the pattern bit for bit and the values
+entry by entry with the device CSR, solves it by the textbook Jacobi-PCG to
+1e-12, and every device solution reports its error against that solution
+(err_vs_oracle).  --c2-only / --no-oracle drop those parts.
+
 Other configurations (each prints its own JSON line):
   --config p3       P3 on a 512 x 512 mesh with interior vertices jittered by
                     U(-0.2h, 0.2h), degree-6 quadrature, PCG in FP64 and mixed
@@ -102,6 +113,23 @@ def spmv_bytes(nnz, n, val_bytes=8, vec_bytes=8):
     return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * vec_bytes * n
 
 
+def sweep_bytes(mode, n, sym):
+    """Bytes one sweep of the DIA Jacobi-PCG must move (each array read or
+    written once; the neighbours' p values and, with `sym`, the mirrored
+    lower-diagonal values are re-reads that the caches serve).  P1 DIA
+    storage: 6 off-diagonals, the row sums and the inverse diagonal.
+      FP64 sweep 1 (k_cg_spmv):  p, r, D^-1, row sums and 3 (sym) or 6
+                                 diagonals read, q written: 64N or 88N
+      FP64 sweep 2 (k_cg_update): p, r, D^-1, x, q read, x, r, p written: 64N
+      FP32-Krylov sweep 1 (k_mx_a): p, r, D^-1, row sums, 6 diagonals read
+                                 (4 B each), q written: 44N
+      FP32-Krylov sweep 2 (k_mx_b): p, r, increment, q, D^-1 read,
+                                 increment, r, p written: 32N"""
+    if mode == "fp64":
+        return (32 + 8 * (3 if sym else 6) + 8) * n, 64 * n
+    return 44 * n, 32 * n
+
+
 def pmc_traffic(kernel, name="pmc_traffic_p1_2048.md"):
     """HBM bytes per dispatch of `kernel` recorded in the committed rocprofv3
     --pmc summary (FETCH_SIZE + WRITE_SIZE of the same dispatches, each scaled
@@ -115,21 +143,29 @@ def pmc_traffic(kernel, name="pmc_traffic_p1_2048.md"):
     return int(m.group(1)) if m else None
 
 
-def roofline_entry(sb, spmv_ms, achieved):
-    """Roofline of the FP64 SpMV: achieved = the CSR algorithmic bytes
-    (12 nnz + 4 (N+1) + 16 N) / measured time.  The DIA mirror stores no
-    indices, so the kernel can beat that figure's rate; hbm_gbs is the
-    counted HBM traffic per launch over the same time."""
-    traffic = pmc_traffic("k_spmv_f64")
-    out = {"bound": "hbm", "kernel": "k_spmv_f64 (FP64 SpMV, DIA storage)",
-           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-           "algorithmic_bytes": sb, "avg_us": 1e3 * spmv_ms}
-    if traffic:
-        out["hbm_gbs"] = traffic / (spmv_ms * 1e-3) / 1e9
-        out["hbm_frac"] = out["hbm_gbs"] / HBM_PEAK_GBS
-        out["note"] = ("achieved uses the CSR algorithmic bytes; the DIA storage reads "
-                       "fewer (traffic = counted HBM bytes per launch, hbm_gbs its rate)")
+def roofline_entry(n, times_ms, sym, mode="fp64"):
+    """Roofline of the dominant kernel of the timed step: the first sweep of
+    the Jacobi-PCG iteration (q = A p with the three dot products), achieved
+    = its algorithmic bytes (sweep_bytes) / its median duration between
+    hipEvents on the solver's stream; the second sweep is reported beside
+    it.  traffic = HBM bytes per dispatch counted by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, calibrated on copy kernels) from the committed summary."""
+    b1, b2 = sweep_bytes(mode, n, sym)
+    k1 = "k_cg_spmv" if mode == "fp64" else "k_mx_a"
+    k2 = "k_cg_update" if mode == "fp64" else "k_mx_b"
+    a1 = b1 / (times_ms[0] * 1e-3) / 1e9
+    a2 = b2 / (times_ms[1] * 1e-3) / 1e9
+    it_us = 1e3 * (times_ms[0] + times_ms[1])
+    out = {"bound": "hbm", "kernel": "%s (Jacobi-PCG sweep 1: q = A p and 3 dot products, "
+                                     "DIA storage%s)" % (k1, ", upper diagonals" if sym else ""),
+           "achieved": a1, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a1 / HBM_PEAK_GBS,
+           "traffic": pmc_traffic(k1), "algorithmic_bytes": b1, "avg_us": 1e3 * times_ms[0],
+           "timing": "median of per-launch hipEvent pairs on the solver stream",
+           "sweep2": {"kernel": k2, "algorithmic_bytes": b2, "avg_us": 1e3 * times_ms[1],
+                      "achieved": a2, "frac": a2 / HBM_PEAK_GBS, "traffic": pmc_traffic(k2)},
+           "iteration": {"us": it_us, "algorithmic_bytes": b1 + b2,
+                         "achieved": (b1 + b2) / (it_us * 1e-6) / 1e9,
+                         "frac": (b1 + b2) / (it_us * 1e-6) / 1e9 / HBM_PEAK_GBS}}
     return out
 
 
@@ -153,54 +189,66 @@ def row_scaled_err(rp, a, b):
 
 
 # ---------------------------------------------------------------- configs
-def cpu_baseline_p1(P, gpu_iters):
-    """Reference-faithful CPU path (the oracle, one thread) on a bounded sample."""
-    from oracle import lfref as L  # CPU restatement, timed here as the CPU baseline
-    m = L.tp_triag_mesh(512)
-    cd, nd = m.dofs(1)
-    a = np.exp(np.random.default_rng(0).standard_normal(m.ne))
-    fl = m.dirichlet_flags(1)
-    t0 = time.perf_counter()
-    L.cpu_path_assemble(m, 1, cd, nd, a, 1.0, L.F_SINSIN, 0.0, fl)
-    t_cell = (time.perf_counter() - t0) / m.ne
-    log("cpu assembly sample: %d cells, %.2f us/cell" % (m.ne, 1e6 * t_cell))
-    rp, col, val = P.plan.csr.download()
-    b = P.b.numpy()
+def cpu_baseline_p1(ne, nv, rp, col, val, b, gpu_iters, oracle=None):
+    """Reference-faithful CPU path (the oracle, one thread): the assembly of
+    the full problem when the oracle check ran (else a 512 x 512 sample
+    scaled per cell) plus Jacobi-PCG per-iteration times scaled to the GPU's
+    iteration count (one thread: 10 iterations timed; all threads: the
+    oracle's full solve)."""
+    L = oracle_lib()
+    if oracle is not None:
+        t_cell = oracle["assembly"]["assembly_s"] / ne
+        asm_sample = "the full %d-cell assembly (one thread)" % ne
+    else:
+        m = L.tp_triag_mesh(512)
+        cd, nd = m.dofs(1)
+        a = np.exp(np.random.default_rng(0).standard_normal(m.ne))
+        fl = m.dirichlet_flags(1)
+        t0 = time.perf_counter()
+        L.cpu_path_assemble(m, 1, cd, nd, a, 1.0, L.F_SINSIN, 0.0, fl)
+        t_cell = (time.perf_counter() - t0) / m.ne
+        asm_sample = ("a 512x512 sample (524,288 cells) scaled per cell to %d cells"
+                      % ne)
+    log("cpu assembly: %.2f us/cell" % (1e6 * t_cell))
     ts = []
     for k in (5, 15):
         t0 = time.perf_counter()
         L.pcg(rp, col, val, b, tol=1e-10, maxit=k)
         ts.append(time.perf_counter() - t0)
     t_iter = (ts[1] - ts[0]) / 10
-    log("cpu pcg sample: %.1f ms/iteration at N=%d" % (1e3 * t_iter, P.nv))
-    step_s = t_cell * P.ne + t_iter * gpu_iters
-    # the same PCG on all the host threads OpenMP is given (OMP_NUM_THREADS)
+    log("cpu pcg sample: %.1f ms/iteration at N=%d" % (1e3 * t_iter, nv))
+    step_s = t_cell * ne + t_iter * gpu_iters
     threads = L.omp_threads()
-    ts = []
-    for k in (10, 50):
-        t0 = time.perf_counter()
-        L.pcg_omp(rp, col, val, b, tol=1e-10, maxit=k)
-        ts.append(time.perf_counter() - t0)
-    t_iter_mt = (ts[1] - ts[0]) / 40
-    log("cpu pcg sample on %d threads: %.2f ms/iteration" % (threads, 1e3 * t_iter_mt))
-    step_mt = t_cell * P.ne + t_iter_mt * gpu_iters
+    if oracle is not None:
+        t_iter_mt = oracle["solve"]["s"] / max(1, oracle["solve"]["iters"])
+        mt_note = "the oracle's full solve to %g (%d iterations)" % (
+            ORACLE_RTOL, oracle["solve"]["iters"])
+    else:
+        ts = []
+        for k in (10, 50):
+            t0 = time.perf_counter()
+            L.pcg_omp(rp, col, val, b, tol=1e-10, maxit=k)
+            ts.append(time.perf_counter() - t0)
+        t_iter_mt = (ts[1] - ts[0]) / 40
+        mt_note = "40 iterations"
+    log("cpu pcg on %d threads: %.2f ms/iteration" % (threads, 1e3 * t_iter_mt))
+    step_mt = t_cell * ne + t_iter_mt * gpu_iters
     return {
-        "value": P.ne / step_s / 1e6,
+        "value": ne / step_s / 1e6,
         "unit": "Melem/s",
         "cores": 1,
         "kind": "port",
-        "sample": ("assembly (Eval per cell, triplets, Dirichlet fix, setFromTriplets) timed on "
-                   "a 512x512 mesh (524,288 cells) and scaled per cell to %d cells; Jacobi-PCG "
-                   "timed for 10 iterations on the full %d-DOF system and scaled to the GPU's "
-                   "%d iterations (extrapolated)" % (P.ne, P.nv, gpu_iters)),
+        "sample": ("assembly (Eval per cell, triplets, Dirichlet fix, setFromTriplets): %s; "
+                   "Jacobi-PCG timed for 10 iterations on the full %d-DOF system and scaled "
+                   "to the GPU's %d iterations (extrapolated)" % (asm_sample, nv, gpu_iters)),
         "assembly_us_per_cell": 1e6 * t_cell,
         "pcg_ms_per_iter": 1e3 * t_iter,
         "pcg_all_cores": {
             "threads": threads, "pcg_ms_per_iter": 1e3 * t_iter_mt,
-            "value": P.ne / step_mt / 1e6,
+            "value": ne / step_mt / 1e6,
             "note": "assembly on one thread as above (the reference's cell loop is serial), "
-                    "Jacobi-PCG with OpenMP loops and reductions timed for 40 iterations and "
-                    "scaled to the GPU's iteration count"},
+                    "Jacobi-PCG with OpenMP loops and reductions, per-iteration time from %s, "
+                    "scaled to the GPU's iteration count" % mt_note},
     }
 
 
@@ -279,6 +327,59 @@ def adjoint_run(capi, ctx, n=1024, fd_dirs=0, seed_theta=1, seed_star=2):
     return out
 
 
+# ---------------------------------------------------------------- the oracle
+# The CPU restatement of the reference pipeline is the checker of the GPU
+# results (never timed as the product): it builds its own mesh, assembles the
+# same problem with the reference's per-cell loop (one thread) and solves it
+# with the textbook Jacobi-PCG (Eigen semantics, plain CSR product) to 1e-12.
+ORACLE_RTOL = 1e-12
+
+
+def oracle_lib():
+    # the OpenMP threads of the oracle's PCG sleep between parallel regions
+    # (read when its library loads)
+    os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+    from oracle import lfref as L  # CPU restatement: the checker
+    return L
+
+
+def oracle_compare_csr(rp_o, col_o, val_o, b_o, rp_g, col_g, val_g, b_g):
+    same = bool(np.array_equal(rp_o, rp_g) and np.array_equal(col_o, col_g))
+    return {"pattern_bitwise_equal": same,
+            "matrix_row_scaled_diff": row_scaled_err(rp_o, val_g, val_o) if same else None,
+            "matrix_bitwise_equal": bool(same and np.array_equal(val_o, val_g)),
+            "rhs_max_rel_diff": float(np.abs(b_g - b_o).max() / max(np.abs(b_o).max(), 1e-300))}
+
+
+def oracle_solve(L, rp, col, val, b, threads=True):
+    t0 = time.perf_counter()
+    if threads:
+        x, it, err = L.pcg_omp(rp, col, val, b, tol=ORACLE_RTOL, maxit=400000)
+    else:
+        x, it, err = L.pcg(rp, col, val, b, tol=ORACLE_RTOL, maxit=400000)
+    dt = time.perf_counter() - t0
+    r = b - L.csr_spmv(rp, col, val, x)
+    return x, {"solver": "Jacobi-PCG, Eigen ConjugateGradient semantics, plain CSR product, "
+                         "%s" % ("OpenMP on %d threads" % L.omp_threads() if threads
+                                 else "one thread"),
+               "rtol": ORACLE_RTOL, "iters": int(it), "relres": float(err),
+               "true_relres": float(np.linalg.norm(r) / np.linalg.norm(b)), "s": dt}
+
+
+def oracle_p1(n, theta, gamma, f_sinsin):
+    """Oracle assembly of the n x n P1 problem with alpha = exp(theta)."""
+    L = oracle_lib()
+    t0 = time.perf_counter()
+    m = L.tp_triag_mesh(n)
+    cd, nd = m.dofs(1)
+    fl = m.dirichlet_flags(1)
+    t1 = time.perf_counter()
+    rp, col, val, b = L.cpu_path_assemble(m, 1, cd, nd, np.exp(theta), gamma,
+                                          L.F_SINSIN if f_sinsin else L.F_CONST, 1.0, fl)
+    t2 = time.perf_counter()
+    return L, (rp, col, val, b), {"mesh_s": t1 - t0, "assembly_s": t2 - t1}
+
+
 def run_p1(args, capi, ctx):
     log("setup n=%d" % args.n)
     P = Problem(capi, ctx, args.n, args.seed)
@@ -305,7 +406,7 @@ def run_p1(args, capi, ctx):
     log("timed %d steps: %.3f s (assembly %.3f ms, pcg %.1f ms, %d iters, relres %.2e)"
         % (args.steps, wall, np.median(asm_ms), np.median(pcg_ms), iters[-1], info.relres))
 
-    # every precision mode, and a tight reference solution to measure errors
+    # every precision mode
     per_mode = {}
     for name, md in modes.items():
         inf = P.solve(md)
@@ -315,10 +416,6 @@ def run_p1(args, capi, ctx):
                           "x": P.x.numpy()}
         log("%s: %.1f ms, %d iters, relres %.2e, true relres %.2e"
             % (name, inf.ms, inf.iters, inf.relres, per_mode[name]["true_relres"]))
-    P.solve(capi.PCG_MIXED_MATRIX, rtol=1e-13)
-    x_ref = P.x.numpy()
-    for name in modes:
-        per_mode[name]["err_vs_ref"] = rel_err(per_mode[name].pop("x"), x_ref)
 
     # assembly rate: median over the timed steps and ten more launches
     asm_ms = asm_ms + [P.assemble() for _ in range(10)]
@@ -330,22 +427,24 @@ def run_p1(args, capi, ctx):
     log("assembly: row-owner %.3f ms, MFMA two-phase %.3f ms"
         % (np.median(asm_ms), np.median(asm_mfma)))
 
-    # roofline of the dominant kernel: the FP64 SpMV sweep
-    xs, ys = capi.Vec(ctx, np.random.default_rng(1).standard_normal(P.nv)), capi.Vec(ctx, P.nv)
-    spmv_ms = capi.spmv_timed(ctx, P.plan.csr, xs, ys, args.spmv_reps)
-    sb = spmv_bytes(P.nnz, P.nv)
-    achieved = sb / (spmv_ms * 1e-3) / 1e9
-    log("spmv: %.1f us, %.0f GB/s (algorithmic)" % (1e3 * spmv_ms, achieved))
-    xs.close()
-    ys.close()
+    # roofline of the dominant kernels: the two sweeps of the FP64 iteration,
+    # each launch timed between its own hipEvents (and the FP32-Krylov sweeps)
+    _, _, sym = capi.solver_storage(P.plan.csr)
+    t64, _ = capi.pcg_sweep_times(ctx, P.plan.csr, P.b, P.x, iters=args.sweep_iters)
+    t32, _ = capi.pcg_sweep_times(ctx, P.plan.csr, P.b, P.x, iters=args.sweep_iters,
+                                  mode=capi.PCG_MIXED)
+    roof = roofline_entry(P.nv, t64, sym)
+    roof_mixed = roofline_entry(P.nv, t32, False, mode="mixed")
+    log("sweeps fp64: %.1f + %.1f us (frac %.3f / %.3f); fp32-Krylov %.1f + %.1f us"
+        % (1e3 * t64[0], 1e3 * t64[1], roof["frac"], roof["sweep2"]["frac"],
+           1e3 * t32[0], 1e3 * t32[1]))
+
+    xs = {name: per_mode[name].pop("x") for name in modes}
 
     adj = None
     if not args.no_adjoint:
         adj = adjoint_run(capi, ctx)
         log("adjoint gradient cost / forward solve = %.2f" % adj["grad_cost_per_solve"])
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline_p1(P, iters[-1])
 
     asm_bytes = 12 * P.ne + 8 * P.ne + 16 * P.nv + 8 * P.nnz + 8 * P.nv
     fp64_s = per_mode["fp64"]["s"]
@@ -380,16 +479,52 @@ def run_p1(args, capi, ctx):
         "grad_cost_per_solve": None if adj is None else adj["grad_cost_per_solve"],
         "adjoint": adj,
         "plan_build_s": P.plan_s,
-        "roofline": roofline_entry(sb, spmv_ms, achieved),
+        "roofline": roof,
+        "roofline_mixed": roof_mixed,
         "pcg_iteration": {
-            "algorithmic_bytes": sb + 80 * P.nv,
-            "us": 1e3 * float(np.median(pcg_ms)) / max(1, iters[-1]),
-            "achieved_gbs": (sb + 80 * P.nv) / (float(np.median(pcg_ms)) * 1e-3
-                                                / max(1, iters[-1])) / 1e9},
-        "cpu_baseline": cpu,
+            "us_in_solve": 1e3 * float(np.median(pcg_ms)) / max(1, iters[-1]),
+            "dia_bytes": roof["iteration"]["algorithmic_bytes"],
+            "dia_gbs_in_solve": roof["iteration"]["algorithmic_bytes"]
+            / (float(np.median(pcg_ms)) * 1e-3 / max(1, iters[-1])) / 1e9,
+            "csr_formula_bytes": spmv_bytes(P.nnz, P.nv) + 80 * P.nv,
+            "note": "dia_bytes: what the DIA sweeps must move (sweep_bytes); csr_formula_bytes: "
+                    "the CSR storage figure 12 nnz + 4 (N+1) + 16 N + 80 N, for reference"},
+        "oracle": None,
+        "cpu_baseline": None,
     }
+    # host copies for the checks that run after every GPU measurement
+    rp, col, val = P.plan.csr.download()
+    bh = P.b.numpy()
+    n, theta, ne, nv = P.n, P.log_alpha, P.ne, P.nv
     P.close()
-    return out
+
+    def checks():
+        # the oracle: its own assembly of the same problem, compared entry by
+        # entry, and its own solve, against which every mode's solution is
+        # measured; then the CPU baseline
+        if not args.no_oracle:
+            L, (rpo, colo, valo, bo), ot = oracle_p1(n, theta, 1.0, True)
+            cmp = oracle_compare_csr(rpo, colo, valo, bo, rp, col, val, bh)
+            log("oracle assembly (%.1f s mesh, %.1f s cells): pattern equal %s, matrix diff "
+                "%.2e, rhs diff %.2e" % (ot["mesh_s"], ot["assembly_s"],
+                                         cmp["pattern_bitwise_equal"],
+                                         cmp["matrix_row_scaled_diff"] or -1,
+                                         cmp["rhs_max_rel_diff"]))
+            xo, sol = oracle_solve(L, rpo, colo, valo, bo)
+            del rpo, colo, valo, bo
+            log("oracle solve: %d iters in %.1f s, relres %.2e, true relres %.2e"
+                % (sol["iters"], sol["s"], sol["relres"], sol["true_relres"]))
+            for name in modes:
+                per_mode[name]["err_vs_oracle"] = rel_err(xs[name], xo)
+            out["oracle"] = {"assembly": {**cmp, **ot, "cells": ne}, "solve": sol,
+                             "note": "the oracle's own mesh, reference-order assembly (alpha = "
+                                     "exp and f = sin sin on the host) and PCG; err_vs_oracle = "
+                                     "||x_gpu - x_oracle|| / ||x_oracle||"}
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline_p1(ne, nv, rp, col, val, bh, iters[-1],
+                                                  out["oracle"])
+
+    return out, checks
 
 
 def run_p3(args, capi, ctx, n=512, seed=0):
@@ -421,24 +556,15 @@ def run_p3(args, capi, ctx, n=512, seed=0):
     plan.assemble(*args_c)
     asm = [plan.assemble(*args_c) for _ in range(3)]
     log("p3: N=%d nnz=%d assembly (exact element order) %.3f ms" % (nd, nnz, np.median(asm)))
-    rp, _, v_exact = plan.csr.download()
+    rp, col_exact, v_exact = plan.csr.download()
     b_exact = b.numpy()
-    # reference solution on the matrix that equals the oracle's bit for bit
-    xr = None
-    if not args.no_solve:
-        x.fill(0.0)
-        ref = capi.pcg(ctx, plan.csr, b, x, rtol=1e-12, mode=capi.PCG_MIXED_MATRIX,
-                       check_every=128, maxit=MAXIT, raise_on_noconv=False)
-        xr = x.numpy()
-        log("p3 reference solve: %d iters, true relres %.2e"
-            % (ref.iters, true_relres(capi, ctx, plan.csr, x, b)))
     # element matrices on the matrix cores; the solves below use this matrix
     plan.assemble(*args_c, mfma=True)
     asm_mfma = [plan.assemble(*args_c, mfma=True) for _ in range(3)]
     _, _, v_mfma = plan.csr.download()
     mat_err = row_scaled_err(rp, v_mfma, v_exact)
     rhs_equal = bool(np.array_equal(b.numpy(), b_exact))
-    del v_exact, v_mfma
+    del v_mfma
     log("p3: assembly (MFMA contraction) %.3f ms, matrix row-scaled diff %.2e, rhs equal %s"
         % (np.median(asm_mfma), mat_err, rhs_equal))
     res = {}
@@ -456,8 +582,7 @@ def run_p3(args, capi, ctx, n=512, seed=0):
                      "true_relres": true_relres(capi, ctx, plan.csr, x, b)}
         log("p3 %s: %.1f ms, %d iters, true relres %.2e" % (name, inf.ms, inf.iters,
                                                             res[name]["true_relres"]))
-    for v in res.values():
-        v["err_vs_ref"] = rel_err(v.pop("x"), xr)
+    xs = {k: v.pop("x") for k, v in res.items()}
     out = {"metric": "P3 assembly Melem/s + PCG time to rtol 1e-10", "config": {
         "workload": "P3 reaction-diffusion, %dx%d jittered (0.2h) triangles, degree-6 "
                     "quadrature, a=exp(N(0,1)), f=sin sin" % (n, n),
@@ -467,13 +592,48 @@ def run_p3(args, capi, ctx, n=512, seed=0):
         "assembly_mfma_ms": float(np.median(asm_mfma)),
         "assembly_mfma_melem_s": cells.shape[0] / (np.median(asm_mfma) * 1e-3) / 1e6,
         "mfma_matrix_row_scaled_diff": mat_err, "mfma_rhs_bitwise_equal": rhs_equal,
-        "pcg_matrix": "MFMA-contracted; err_vs_ref against the exact-order matrix's solution",
-        "pcg": res,
+        "pcg_matrix": "MFMA-contracted; err_vs_oracle against the oracle's solution of its "
+                      "own reference-order system",
+        "pcg": res, "oracle": None,
         "mixed_speedup_vs_fp64": {k: res["fp64"]["s"] / v["s"] for k, v in res.items()}
         if "fp64" in res else None}
     for hh in (b, x, theta, plan, fe, mesh):
         hh.close()
-    return out
+
+    def checks():
+        # the oracle: its own jittered mesh (same coordinates), reference-order
+        # assembly compared with the exact-order device assembly, and its PCG
+        # solve, against which the solutions (of the matrix-core system) are
+        # measured
+        if args.no_oracle:
+            return
+        L = oracle_lib()
+        t0 = time.perf_counter()
+        mo = L.tp_triag_mesh(n)
+        mo = L.TriMesh(xy, mo.cells, mo.edges)
+        cdo, ndo = mo.dofs(3)
+        flo = mo.dirichlet_flags(3)
+        t1 = time.perf_counter()
+        rpo, colo, valo, bo = L.cpu_path_assemble(
+            mo, 3, cdo, ndo, np.exp(np.random.default_rng(seed + 1).standard_normal(mo.ne)),
+            1.0, L.F_SINSIN, 0.0, flo)
+        t2 = time.perf_counter()
+        cmp = oracle_compare_csr(rpo, colo, valo, bo, rp, col_exact, v_exact, b_exact)
+        log("p3 oracle assembly (%.1f s mesh, %.1f s cells): pattern equal %s, matrix diff "
+            "%.2e, rhs diff %.2e" % (t1 - t0, t2 - t1, cmp["pattern_bitwise_equal"],
+                                     cmp["matrix_row_scaled_diff"] or -1,
+                                     cmp["rhs_max_rel_diff"]))
+        sol = None
+        if xs:
+            xo, sol = oracle_solve(L, rpo, colo, valo, bo)
+            log("p3 oracle solve: %d iters in %.1f s, relres %.2e, true relres %.2e"
+                % (sol["iters"], sol["s"], sol["relres"], sol["true_relres"]))
+            for k, v in res.items():
+                v["err_vs_oracle"] = rel_err(xs[k], xo)
+        out["oracle"] = {"assembly": {**cmp, "mesh_s": t1 - t0, "assembly_s": t2 - t1},
+                         "solve": sol}
+
+    return out, checks
 
 
 def run_batched(args, capi, ctx, nprob=256, seed=4):
@@ -493,13 +653,9 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
              capi.coeff(capi.COEFF_CONST, 1.0), b)
     plan.assemble(*acoef)
     batch = capi.Batch(ctx, plan.csr, voff)
-    # reference: every problem to rtol 1e-13 (FP32 matrix, FP64 vectors and
-    # true-residual stopping test)
-    x_ref = None
-    if not args.no_ref:
-        x.fill(0.0)
-        batch.pcg(b, x, rtol=1e-13, maxit=MAXIT, mode=capi.PCG_MIXED_MATRIX)
-        x_ref = x.numpy()
+    rp, col, val = plan.csr.download()
+    bh = b.numpy()
+    sols = {}
     modes = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
              "mixed_matrix": capi.PCG_MIXED_MATRIX}
     wanted = args.batch_modes.split(",")
@@ -509,7 +665,7 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
         x.fill(0.0)
         batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md)  # tunes / captures the graph
         times = []
-        for _ in range(max(1, args.steps)):
+        for _ in range(max(1, args.batch_steps)):
             ctx.sync()
             t0 = time.perf_counter()
             asm_ms = plan.assemble(*acoef)
@@ -517,9 +673,7 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
             summ, infos = batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md)
             ctx.sync()
             times.append((time.perf_counter() - t0, asm_ms, summ.ms))
-        xs = x.numpy()
-        errs = ([rel_err(xs[voff[k]:voff[k + 1]], x_ref[voff[k]:voff[k + 1]])
-                 for k in range(nprob)] if x_ref is not None else [float("nan")])
+        sols[name] = x.numpy()
         it = [i.iters for i in infos]
         wall = float(np.median([t[0] for t in times]))
         per_mode[name] = {
@@ -528,12 +682,11 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
             "pcg_ms": float(np.median([t[2] for t in times])), "iters_max": int(max(it)),
             "iters_min": int(min(it)), "iters_sum": int(sum(it)),
             "refinements_sum": int(sum(i.refinements for i in infos)),
-            "converged": bool(summ.converged), "worst_relres": summ.relres,
-            "worst_err_vs_ref": float(max(errs))}
-        log("batched %s: %.1f ms/step (pcg %.1f ms), iters %d..%d, worst err %.2e"
-            % (name, 1e3 * wall, per_mode[name]["pcg_ms"], min(it), max(it), max(errs)))
-    plain = (None if args.no_plain or x_ref is None else
-             plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff, x_ref))
+            "converged": bool(summ.converged), "worst_relres": summ.relres}
+        log("batched %s: %.1f ms/step (pcg %.1f ms), iters %d..%d"
+            % (name, 1e3 * wall, per_mode[name]["pcg_ms"], min(it), max(it)))
+    plain = (None if args.no_plain else
+             plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff))
     sel = per_mode[args.batch_mode if args.batch_mode in per_mode else next(iter(per_mode))]
     out = {"metric": "batched P1 assemble + PCG, Melem/s", "value": sel["melem_s"],
            "unit": "Melem/s", "pcg_mode": args.batch_mode,
@@ -545,17 +698,76 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
                        capi.STORAGE_BLOCK_DIA: "per-problem DIA"}.get(batch.storage()),
            "speedup_vs_fp64": ({k: per_mode["fp64"]["ms_per_step"] / v["ms_per_step"]
                                 for k, v in per_mode.items()} if "fp64" in per_mode else None),
-           "tuned_rows_per_workgroup": summ.refinements}
+           "tuned_rows_per_workgroup": summ.refinements, "oracle": None}
     if plain is not None:
+        sols["plain_fp64"] = plain.pop("x")
         out["plain_fp64_per_problem"] = plain
         out["speedup_vs_plain_fp64"] = {k: plain["ms_per_step"] / v["ms_per_step"]
                                         for k, v in per_mode.items()}
     for hh in (batch, b, x, theta, plan, fe, mesh):
         hh.close()
-    return out
+
+    def checks():
+        # the oracle: every problem assembled by the reference-order CPU path
+        # on its own mesh and solved by the textbook PCG (one thread per
+        # problem, problems in parallel), against the blocks of the GPU's
+        # matrix and each mode's solution
+        if args.no_oracle:
+            return
+        x_ref, out["oracle"] = oracle_batched(rp, col, val, bh, sizes, g, coff, voff)
+        for name, xv in sols.items():
+            errs = [rel_err(xv[voff[k]:voff[k + 1]], x_ref[voff[k]:voff[k + 1]])
+                    for k in range(nprob)]
+            tgt = out["plain_fp64_per_problem"] if name == "plain_fp64" else per_mode[name]
+            tgt["worst_err_vs_oracle"] = float(max(errs))
+            log("batched %s: worst error vs the oracle %.2e" % (name, max(errs)))
+
+    return out, checks
 
 
-def plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff, x_ref):
+def oracle_batched(rp, col, val, bg, sizes, g, coff, voff):
+    """Oracle systems and solutions of the batched problems (alpha = exp(g),
+    gamma = 0, f = 1, zero Dirichlet data), compared block by block with the
+    GPU's matrix (rp, col, val) and load vector bg."""
+    import concurrent.futures as cf
+    L = oracle_lib()
+
+    def one(k):
+        n = int(sizes[k])
+        m = L.tp_triag_mesh(n)
+        cd, nd = m.dofs(1)
+        rpo, colo, valo, bo = L.cpu_path_assemble(m, 1, cd, nd, np.exp(g[coff[k]:coff[k + 1]]),
+                                                  0.0, L.F_CONST, 1.0, m.dirichlet_flags(1))
+        lo, hi = int(voff[k]), int(voff[k + 1])
+        rpg = rp[lo:hi + 1] - rp[lo]
+        cmp = oracle_compare_csr(rpo, colo, valo, bo, rpg, col[rp[lo]:rp[hi]] - lo,
+                                 val[rp[lo]:rp[hi]], bg[lo:hi])
+        xo, sol = oracle_solve(L, rpo, colo, valo, bo, threads=False)
+        return xo, cmp, sol
+
+    t0 = time.perf_counter()
+    workers = min(16, os.cpu_count() or 1)
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        res = list(ex.map(one, range(len(sizes))))
+    dt = time.perf_counter() - t0
+    x_ref = np.concatenate([r[0] for r in res])
+    cmps = [r[1] for r in res]
+    sols = [r[2] for r in res]
+    diffs = [c["matrix_row_scaled_diff"] for c in cmps if c["matrix_row_scaled_diff"] is not None]
+    out = {"problems": len(sizes), "s": dt, "threads": workers,
+           "pattern_bitwise_equal": all(c["pattern_bitwise_equal"] for c in cmps),
+           "matrix_bitwise_equal": all(c["matrix_bitwise_equal"] for c in cmps),
+           "matrix_row_scaled_diff": float(max(diffs)) if diffs else None,
+           "rhs_max_rel_diff": float(max(c["rhs_max_rel_diff"] for c in cmps)),
+           "solve": {"rtol": ORACLE_RTOL, "iters_sum": int(sum(s_["iters"] for s_ in sols)),
+                     "worst_true_relres": float(max(s_["true_relres"] for s_ in sols))}}
+    log("batched oracle: %d problems in %.1f s (%d threads), pattern equal %s, matrix diff "
+        "%.2e" % (len(sizes), dt, workers, out["pattern_bitwise_equal"],
+                  out["matrix_row_scaled_diff"] or -1))
+    return x_ref, out
+
+
+def plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff):
     """The same problems through the plain FP64 path: one plan, one assembly
     and one lfgpu_pcg call per problem, in sequence."""
     probs = []
@@ -580,22 +792,21 @@ def plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff, x_ref):
 
     step()  # builds every solver mirror
     times = []
-    for _ in range(max(1, args.steps)):
+    for _ in range(max(1, args.batch_steps)):
         ctx.sync()
         t0 = time.perf_counter()
         its = step()
         ctx.sync()
         times.append(time.perf_counter() - t0)
-    errs = [rel_err(pr[4].numpy(), x_ref[voff[k]:voff[k + 1]]) for k, pr in enumerate(probs)]
+    x_all = np.concatenate([pr[4].numpy() for pr in probs])
     wall = float(np.median(times))
-    log("plain fp64 per problem: %.1f ms/step, iters %d..%d, worst err %.2e"
-        % (1e3 * wall, min(its), max(its), max(errs)))
+    log("plain fp64 per problem: %.1f ms/step, iters %d..%d" % (1e3 * wall, min(its), max(its)))
     for pr in probs:
         for hh in pr[::-1]:
             hh.close()
     fe.close()
     return {"ms_per_step": 1e3 * wall, "calls_per_step": 2 * len(sizes),
-            "iters_sum": int(sum(its)), "worst_err_vs_ref": float(max(errs))}
+            "iters_sum": int(sum(its)), "x": x_all}
 
 
 def _fkpp_setup(capi, ctx, n, c=1.2, h=1.0):
@@ -878,8 +1089,14 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="p3: assembly only")
     ap.add_argument("--no-plain", action="store_true",
                     help="batched: skip the per-problem plain FP64 comparison")
-    ap.add_argument("--no-ref", action="store_true",
-                    help="batched: skip the 1e-13 reference solve (one graph capture per mode)")
+    ap.add_argument("--no-oracle", action="store_true",
+                    help="skip the oracle's assembly and solves (parity at full size)")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="default configuration: only the 2048^2 P1 problem (no P3 / batched)")
+    ap.add_argument("--batch-steps", type=int, default=3,
+                    help="batched: timed repetitions per precision mode")
+    ap.add_argument("--sweep-iters", type=int, default=200,
+                    help="PCG iterations whose sweeps are timed one by one (roofline)")
     ap.add_argument("--fd-dirs", type=int, default=64)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--fkpp-n", type=int, default=1024)
@@ -896,12 +1113,22 @@ def main():
 
     from lehrfempp_amd import capi
     ctx = capi.Context(0)
+    checks = []
     if args.config == "p1":
-        out = run_p1(args, capi, ctx)
+        out, chk = run_p1(args, capi, ctx)
+        checks.append(chk)
+        if not args.c2_only:
+            # the P3 and batched configurations ride along as extra keys
+            out["c3"], chk = run_p3(args, capi, ctx)
+            checks.append(chk)
+            out["c4"], chk = run_batched(args, capi, ctx)
+            checks.append(chk)
     elif args.config == "p3":
-        out = run_p3(args, capi, ctx)
+        out, chk = run_p3(args, capi, ctx)
+        checks.append(chk)
     elif args.config == "batched":
-        out = run_batched(args, capi, ctx)
+        out, chk = run_batched(args, capi, ctx)
+        checks.append(chk)
     elif args.config == "traffic":
         out = run_traffic(args, capi, ctx)
     elif args.config == "fkpp":
@@ -915,6 +1142,10 @@ def main():
         out = {"metric": "adjoint gradient cost / forward solve", "config": {
             "workload": "P1 1024x1024, theta ~ N(0,1) per cell, u_obs = u(theta*) + 1% noise"},
             **out}
+    # the oracle checks and the CPU baseline run after every GPU measurement,
+    # so that no CPU-side work overlaps the timed device work
+    for chk in checks:
+        chk()
     ctx.close()
     print(json.dumps(out))
 

@@ -313,6 +313,27 @@ typedef struct lfgpu_pcg_info {
 int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *b,
               lfgpu_vec *x, const lfgpu_pcg_opts *opts, lfgpu_pcg_info *info);
 
+/* Storage of the solver's copy of A, built by the first SpMV / PCG call
+ * (storage -1 before): LFGPU_STORAGE_DIA or LFGPU_STORAGE_ELL (defined with
+ * the batched solves below), *ndiag the stored off-diagonals of DIA, *sym 1
+ * when the DIA values are mirror-symmetric bit for bit, so that the FP64
+ * product sweep reads the upper diagonals only (ndiag and sym may be NULL).
+ * Introspection for measurement; no reference counterpart. */
+int lfgpu_solver_storage(const lfgpu_csr *A, int *storage, int *ndiag, int *sym);
+
+/* lfgpu_pcg with every sweep kernel of the iterations bracketed by its own
+ * hipEvent pair on the context stream (plain launches, no graph replay), for
+ * measuring kernel durations; at most max_kernels kernels are timed.
+ * out_ms[0], out_ms[1], out_ms[2]: median duration in milliseconds of the
+ * first sweep of an iteration (the product), the second sweep (the vector
+ * update) and the one-workgroup reduction kernels of the sliced-ELL row
+ * sweeps (0 where a kind did not run).  Measurement support; no reference
+ * counterpart.  Returns what lfgpu_pcg returns. */
+int lfgpu_pcg_sweep_times(lfgpu_ctx *ctx, const lfgpu_csr *A,
+                          const lfgpu_vec *b, lfgpu_vec *x,
+                          const lfgpu_pcg_opts *opts, int max_kernels,
+                          double *out_ms, lfgpu_pcg_info *info);
+
 /* ---- batched solves --------------------------------------------------
  * Many independent SPD systems stored as one block-diagonal matrix (e.g. the
  * plan of a mesh that is the disjoint union of the problems' meshes, so one
@@ -322,7 +343,11 @@ int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *b,
  * advance together, one pair of sweeps per iteration; chunks of iterations
  * are captured into a hipGraph once and replayed, and the workgroup tile is
  * chosen by timing candidates on the first solve.  No reference
- * counterpart: a batch replaces nprob separate solve calls. */
+ * counterpart: a batch replaces nprob separate solve calls.
+ * lfgpu_batch_create checks on the device that every entry of A lies inside
+ * its row's problem block and returns LFGPU_EINVAL otherwise.
+ * opts->graph: >= 0 replays the captured chunk (default); -1 launches each
+ * chunk's kernels directly (the same iterates, no graph is captured). */
 typedef struct lfgpu_batch lfgpu_batch;
 int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
                        const int32_t *row_offsets, lfgpu_batch **out);

@@ -104,6 +104,10 @@ SIGNATURES = {
     "lfgpu_spmv": (C.c_int, [_P, _P, _P, _P]),
     "lfgpu_spmv_timed": (C.c_int, [_P, _P, _P, _P, C.c_int, _d]),
     "lfgpu_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo)]),
+    "lfgpu_solver_storage": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]),
+    "lfgpu_pcg_sweep_times": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.c_int,
+                                         _d, C.POINTER(PcgInfo)]),
     "lfgpu_batch_create": (C.c_int, [_P, _P, C.c_int, _i32, C.POINTER(_P)]),
     "lfgpu_batch_destroy": (C.c_int, [_P]),
     "lfgpu_batch_storage": (C.c_int, [_P, C.POINTER(C.c_int)]),
@@ -441,6 +445,29 @@ def pcg(ctx, A: Csr, b: Vec, x: Vec, rtol=1e-10, maxit=None, mode=PCG_FP64, chec
     return info
 
 
+def solver_storage(A: Csr):
+    """(storage, ndiag, sym) of the solver's copy of A (storage -1 before the
+    first solve)."""
+    st, nd, sy = C.c_int(), C.c_int(), C.c_int()
+    check(lib().lfgpu_solver_storage(A.h, C.byref(st), C.byref(nd), C.byref(sy)))
+    return st.value, nd.value, bool(sy.value)
+
+
+def pcg_sweep_times(ctx, A: Csr, b: Vec, x: Vec, iters=200, mode=PCG_FP64, max_kernels=None):
+    """Runs `iters` PCG iterations (rtol 0) with every sweep kernel timed by
+    its own hipEvent pair; returns the median durations (ms) of the first
+    sweep, the second sweep and the row-sweep reduction kernels."""
+    n = A.info()[0]
+    opts = PcgOpts(0.0, int(iters), mode, 64, 0.0, 0, 0, -1)
+    out = (C.c_double * 3)()
+    info = PcgInfo()
+    rc = lib().lfgpu_pcg_sweep_times(ctx.h, A.h, b.h, x.h, C.byref(opts),
+                                     int(max_kernels or 4 * iters + 8), out, C.byref(info))
+    if rc != OK and rc != ENOCONV:
+        check(rc)
+    return list(out), info
+
+
 class Batch:
     """Batched Jacobi-PCG over a block-diagonal matrix (lfgpu_batch_*)."""
 
@@ -453,8 +480,12 @@ class Batch:
                                        C.byref(self.h)))
 
     def pcg(self, b: Vec, x: Vec, rtol=1e-10, maxit=100000, check_every=64,
-            raise_on_noconv=True, mode=PCG_FP64, refine_delta=0.0, refine_max_interval=0):
-        opts = PcgOpts(rtol, int(maxit), mode, check_every, refine_delta, refine_max_interval, 0)
+            raise_on_noconv=True, mode=PCG_FP64, refine_delta=0.0, refine_max_interval=0,
+            graph=0):
+        """lfgpu_batch_pcg.  graph: 0 = replay the captured chunk, -1 = plain
+        launches."""
+        opts = PcgOpts(rtol, int(maxit), mode, check_every, refine_delta, refine_max_interval, 0,
+                       int(graph))
         infos = (PcgInfo * self.nprob)()
         summ = PcgInfo()
         rc = lib().lfgpu_batch_pcg(self.ctx.h, self.h, b.h, x.h, C.byref(opts), infos,

@@ -100,6 +100,11 @@ struct lfgpu_solver_cache {
   // holds the kernels' argument blocks, defined with them below)
   void *cg_graphs = nullptr;
   int cg_graph_next = 0;
+  // sliced ELL of a single (not batched) matrix: the storage holds the rows
+  // in a locality order, perm[new] = old row, iperm[old] = new row, and the
+  // solver works on permuted copies of x and b
+  int32_t *perm = nullptr, *iperm = nullptr;
+  double *xp = nullptr, *bp = nullptr;
 };
 
 namespace lfgpu {
@@ -113,7 +118,8 @@ void solver_cache_free(lfgpu_csr *A) {
                   (void *)c->dinv, (void *)c->dinv32, (void *)c->rowsum, (void *)c->rowsum32,
                   (void *)c->r, (void *)c->p, (void *)c->q, (void *)c->r32, (void *)c->p32,
                   (void *)c->q32, (void *)c->xa32, (void *)c->xlo, (void *)c->r64, (void *)c->xa64,
-                  (void *)c->part, (void *)c->cg1, (void *)c->rpart, (void *)c->st})
+                  (void *)c->part, (void *)c->cg1, (void *)c->rpart, (void *)c->st,
+                  (void *)c->perm, (void *)c->iperm, (void *)c->xp, (void *)c->bp})
     dfree(p);
   delete c;
   A->cache = nullptr;
@@ -303,30 +309,55 @@ __device__ __forceinline__ float apply32(const MatView &m, const float *__restri
 // ---------------------------------------------------------------------------
 // building the mirror
 // ---------------------------------------------------------------------------
+// Sliced-ELL rows are stored in the order perm (row r of the storage is row
+// perm[r] of the CSR; NULL = identity) with columns renamed by iperm; the
+// entries of a row keep their CSR order.
 __global__ void k_slice_width(const int32_t *row_ptr, const int32_t *col, int32_t n,
-                              int32_t nslices, int32_t *w) {
+                              int32_t nslices, const int32_t *perm, int32_t *w) {
   const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslices) return;
   int32_t m = 0;
   const int32_t r0 = s * kSlice, r1 = min(r0 + kSlice, n);
   for (int32_t r = r0; r < r1; ++r) {
+    const int32_t o = perm ? perm[r] : r;
     int32_t k = 0;
-    for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) k += (col[p] != r);
+    for (int32_t p = row_ptr[o]; p < row_ptr[o + 1]; ++p) k += (col[p] != o);
     m = max(m, k);
   }
   w[s] = m * kSlice;
 }
 
 __global__ void k_fill_ell_cols(const int32_t *row_ptr, const int32_t *col, int32_t n,
-                                const int32_t *slice_off, int32_t *s_col) {
+                                const int32_t *slice_off, const int32_t *perm,
+                                const int32_t *iperm, int32_t *s_col) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
+  const int32_t o = perm ? perm[r] : r;
   const int32_t s = r / kSlice, l = r % kSlice;
   const int32_t off = slice_off[s], w = (slice_off[s + 1] - off) / kSlice;
   int32_t k = 0;
-  for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
-    if (col[p] != r) s_col[off + (k++) * kSlice + l] = col[p];
+  for (int32_t p = row_ptr[o]; p < row_ptr[o + 1]; ++p)
+    if (col[p] != o) s_col[off + (k++) * kSlice + l] = iperm ? iperm[col[p]] : col[p];
   for (; k < w; ++k) s_col[off + k * kSlice + l] = r;
+}
+
+// locality key of a row: its smallest column (the CSR columns are sorted)
+__global__ void k_row_key(const int32_t *row_ptr, const int32_t *col, int32_t n, int32_t *key) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  key[r] = row_ptr[r + 1] > row_ptr[r] ? col[row_ptr[r]] : r;
+}
+
+__global__ void k_gather(const double *__restrict__ src, const int32_t *__restrict__ perm,
+                         int32_t n, double *__restrict__ dst) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[r] = src[perm[r]];
+}
+
+__global__ void k_scatter(const double *__restrict__ src, const int32_t *__restrict__ perm,
+                          int32_t n, double *__restrict__ dst) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[perm[r]] = src[r];
 }
 
 struct DiagSet {
@@ -355,10 +386,11 @@ __global__ void k_dia_check(const int32_t *row_ptr, const int32_t *col, int32_t 
 // ELL: the slot layout of k_fill_ell_cols.
 __global__ void k_fill_vals(const int32_t *row_ptr, const int32_t *col, const double *val,
                             int32_t n, int fmt, DiagSet ds, const int32_t *slice_off,
-                            double *mval, float *mval32, double *rowsum, float *rowsum32,
-                            double *dinv, float *dinv32) {
+                            const int32_t *perm, double *mval, float *mval32, double *rowsum,
+                            float *rowsum32, double *dinv, float *dinv32) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
+  const int32_t o = perm ? perm[r] : r;  // the CSR row stored as row r
   double d = 1.0, sum = 0.0, comp = 0.0;
   if (fmt == FMT_DIA) {
     for (int q = 0; q < ds.ndiag; ++q) {
@@ -373,13 +405,13 @@ __global__ void k_fill_vals(const int32_t *row_ptr, const int32_t *col, const do
     w = (slice_off[s + 1] - off) / kSlice;
   }
   int32_t k = 0;
-  for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) {
+  for (int32_t p = row_ptr[o]; p < row_ptr[o + 1]; ++p) {
     const double v = val[p];
     const double t = sum + v;
     comp += fabs(sum) >= fabs(v) ? (sum - t) + v : (v - t) + sum;
     sum = t;
     const int32_t c = col[p];
-    if (c == r) {
+    if (c == o) {
       if (v != 0.0) d = 1.0 / v;
       continue;
     }
@@ -493,8 +525,51 @@ int detect_dia(lfgpu_ctx *ctx, const lfgpu_csr *A, DiagSet *ds) {
   return LFGPU_OK;
 }
 
-int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32) {
+// Locality order of the rows of a sliced-ELL matrix: a stable counting sort
+// by each row's smallest column.  On a finite element matrix whose lowest
+// DOFs (the vertices) are numbered along the mesh, the rows of every DOF kind
+// (vertex, edge, cell DOFs) near one part of the mesh end up next to each
+// other, so a window of consecutive rows gathers from a compact window of
+// the (equally permuted) vector instead of from three distant ranges.
+int ell_locality_order(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_solver_cache *c) {
   hipStream_t s = ctx->stream;
+  const int32_t n = A->nrows;
+  int32_t *dkey = nullptr;
+  LFG_TRY(dalloc(&dkey, (size_t)n, "row keys"));
+  hipLaunchKernelGGL(k_row_key, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col, n,
+                     dkey);
+  std::vector<int32_t> key(n), perm(n), iperm(n);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(key.data(), dkey, (size_t)n * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  dfree(dkey);
+  if (e != hipSuccess) return hip_fail(e, "row keys");
+  std::vector<int32_t> start((size_t)n + 1, 0);
+  for (int32_t r = 0; r < n; ++r) start[(size_t)key[r] + 1]++;
+  for (int32_t k = 0; k < n; ++k) start[(size_t)k + 1] += start[k];
+  for (int32_t r = 0; r < n; ++r) perm[start[key[r]]++] = r;
+  for (int32_t r = 0; r < n; ++r) iperm[perm[r]] = r;
+  LFG_TRY(dalloc(&c->perm, (size_t)n, "row order"));
+  LFG_TRY(dalloc(&c->iperm, (size_t)n, "inverse row order"));
+  LFG_TRY(dalloc(&c->xp, (size_t)n, "permuted solution"));
+  LFG_TRY(dalloc(&c->bp, (size_t)n, "permuted right-hand side"));
+  LFG_HIP(hipMemcpyAsync(c->perm, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+  LFG_HIP(hipMemcpyAsync(c->iperm, iperm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+  LFG_HIP(hipStreamSynchronize(s));
+  return LFGPU_OK;
+}
+
+// Builds (or refreshes) the solver's storage of A.  allow_perm: a new
+// sliced-ELL storage may keep its rows in the locality order; the batched
+// solver needs the original row order (its problems own row ranges) and
+// rebuilds a permuted storage.
+int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
+  hipStream_t s = ctx->stream;
+  if (A->cache && A->cache->perm && !allow_perm) {
+    LFG_HIP(hipStreamSynchronize(s));
+    solver_cache_free(A);
+  }
   lfgpu_solver_cache *c = A->cache;
   const int32_t n = A->nrows;
   if (!c) {
@@ -509,12 +584,13 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32) {
       c->mlen = (int64_t)ds.ndiag * n;
     } else {
       c->fmt = FMT_ELL;
+      if (allow_perm && n > 0) LFG_TRY(ell_locality_order(ctx, A, c));
       c->nslices = (n + kSlice - 1) / kSlice;
       int32_t *w = nullptr;
       LFG_TRY(dalloc(&w, (size_t)c->nslices, "slice widths"));
       LFG_TRY(dalloc(&c->slice_off, (size_t)c->nslices + 1, "slice offsets"));
       hipLaunchKernelGGL(k_slice_width, dim3(grid_for(c->nslices, 256)), dim3(256), 0, s,
-                         A->row_ptr, A->col, n, c->nslices, w);
+                         A->row_ptr, A->col, n, c->nslices, c->perm, w);
       LFG_HIP(hipGetLastError());
       int64_t len = 0;
       int rc = exclusive_scan_i32(s, w, c->slice_off, c->nslices, &len);
@@ -523,7 +599,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32) {
       c->mlen = len;
       LFG_TRY(dalloc(&c->s_col, (size_t)std::max<int64_t>(len, 1), "ELL columns"));
       hipLaunchKernelGGL(k_fill_ell_cols, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr,
-                         A->col, n, c->slice_off, c->s_col);
+                         A->col, n, c->slice_off, c->perm, c->iperm, c->s_col);
       LFG_HIP(hipGetLastError());
     }
     LFG_TRY(dalloc(&c->val, (size_t)std::max<int64_t>(c->mlen, 1), "solver values"));
@@ -554,7 +630,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32) {
     ds.ndiag = c->ndiag;
     for (int d = 0; d < kMaxDiag; ++d) ds.off[d] = c->off[d];
     hipLaunchKernelGGL(k_fill_vals, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col,
-                       A->val, n, c->fmt, ds, c->slice_off, c->val,
+                       A->val, n, c->fmt, ds, c->slice_off, c->perm, c->val,
                        c->have32 ? c->val32 : nullptr, c->rowsum,
                        c->have32 ? c->rowsum32 : nullptr, c->dinv,
                        c->have32 ? c->dinv32 : nullptr);
@@ -576,9 +652,26 @@ __global__ __launch_bounds__(256) void k_spmv_f64(MatView m, const double *__res
   y[r] = apply64<FMT, true>(m, x, r, x[r]);
 }
 
+// y = A x with the storage's rows in the locality order: x is gathered into
+// that order, and row r of the storage is written to y[perm[r]]
+__global__ __launch_bounds__(256) void k_spmv_perm(MatView m, const double *__restrict__ xp,
+                                                   const int32_t *__restrict__ perm,
+                                                   double *__restrict__ y) {
+  const int32_t r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= m.n) return;
+  y[perm[r]] = apply64<FMT_ELL>(m, xp, r, xp[r]);
+}
+
 int launch_spmv(hipStream_t s, const lfgpu_solver_cache *c, int32_t n, const double *x,
                 double *y) {
   const MatView m = view_of(c, n);
+  if (c->perm) {
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, x, c->perm, n, c->xp);
+    hipLaunchKernelGGL(k_spmv_perm, dim3(grid_for(n, 256)), dim3(256), 0, s, m, c->xp, c->perm,
+                       y);
+    LFG_HIP(hipGetLastError());
+    return LFGPU_OK;
+  }
   if (c->fmt == FMT_DIA)
     hipLaunchKernelGGL(k_spmv_f64<FMT_DIA>, dim3(grid_for(n, 256)), dim3(256), 0, s, m, x, y);
   else
@@ -596,6 +689,35 @@ int launch_spmv(hipStream_t s, const lfgpu_solver_cache *c, int32_t n, const dou
 // sweeps (each measured fastest for its kernels on MI355X at 2048^2).
 constexpr int kSB64 = 256;
 constexpr int kSBMix = 512;
+
+
+// Per-kernel timing of the solver sweeps (lfgpu_pcg_sweep_times): while a
+// timer is installed on the calling thread, the chunk launchers bracket every
+// sweep kernel with a pair of events on the solver's stream.  Kind 0 is the
+// first sweep of an iteration (the product), 1 the second (the update), 2 a
+// one-workgroup reduction kernel of the row sweeps.
+struct SweepTimer {
+  std::vector<hipEvent_t> ev;  // start / end pairs in launch order
+  std::vector<int> kind;       // one per pair
+  size_t used = 0;             // pairs recorded
+  hipError_t err = hipSuccess;
+};
+thread_local SweepTimer *t_timer = nullptr;
+
+inline void tmark_begin(hipStream_t s, int kind) {
+  SweepTimer *t = t_timer;
+  if (!t || 2 * t->used + 1 >= t->ev.size()) return;
+  const hipError_t e = hipEventRecord(t->ev[2 * t->used], s);
+  if (e != hipSuccess && t->err == hipSuccess) t->err = e;
+  t->kind[t->used] = kind;
+}
+inline void tmark_end(hipStream_t s) {
+  SweepTimer *t = t_timer;
+  if (!t || 2 * t->used + 1 >= t->ev.size()) return;
+  const hipError_t e = hipEventRecord(t->ev[2 * t->used + 1], s);
+  if (e != hipSuccess && t->err == hipSuccess) t->err = e;
+  t->used += 1;
+}
 
 struct Sweep {
   int32_t n;
@@ -1064,10 +1186,18 @@ template <int FMT>
 void launch_cgr_chunk(hipStream_t s, const RowArgs &a, int count) {
   const dim3 G(a.nwg), B(kRB), B1(kRedB);
   for (int k = 0; k < count; ++k) {
+    tmark_begin(s, 0);
     hipLaunchKernelGGL(k_cgr_spmv<FMT>, G, B, 0, s, a);
+    tmark_end(s);
+    tmark_begin(s, 2);
     hipLaunchKernelGGL(k_cgr_red_s, dim3(1), B1, 0, s, a);
+    tmark_end(s);
+    tmark_begin(s, 1);
     hipLaunchKernelGGL(k_cgr_update, G, B, 0, s, a);
+    tmark_end(s);
+    tmark_begin(s, 2);
     hipLaunchKernelGGL(k_cgr_red_u, dim3(1), B1, 0, s, a);
+    tmark_end(s);
   }
 }
 
@@ -1284,8 +1414,11 @@ __global__ __launch_bounds__(SB) void k_cg1(Cg1Args a, int par) {
 template <int FMT>
 void launch_cg1_chunk(hipStream_t s, const Sweep &sw, const Cg1Args &a, int count) {
   const dim3 G(sw.ngroups), B(kSB64);
-  for (int k = 0; k < count; ++k)
+  for (int k = 0; k < count; ++k) {
+    tmark_begin(s, 0);
     hipLaunchKernelGGL((k_cg1<FMT, kSB64>), G, B, 0, s, a, k & 1);
+    tmark_end(s);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1782,10 +1915,18 @@ template <int FMT, class VT>
 void launch_mxr_chunk(hipStream_t s, const MixArgs<VT> &m, int count) {
   const dim3 G(m.nwg), B(kRB), B1(kRedB);
   for (int k = 0; k < count; ++k) {
+    tmark_begin(s, 0);
     hipLaunchKernelGGL((k_mxr_a<FMT, VT>), G, B, 0, s, m);
+    tmark_end(s);
+    tmark_begin(s, 2);
     hipLaunchKernelGGL(k_mxr_red_a<VT>, dim3(1), B1, 0, s, m);
+    tmark_end(s);
+    tmark_begin(s, 1);
     hipLaunchKernelGGL(k_mxr_b<VT>, G, B, 0, s, m);
+    tmark_end(s);
+    tmark_begin(s, 2);
     hipLaunchKernelGGL(k_mxr_red_b<VT>, dim3(1), B1, 0, s, m);
+    tmark_end(s);
   }
 }
 
@@ -1794,11 +1935,15 @@ void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count,
   const dim3 G(sw.ngroups), B(SB);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
+    tmark_begin(s, 0);
     if (FMT == FMT_DIA && sym)
       hipLaunchKernelGGL((k_cg_spmv<FMT, SB, true>), G, B, 0, s, a, par);
     else
       hipLaunchKernelGGL((k_cg_spmv<FMT, SB>), G, B, 0, s, a, par);
+    tmark_end(s);
+    tmark_begin(s, 1);
     hipLaunchKernelGGL(k_cg_update<SB>, G, B, 0, s, a, par);
+    tmark_end(s);
   }
 }
 
@@ -1859,7 +2004,7 @@ void launch_fp64_chunk(hipStream_t s, int kind, const CgArgs &ca, const RowArgs 
 // One chunk of FP64 iterations, eagerly or from a (re)captured graph.
 int run_fp64_chunk(lfgpu_solver_cache *c, hipStream_t s, int kind, const CgArgs &ca,
                    const RowArgs &ra, int count, bool sym, bool use_graph) {
-  if (!use_graph) {
+  if (!use_graph || t_timer) {
     launch_fp64_chunk(s, kind, ca, ra, count, sym);
     LFG_HIP(hipGetLastError());
     return LFGPU_OK;
@@ -1907,8 +2052,12 @@ void launch_mix_chunk(hipStream_t s, const Sweep &sw, const MixArgs<VT> &m, int 
   const dim3 G(sw.ngroups), B(kSBMix);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
+    tmark_begin(s, 0);
     hipLaunchKernelGGL((k_mx_a<FMT, VT, kSBMix>), G, B, 0, s, m, par);
+    tmark_end(s);
+    tmark_begin(s, 1);
     hipLaunchKernelGGL((k_mx_b<VT, kSBMix>), G, B, 0, s, m, par);
+    tmark_end(s);
   }
 }
 
@@ -2459,14 +2608,19 @@ struct lfgpu_batch {
   BState *h_st = nullptr;  // pinned
   BMState *mst = nullptr;    // mixed-precision states (allocated on first use)
   BMState *h_mst = nullptr;  // pinned
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int graph_count = 0;
-  int graph_mode = -1;
-  const double *graph_x = nullptr;
-  const double *graph_b = nullptr;
-  double graph_delta2 = 0.0;
-  int graph_max_interval = 0;
+  // one captured chunk of iterations per precision mode (index = mode), so
+  // alternating modes replay their own graphs instead of re-capturing; a
+  // slot is reused while every captured argument is unchanged
+  struct Slot {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int count = 0;
+    const double *x = nullptr;
+    const double *b = nullptr;
+    double delta2 = 0.0;
+    int max_interval = 0;
+    bool sym = false;
+  } slot[3];
   double tune_ms[4] = {0, 0, 0, 0};
   // storage used by the batched sweeps: the matrix's own mirror (DIA or
   // ELL) or per-problem DIA (FMT_BDIA); -1 until the first solve decides
@@ -2595,6 +2749,19 @@ __global__ void k_bdia_sym(const double *bval, int32_t n, const int32_t *row_off
   if (bad) atomicOr(flag, 1);
 }
 
+// flag = 1 if a row has an entry outside its own problem's rows (the
+// matrix is not block diagonal over the row offsets)
+__global__ void k_batch_blockcheck(const int32_t *row_ptr, const int32_t *col, int32_t n,
+                                   const int32_t *row_off, int nprob, int *flag) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int p = batch_prob_of(row_off, nprob, r);
+  const int32_t lo = row_off[p], hi = row_off[p + 1];
+  bool bad = false;
+  for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) bad = bad || col[k] < lo || col[k] >= hi;
+  if (bad) atomicOr(flag, 1);
+}
+
 // Chooses the batch's storage on its first solve: the matrix mirror when it
 // is DIA already, per-problem DIA when every problem is a banded matrix
 // whose offsets near its middle row cover all of its rows (and whole
@@ -2607,9 +2774,6 @@ int batch_choose_format(lfgpu_batch *bt) {
   hipStream_t s = bt->ctx->stream;
   const int np = bt->nprob;
   const int32_t n = A->nrows;
-  LFG_TRY(dalloc(&bt->d_row_off, (size_t)np + 1, "batch row offsets"));
-  LFG_HIP(hipMemcpyAsync(bt->d_row_off, bt->row_off.data(), ((size_t)np + 1) * 4,
-                         hipMemcpyHostToDevice, s));
   int32_t *mid = nullptr;
   LFG_TRY(dalloc(&mid, (size_t)np * (kMaxDiag + 2), "batch middle rows"));
   hipLaunchKernelGGL(k_bdia_mid, dim3(grid_for(np, 256)), dim3(256), 0, s, A->row_ptr, A->col,
@@ -2716,11 +2880,49 @@ MatView batch_matview(const lfgpu_batch *bt) {
 }
 
 void batch_release_graph(lfgpu_batch *bt) {
-  if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
-  if (bt->graph) (void)hipGraphDestroy(bt->graph);
-  bt->exec = nullptr;
-  bt->graph = nullptr;
-  bt->graph_count = 0;
+  for (auto &g : bt->slot) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = lfgpu_batch::Slot{};
+  }
+}
+
+// Captures one chunk of `count` iterations into slot `mode` unless the slot
+// already holds a chunk with the same arguments.  The stream is drained
+// first, so nothing queued before the capture is still pending while it
+// runs.
+template <class Launch>
+int batch_capture(lfgpu_batch *bt, int mode, int count, const double *x, const double *b,
+                  double delta2, int max_interval, bool sym, Launch &&launch) {
+  lfgpu_batch::Slot &g = bt->slot[mode];
+  if (g.exec && g.count == count && g.x == x && g.b == b && g.delta2 == delta2 &&
+      g.max_interval == max_interval && g.sym == sym)
+    return LFGPU_OK;
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g = lfgpu_batch::Slot{};
+  hipStream_t s = bt->ctx->stream;
+  LFG_HIP(hipStreamSynchronize(s));
+  LFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  launch();
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture (batched chunk)");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate (batched chunk)");
+  }
+  g.graph = graph;
+  g.exec = exec;
+  g.count = count;
+  g.x = x;
+  g.b = b;
+  g.delta2 = delta2;
+  g.max_interval = max_interval;
+  g.sym = sym;
+  return LFGPU_OK;
 }
 
 // Workgroup table for rows_per_group R (a multiple of 256).
@@ -2903,26 +3105,19 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
   hipLaunchKernelGGL(k_bmix_init_finish<VT>, dim3(bt->nprob), B, 0, s, a);
   hipLaunchKernelGGL(k_bmix_zero_if_bb0<VT>, G, B, 0, s, a);
   LFG_HIP(hipGetLastError());
-  // the captured arguments depend on the layout, x, b and the refinement
-  // options; recapture when any of them changes
-  if (!bt->exec || bt->graph_count != check || bt->graph_x != x || bt->graph_b != b ||
-      bt->graph_mode != mode || bt->graph_delta2 != delta2 ||
-      bt->graph_max_interval != max_interval) {
-    batch_release_graph(bt);
-    LFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    batch_mixed_chunk<VT>(bt, a, check);
-    hipError_t e = hipStreamEndCapture(s, &bt->graph);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    LFG_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
-    bt->graph_count = check;
-    bt->graph_x = x;
-    bt->graph_b = b;
-    bt->graph_mode = mode;
-    bt->graph_delta2 = delta2;
-    bt->graph_max_interval = max_interval;
-  }
+  // the captured arguments depend on the layout (a new layout releases every
+  // slot), x, b and the refinement options
+  const bool use_graph = opts->graph >= 0;
+  if (use_graph)
+    LFG_TRY(batch_capture(bt, mode, check, x, b, delta2, max_interval, false,
+                          [&]() { batch_mixed_chunk<VT>(bt, a, check); }));
   for (;;) {
-    LFG_HIP(hipGraphLaunch(bt->exec, s));
+    if (use_graph) {
+      LFG_HIP(hipGraphLaunch(bt->slot[mode].exec, s));
+    } else {
+      batch_mixed_chunk<VT>(bt, a, check);
+      LFG_HIP(hipGetLastError());
+    }
     LFG_HIP(hipMemcpyAsync(bt->h_mst, bt->mst, bt->nprob * sizeof(BMState),
                            hipMemcpyDeviceToHost, s));
     LFG_HIP(hipStreamSynchronize(s));
@@ -2942,7 +3137,7 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
 // ===========================================================================
 namespace lfgpu {
 int spmv_async(lfgpu_ctx *ctx, lfgpu_csr *A, const double *x, double *y) {
-  LFG_TRY(ensure_cache(ctx, A, false));
+  LFG_TRY(ensure_cache(ctx, A, false, true));
   return launch_spmv(ctx->stream, A->cache, A->nrows, x, y);
 }
 }  // namespace lfgpu
@@ -2953,7 +3148,7 @@ extern "C" int lfgpu_spmv(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *
   LFG_REQUIRE(x->d != y->d, LFGPU_EINVAL, "SpMV input and output must differ");
   lfgpu_csr *A = const_cast<lfgpu_csr *>(Ac);
   LFG_REQUIRE(A->nrows == A->ncols, LFGPU_EINVAL, "SpMV needs a square matrix");
-  LFG_TRY(ensure_cache(ctx, A, false));
+  LFG_TRY(ensure_cache(ctx, A, false, true));
   LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   LFG_TRY(launch_spmv(ctx->stream, A->cache, A->nrows, x->d, y->d));
   LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -2972,7 +3167,7 @@ extern "C" int lfgpu_spmv_timed(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu
               "SpMV size mismatch");
   lfgpu_csr *A = const_cast<lfgpu_csr *>(Ac);
   LFG_REQUIRE(A->nrows == A->ncols, LFGPU_EINVAL, "SpMV needs a square matrix");
-  LFG_TRY(ensure_cache(ctx, A, false));
+  LFG_TRY(ensure_cache(ctx, A, false, true));
   LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   for (int k = 0; k < reps; ++k) LFG_TRY(launch_spmv(ctx->stream, A->cache, A->nrows, x->d, y->d));
   LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -3022,7 +3217,7 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
                   opts->mode == LFGPU_PCG_MIXED_MATRIX,
               LFGPU_EINVAL, "unknown PCG mode");
   const bool mixed = opts->mode != LFGPU_PCG_FP64;
-  LFG_TRY(ensure_cache(ctx, A, mixed));
+  LFG_TRY(ensure_cache(ctx, A, mixed, true));
   lfgpu_solver_cache *c = A->cache;
   hipStream_t s = ctx->stream;
   const int32_t n = A->nrows;
@@ -3065,19 +3260,39 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   const int nwg = (int)((n + kRB - 1) / kRB);
   if (rowsweep && !c->rpart)
     LFG_TRY(dalloc(&c->rpart, (size_t)3 * std::max(nwg, 1), "row-sweep partials"));
+  PcgState *hs = reinterpret_cast<PcgState *>(ctx->pinned);
   PcgState h{};
   h.thr = opts->rtol * opts->rtol;
   h.maxit = opts->maxit;
   LFG_HIP(hipMemcpyAsync(c->st, &h, sizeof(PcgState), hipMemcpyHostToDevice, s));
   LFG_HIP(hipEventRecord(ctx->ev0, s));
+  // storage rows in the locality order: the solve runs on permuted copies of
+  // x and b, and the solution is scattered back before the timing stops
+  double *xd = x->d;
+  const double *bd = b->d;
+  if (c->perm) {
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, x->d, c->perm, n, c->xp);
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n, 256)), dim3(256), 0, s, b->d, c->perm, n, c->bp);
+    LFG_HIP(hipGetLastError());
+    xd = c->xp;
+    bd = c->bp;
+  }
+  auto finish = [&](bool mixed_mode) {
+    if (c->perm) {
+      hipLaunchKernelGGL(k_scatter, dim3(grid_for(n, 256)), dim3(256), 0, s, c->xp, c->perm, n,
+                         x->d);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return hip_fail(e, "PCG solution scatter");
+    }
+    return pcg_finish(ctx, hs, mixed_mode, info);
+  };
 
   const MatView mv = view_of(c, n);
-  PcgState *hs = reinterpret_cast<PcgState *>(ctx->pinned);
   if (single) {
     Cg1Args a1;
     a1.m = mv;
     a1.dinv = c->dinv;
-    a1.x = x->d;
+    a1.x = xd;
     a1.p = c->p;
     for (int q = 0; q < 2; ++q) {
       a1.r[q] = c->cg1 + (size_t)q * n;
@@ -3089,14 +3304,14 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
     a1.sw = sw;
     const dim3 B(kSB64);
     if (c->fmt == FMT_DIA) {
-      hipLaunchKernelGGL((k_cg1_init_r<FMT_DIA, kSB64>), G, B, 0, s, a1, b->d);
+      hipLaunchKernelGGL((k_cg1_init_r<FMT_DIA, kSB64>), G, B, 0, s, a1, bd);
       hipLaunchKernelGGL((k_cg1_init_w<FMT_DIA, kSB64>), G, B, 0, s, a1);
     } else {
-      hipLaunchKernelGGL((k_cg1_init_r<FMT_ELL, kSB64>), G, B, 0, s, a1, b->d);
+      hipLaunchKernelGGL((k_cg1_init_r<FMT_ELL, kSB64>), G, B, 0, s, a1, bd);
       hipLaunchKernelGGL((k_cg1_init_w<FMT_ELL, kSB64>), G, B, 0, s, a1);
     }
     hipLaunchKernelGGL(k_cg1_init_finish<kSB64>, dim3(1), B, 0, s, a1);
-    hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, c->st);
+    hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, xd, n, c->st);
     LFG_HIP(hipGetLastError());
     for (;;) {
       if (c->fmt == FMT_DIA)
@@ -3108,27 +3323,27 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
       LFG_HIP(hipStreamSynchronize(s));
       if (hs->done) break;
     }
-    return pcg_finish(ctx, hs, false, info);
+    return finish(false);
   }
   if (rowsweep && mixed) {
     const double d2 = opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
     const int mi = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
     if (opts->mode == LFGPU_PCG_MIXED) {
-      MixArgs<float> m{mv, c->dinv32, c->dinv, x->d, c->xlo, b->d, c->r, c->r32, c->p32,
+      MixArgs<float> m{mv, c->dinv32, c->dinv, xd, c->xlo, bd, c->r, c->r32, c->p32,
                        c->q32, c->xa32, c->part, c->st, sw, d2, mi, c->rpart, nwg};
       LFG_TRY(run_mixed_rows(s, c, m, n, check, hs));
     } else {
-      MixArgs<double> m{mv, c->dinv32, c->dinv, x->d, c->xlo, b->d, c->r64, c->r, c->p,
+      MixArgs<double> m{mv, c->dinv32, c->dinv, xd, c->xlo, bd, c->r64, c->r, c->p,
                         c->q, c->xa64, c->part, c->st, sw, d2, mi, c->rpart, nwg};
       LFG_TRY(run_mixed_rows(s, c, m, n, check, hs));
     }
-    return pcg_finish(ctx, hs, true, info);
+    return finish(true);
   }
   if (rowsweep) {
-    const RowArgs ra{mv, c->dinv, x->d, c->r, c->p, c->q, c->rpart, nwg, c->st};
-    hipLaunchKernelGGL(k_cgr_init<FMT_ELL>, dim3(nwg), dim3(kRB), 0, s, ra, b->d);
+    const RowArgs ra{mv, c->dinv, xd, c->r, c->p, c->q, c->rpart, nwg, c->st};
+    hipLaunchKernelGGL(k_cgr_init<FMT_ELL>, dim3(nwg), dim3(kRB), 0, s, ra, bd);
     hipLaunchKernelGGL(k_cgr_red_init, dim3(1), dim3(kRedB), 0, s, ra);
-    hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, c->st);
+    hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, xd, n, c->st);
     LFG_HIP(hipGetLastError());
     for (;;) {
       LFG_TRY(run_fp64_chunk(c, s, CG_ROWS_ELL, CgArgs{}, ra, check, false, use_graph));
@@ -3136,41 +3351,41 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
       LFG_HIP(hipStreamSynchronize(s));
       if (hs->done) break;
     }
-    return pcg_finish(ctx, hs, false, info);
+    return finish(false);
   }
   // the setup sweep leaves r = b - A x in the buffer the chosen mode reads
   // its true residual from
   double *r_init = opts->mode == LFGPU_PCG_MIXED_MATRIX ? c->r64 : c->r;
-  CgArgs a{mv, c->dinv, x->d, r_init, c->p, c->q, c->part, c->st, sw};
+  CgArgs a{mv, c->dinv, xd, r_init, c->p, c->q, c->part, c->st, sw};
   if (sb == kSBMix) {
     const dim3 B(kSBMix);
     if (c->fmt == FMT_DIA)
-      hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSBMix>), G, B, 0, s, a, b->d);
+      hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSBMix>), G, B, 0, s, a, bd);
     else
-      hipLaunchKernelGGL((k_cg_init<FMT_ELL, kSBMix>), G, B, 0, s, a, b->d);
+      hipLaunchKernelGGL((k_cg_init<FMT_ELL, kSBMix>), G, B, 0, s, a, bd);
     hipLaunchKernelGGL(k_cg_init_finish<kSBMix>, dim3(1), B, 0, s, a);
   } else {
     const dim3 B(kSB64);
     if (c->fmt == FMT_DIA)
-      hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSB64>), G, B, 0, s, a, b->d);
+      hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSB64>), G, B, 0, s, a, bd);
     else
-      hipLaunchKernelGGL((k_cg_init<FMT_ELL, kSB64>), G, B, 0, s, a, b->d);
+      hipLaunchKernelGGL((k_cg_init<FMT_ELL, kSB64>), G, B, 0, s, a, bd);
     hipLaunchKernelGGL(k_cg_init_finish<kSB64>, dim3(1), B, 0, s, a);
   }
-  hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, c->st);
+  hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, xd, n, c->st);
   LFG_HIP(hipGetLastError());
 
   const double delta2 =
       opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
   const int max_interval = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
   if (opts->mode == LFGPU_PCG_MIXED) {
-    MixArgs<float> m{mv, c->dinv32, c->dinv, x->d, c->xlo, b->d, c->r, c->r32, c->p32, c->q32,
+    MixArgs<float> m{mv, c->dinv32, c->dinv, xd, c->xlo, bd, c->r, c->r32, c->p32, c->q32,
                      c->xa32, c->part, c->st, sw, delta2, max_interval};
     LFG_TRY(run_mixed(s, c, sw, m, check, hs));
   } else if (opts->mode == LFGPU_PCG_MIXED_MATRIX) {
     // FP64 vectors: p, q and the increment accumulator reuse the FP64
     // work vectors; r64 needs its own buffer next to the recurrence r.
-    MixArgs<double> m{mv, c->dinv32, c->dinv, x->d, c->xlo, b->d, c->r64, c->r, c->p, c->q,
+    MixArgs<double> m{mv, c->dinv32, c->dinv, xd, c->xlo, bd, c->r64, c->r, c->p, c->q,
                       c->xa64, c->part, c->st, sw, delta2, max_interval};
     LFG_TRY(run_mixed(s, c, sw, m, check, hs));
   } else {
@@ -3185,7 +3400,7 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
       if (hs->done) break;
     }
   }
-  return pcg_finish(ctx, hs, mixed, info);
+  return finish(mixed);
 }
 
 extern "C" int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
@@ -3207,6 +3422,32 @@ extern "C" int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
     hipError_t e = hipHostMalloc(&bt->h_st, nprob * sizeof(BState), hipHostMallocDefault);
     if (e != hipSuccess) rc = hip_fail(e, "pinned batch states");
   }
+  if (rc == LFGPU_OK) rc = dalloc(&bt->d_row_off, (size_t)nprob + 1, "batch row offsets");
+  // every problem is solved on its own rows only: refuse entries that couple
+  // two problems
+  int *flag = nullptr;
+  if (rc == LFGPU_OK) rc = dalloc(&flag, 1, "batch block check flag");
+  if (rc == LFGPU_OK) {
+    hipStream_t s = ctx->stream;
+    int h = 1;
+    hipError_t e = hipMemcpyAsync(bt->d_row_off, bt->row_off.data(), ((size_t)nprob + 1) * 4,
+                                  hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(flag, 0, 4, s);
+    if (e == hipSuccess && A->nrows > 0) {
+      hipLaunchKernelGGL(k_batch_blockcheck, dim3(grid_for(A->nrows, 256)), dim3(256), 0, s,
+                         A->row_ptr, A->col, A->nrows, bt->d_row_off, nprob, flag);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = hip_fail(e, "batch block check");
+    else if (h != 0) {
+      set_error("lfgpu_batch_create: the matrix couples rows of different problems (it is "
+                "not block diagonal over row_offsets)");
+      rc = LFGPU_EINVAL;
+    }
+  }
+  dfree(flag);
   if (rc != LFGPU_OK) {
     lfgpu_batch_destroy(bt);
     return rc;
@@ -3254,7 +3495,7 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
               LFGPU_EINVAL, "unknown PCG mode");
   LFG_REQUIRE(opts->rtol >= 0.0 && opts->maxit >= 0, LFGPU_EINVAL, "bad PCG options");
   const bool mixed = opts->mode != LFGPU_PCG_FP64;
-  LFG_TRY(ensure_cache(ctx, bt->A, mixed));
+  LFG_TRY(ensure_cache(ctx, bt->A, mixed, false));
   hipStream_t s = ctx->stream;
   if (bt->fmt < 0) LFG_TRY(batch_choose_format(bt));
   const float *before32 = bt->bval32;
@@ -3278,21 +3519,20 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
     const BArgs a = batch_args(bt, x->d);
     // One chunk of iterations is captured into a graph once and replayed
     // until every problem has stopped; the captured kernel arguments depend
-    // on the batch layout and on x only.
-    if (!bt->exec || bt->graph_count != check || bt->graph_x != x->d ||
-        bt->graph_mode != LFGPU_PCG_FP64) {
-      batch_release_graph(bt);
-      LFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      batch_chunk(bt, a, check);
-      hipError_t e = hipStreamEndCapture(s, &bt->graph);
-      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-      LFG_HIP(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0));
-      bt->graph_count = check;
-      bt->graph_x = x->d;
-      bt->graph_mode = LFGPU_PCG_FP64;
-    }
+    // on the batch layout, on x and on the product kernel, which reads the
+    // upper diagonals only while the values are bitwise symmetric (bsym is
+    // recomputed whenever the values change)
+    const bool use_graph = opts->graph >= 0;
+    if (use_graph)
+      LFG_TRY(batch_capture(bt, LFGPU_PCG_FP64, check, x->d, nullptr, 0.0, 0, bt->bsym,
+                            [&]() { batch_chunk(bt, a, check); }));
     for (;;) {
-      LFG_HIP(hipGraphLaunch(bt->exec, s));
+      if (use_graph) {
+        LFG_HIP(hipGraphLaunch(bt->slot[LFGPU_PCG_FP64].exec, s));
+      } else {
+        batch_chunk(bt, a, check);
+        LFG_HIP(hipGetLastError());
+      }
       LFG_HIP(hipMemcpyAsync(bt->h_st, bt->st, bt->nprob * sizeof(BState),
                              hipMemcpyDeviceToHost, s));
       LFG_HIP(hipStreamSynchronize(s));
@@ -3342,6 +3582,58 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
     return LFGPU_ENOCONV;
   }
   return LFGPU_OK;
+}
+
+extern "C" int lfgpu_solver_storage(const lfgpu_csr *A, int *storage, int *ndiag, int *sym) {
+  LFG_REQUIRE(A && storage, LFGPU_EINVAL, "NULL argument to lfgpu_solver_storage");
+  const lfgpu_solver_cache *c = A->cache;
+  *storage = !c ? -1 : c->fmt == FMT_DIA ? LFGPU_STORAGE_DIA : LFGPU_STORAGE_ELL;
+  if (ndiag) *ndiag = c ? c->ndiag : 0;
+  if (sym) *sym = c && c->sym64 ? 1 : 0;
+  return LFGPU_OK;
+}
+
+extern "C" int lfgpu_pcg_sweep_times(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *b,
+                                     lfgpu_vec *x, const lfgpu_pcg_opts *opts, int max_kernels,
+                                     double *out_ms, lfgpu_pcg_info *info) {
+  LFG_REQUIRE(ctx && A && b && x && opts && out_ms && max_kernels > 0, LFGPU_EINVAL,
+              "bad lfgpu_pcg_sweep_times arguments");
+  SweepTimer t;
+  t.ev.assign(2 * (size_t)max_kernels, nullptr);
+  t.kind.assign((size_t)max_kernels, -1);
+  hipError_t e = hipSuccess;
+  for (size_t k = 0; k < t.ev.size() && e == hipSuccess; ++k) e = hipEventCreate(&t.ev[k]);
+  auto destroy = [&]() {
+    for (hipEvent_t ev : t.ev)
+      if (ev) (void)hipEventDestroy(ev);
+  };
+  if (e != hipSuccess) {
+    destroy();
+    return hip_fail(e, "sweep timer events");
+  }
+  lfgpu_pcg_opts o = *opts;
+  o.graph = -1;  // plain launches: every kernel between its own events
+  t_timer = &t;
+  const int rc = lfgpu_pcg(ctx, A, b, x, &o, info);
+  t_timer = nullptr;
+  e = hipStreamSynchronize(ctx->stream);
+  std::vector<double> d[3];
+  for (size_t k = 0; k < t.used && e == hipSuccess && t.err == hipSuccess; ++k) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]);
+    if (t.kind[k] >= 0 && t.kind[k] < 3) d[t.kind[k]].push_back(ms);
+  }
+  destroy();
+  if (e == hipSuccess) e = t.err;
+  if (e != hipSuccess) return hip_fail(e, "sweep timing");
+  for (int q = 0; q < 3; ++q) {
+    out_ms[q] = 0.0;
+    if (d[q].empty()) continue;
+    std::sort(d[q].begin(), d[q].end());
+    const size_t m = d[q].size();
+    out_ms[q] = m % 2 ? d[q][m / 2] : 0.5 * (d[q][m / 2 - 1] + d[q][m / 2]);
+  }
+  return rc;
 }
 
 namespace lfgpu {

@@ -577,7 +577,7 @@ namespace solve {
 inline lfgpu_pcg_info PCG(const Context &ctx, const assemble::CsrMatrix &A, const DeviceVector &b,
                           DeviceVector &x, double rtol = 1e-10, int maxit = 0,
                           int mode = LFGPU_PCG_FP64) {
-  lfgpu_pcg_opts o{rtol, maxit > 0 ? maxit : 2 * A.rows(), mode, 64, 0.0, 0, 0};
+  lfgpu_pcg_opts o{rtol, maxit > 0 ? maxit : 2 * A.rows(), mode, 64, 0.0, 0, 0, 0};
   lfgpu_pcg_info info{};
   check(lfgpu_pcg(ctx.get(), A.get(), b.get(), x.get(), &o, &info));
   return info;

@@ -87,6 +87,62 @@ def test_c2_full_size_properties(ctx):
         h.close()
 
 
+def _c3_system(ctx, n=512):
+    """The jittered 512 x 512 P3 system of the benchmark (a = exp(N(0,1)) per
+    cell, gamma = 1, f = sin sin), assembled on the matrix cores."""
+    xy, cells, edges = capi.tp_triag_mesh(n)
+    rng = np.random.default_rng(0)
+    h = 1.0 / n
+    inner = (xy[:, 0] > 0) & (xy[:, 0] < 1) & (xy[:, 1] > 0) & (xy[:, 1] < 1)
+    xy = xy.copy()
+    xy[inner] += rng.uniform(-0.2 * h, 0.2 * h, size=(int(inner.sum()), 2))
+    ev, ce, co = capi.mesh_edges(xy.shape[0], cells, edges)
+    cd, nd = capi.dof_table(xy.shape[0], cells, len(ev), ce, co, 3)
+    nv = xy.shape[0]
+    vb = ~inner
+    eb = vb[ev[:, 0]] & vb[ev[:, 1]] & (
+        (xy[ev[:, 0], 0] == xy[ev[:, 1], 0]) | (xy[ev[:, 0], 1] == xy[ev[:, 1], 1]))
+    fixed = np.zeros(nd, np.uint8)
+    fixed[:nv] = vb
+    be = np.nonzero(eb)[0]
+    fixed[nv + 2 * be] = 1
+    fixed[nv + 2 * be + 1] = 1
+    mesh = capi.Mesh(ctx, xy, cells, cd, nd)
+    fe = capi.FE(ctx, 3)
+    plan = capi.Plan(ctx, mesh, fixed)
+    theta = capi.Vec(ctx, np.random.default_rng(1).standard_normal(cells.shape[0]))
+    b = capi.Vec(ctx, nd)
+    plan.assemble(fe, capi.coeff(capi.COEFF_EXPCELL, 1.0, theta), capi.coeff(capi.COEFF_CONST, 1.0),
+                  capi.coeff(capi.COEFF_SINSIN, 1.0), b, mfma=True)
+    return nd, plan, b, (theta, fe, mesh)
+
+
+def test_c3_full_size_pcg_modes(ctx):
+    """The full-size P3 system (2,362,369 DOFs) in every precision mode:
+    each converges (the mixed modes on their FP64 true residual), the
+    solutions agree with one another to 1e-8 (rtol 1e-10 times the
+    conditioning of the smooth modes), and the FP64 residual of the mixed
+    solutions, computed by the device product, is below 2e-10."""
+    nd, plan, b, keep = _c3_system(ctx)
+    x, y = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
+    sols = {}
+    for name, md in (("fp64", capi.PCG_FP64), ("mixed", capi.PCG_MIXED),
+                     ("mixed_matrix", capi.PCG_MIXED_MATRIX)):
+        x.fill(0.0)
+        info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, mode=md, maxit=60000, check_every=128)
+        assert info.converged, name
+        sols[name] = x.numpy()
+        if md != capi.PCG_FP64:
+            capi.spmv(ctx, plan.csr, x, y)
+            bh = b.numpy()
+            assert np.linalg.norm(bh - y.numpy()) <= 2e-10 * np.linalg.norm(bh), name
+    for name in ("mixed", "mixed_matrix"):
+        err = np.linalg.norm(sols[name] - sols["fp64"]) / np.linalg.norm(sols["fp64"])
+        assert err <= 1e-8, (name, err)
+    for h in (y, x, b, plan, *keep):
+        h.close()
+
+
 def test_c3_full_size_row_sums(ctx):
     n = 512
     xy, cells, edges = capi.tp_triag_mesh(n)
@@ -203,3 +259,44 @@ def test_c5_full_size_adjoint_gradient(ctx):
         fd = (obj.value(theta + eps * dv)[0] - obj.value(theta - eps * dv)[0]) / (2 * eps)
         assert abs(fd - g @ dv) <= 1e-5 * abs(fd), (fd, g @ dv)
     obj.close()
+
+
+def test_c5_full_size_fd_64_local_directions(ctx):
+    """dJ/dtheta on the 1024 x 1024 mesh against central differences on 64
+    random unit directions, each supported on 8 cells drawn from the 5 % of
+    cells with the largest |dJ/dtheta_e| (seeded).  A direction spread over all
+    2M cells has g.d ~ |g| / 1400, which the solves' noise in J / (2 eps)
+    swamps; on a few cells g.d is of the size of single gradient entries.
+    eps = 1e-4 (J is smooth in theta: the O(eps^2) term is ~1e-8 relative),
+    solves to 1e-13; every direction must agree to 1e-4 relative."""
+    from lehrfempp_amd.autograd import AdjointObjective
+    n = 1024
+    xy, cells, _ = capi.tp_triag_mesh(n)
+    nv, ne = xy.shape[0], cells.shape[0]
+    j, i = np.divmod(np.arange(nv), n + 1)
+    flags = ((i == 0) | (i == n) | (j == 0) | (j == n)).astype(np.uint8)
+    theta = np.random.default_rng(1).standard_normal(ne)
+    theta_star = np.random.default_rng(2).standard_normal(ne)
+    obj = AdjointObjective(ctx, xy, cells, flags, np.zeros(nv), rtol=1e-13)
+    _, ustar = obj.value(theta_star)
+    obj.close()
+    noise = np.random.default_rng(3).standard_normal(nv) * (1 - flags)
+    obj = AdjointObjective(ctx, xy, cells, flags, ustar + 0.01 * np.abs(ustar).max() * noise,
+                           rtol=1e-13)
+    obj.value(theta)
+    g = obj.gradient(theta)
+    top = np.argsort(-np.abs(g))[: ne // 20]
+    rng = np.random.default_rng(11)
+    eps = 1e-4
+    errs = []
+    for _ in range(64):
+        cells_d = rng.choice(top, 8, replace=False)
+        dv = np.zeros(ne)
+        dv[cells_d] = rng.standard_normal(8)
+        dv /= np.linalg.norm(dv)
+        fd = (obj.value(theta + eps * dv)[0] - obj.value(theta - eps * dv)[0]) / (2 * eps)
+        errs.append(abs(fd - g @ dv) / abs(g @ dv))
+    obj.close()
+    print("fd 64 local directions: max rel err %.2e, median %.2e" % (max(errs),
+                                                                     float(np.median(errs))))
+    assert max(errs) <= 1e-4, max(errs)

@@ -688,6 +688,115 @@ def test_batched_block_dia_storage_gives_the_ell_iterates(ctx, mode):
             h.close()
 
 
+def test_batched_fp64_symmetry_change_between_solves(ctx):
+    # one batch, one solution vector, the same chunk length: the values first
+    # bitwise symmetric (alpha = 1 on a jittered mesh: G_i . G_j commutes), so
+    # the FP64 product reads only the upper diagonals, then asymmetric
+    # (alpha per cell: the products round differently) -- the second solve
+    # must not replay the first one's captured upper-diagonal product.  Each
+    # solve is checked bit for bit against a batch created for that matrix.
+    sizes = [12, 20, 7, 16]
+    xy, cells, flags, voff, coff = capi.batched_tp_mesh(sizes)
+    rng = np.random.default_rng(41)
+    xy = xy.copy()
+    for k, n in enumerate(sizes):
+        lo, hi = int(voff[k]), int(voff[k + 1])
+        v = xy[lo:hi]
+        inner = (v[:, 0] > 0) & (v[:, 0] < 1) & (v[:, 1] > 0) & (v[:, 1] < 1)
+        v[inner] += rng.uniform(-0.2 / n, 0.2 / n, (int(inner.sum()), 2))
+    g, fe = capi.Mesh(ctx, xy, cells), capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, flags)
+    va = capi.Vec(ctx, np.exp(rng.standard_normal(cells.shape[0])))
+    b, x, xf = capi.Vec(ctx, g.ndofs), capi.Vec(ctx, g.ndofs), capi.Vec(ctx, g.ndofs)
+    batch = capi.Batch(ctx, plan.csr, voff)
+    for alpha, symmetric in ((capi.coeff(capi.COEFF_CONST, 1.0), True),
+                             (capi.coeff(capi.COEFF_CELL, 1.0, va), False)):
+        plan.assemble(fe, alpha, capi.coeff(capi.COEFF_CONST, 1.0),
+                      capi.coeff(capi.COEFF_CONST, 1.0), b)
+        rp, col, val = plan.csr.download()
+        assert _bitwise_symmetric(rp, col, val) == symmetric
+        for _ in range(2):
+            x.fill(0.0)
+            summ, infos = batch.pcg(b, x, rtol=1e-10)
+            assert summ.converged
+        assert batch.storage() == capi.STORAGE_BLOCK_DIA
+        fresh = capi.Batch(ctx, plan.csr, voff)
+        xf.fill(0.0)
+        summ_f, infos_f = fresh.pcg(b, xf, rtol=1e-10)
+        fresh.close()
+        assert [i.iters for i in infos] == [i.iters for i in infos_f], symmetric
+        assert np.array_equal(x.numpy(), xf.numpy()), symmetric
+    for h in (batch, xf, x, b, va, plan, fe, g):
+        h.close()
+
+
+def test_batched_rejects_coupled_problems(ctx):
+    # a matrix with an entry coupling problem 0 and problem 1 is refused;
+    # the same matrix without it is accepted
+    n = 6
+    rows = np.arange(n, dtype=np.int32)
+    for coupled in (True, False):
+        r, c, v = list(rows), list(rows), [2.0] * n
+        if coupled:
+            r += [2, 3]
+            c += [3, 2]
+            v += [-1.0, -1.0]
+        A = capi.coo_to_csr(ctx, np.array(r, np.int32), np.array(c, np.int32), np.array(v),
+                            n, n)
+        if coupled:
+            with pytest.raises(capi.LfgpuError) as e:
+                capi.Batch(ctx, A, np.array([0, 3, 6], np.int32))
+            assert e.value.code == capi.EINVAL
+        else:
+            capi.Batch(ctx, A, np.array([0, 3, 6], np.int32)).close()
+        A.close()
+
+
+@pytest.mark.parametrize("degree", [1, 3])
+def test_ell_locality_order_single_and_batched_solves(ctx, degree):
+    # sliced-ELL storage of a single matrix keeps its rows in the locality
+    # order (by smallest column) and solves on permuted copies of x and b;
+    # a batch on the same matrix needs the original row order and rebuilds
+    # the storage; a single solve afterwards runs on that storage.  All
+    # solutions and products must match the oracle.
+    m = delaunay_mesh(500, 20 + degree) if degree == 1 else jittered_mesh(12)
+    cd, nd = m.dofs(degree)
+    g = gpu_mesh(ctx, m, degree)
+    fe = capi.FE(ctx, degree)
+    plan = capi.Plan(ctx, g, m.dirichlet_flags(degree))
+    va = capi.Vec(ctx, np.exp(np.random.default_rng(degree).standard_normal(m.ne)))
+    b = capi.Vec(ctx, nd)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 1.0),
+                  capi.coeff(capi.COEFF_SINSIN, 1.0), b)
+    rp, col, val = plan.csr.download()
+    bh = b.numpy()
+    xd = L.band_cholesky_solve(rp, col, val, bh)
+    xv = np.random.default_rng(3).standard_normal(nd)
+    x, y = capi.Vec(ctx, xv), capi.Vec(ctx, nd)
+    sols = []
+    for stage in ("single", "batched", "single again"):
+        if stage == "batched":
+            batch = capi.Batch(ctx, plan.csr, np.array([0, nd], np.int32))
+            x.fill(0.0)
+            summ, _ = batch.pcg(b, x, rtol=1e-12)
+            assert summ.converged and batch.storage() == capi.STORAGE_ELL
+            batch.close()
+        else:
+            x.fill(0.0)
+            info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-12)
+            assert info.converged
+            assert capi.solver_storage(plan.csr)[0] == capi.STORAGE_ELL
+        sols.append(x.numpy())
+        assert np.linalg.norm(sols[-1] - xd) <= 1e-8 * np.linalg.norm(xd), stage
+        x.upload(xv)
+        capi.spmv(ctx, plan.csr, x, y)
+        yo = L.csr_spmv(rp, col, val, xv)
+        scale = L.csr_spmv(rp, col, np.abs(val), np.abs(xv))
+        assert np.all(np.abs(y.numpy() - yo) <= 1e-14 * scale + 1e-300), stage
+    for h in (y, x, b, va, plan, fe, g):
+        h.close()
+
+
 # ---------------------------------------------- adjoint gradient end to end
 def test_adjoint_gradient_against_finite_differences(ctx):
     # J(theta) = 1/2 (u - u_obs)^T M (u - u_obs), A(exp(theta)) u = f;
