@@ -46,17 +46,34 @@ def _compile(src: str) -> str:
 
 DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "homdir_demo.cc")
 DEMO_BIN = os.path.join(ROOT, "tests", "cpp", "homdir_demo")
+BINDING_SRC = os.path.join(ROOT, "tests", "cpp", "binding_demo.cc")
+BINDING_BIN = os.path.join(ROOT, "tests", "cpp", "binding_demo")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+
+def binding_demo_cmd(out: str) -> list:
+    """g++ line of the LehrFEM++-binding driver (tests/cpp/binding_demo.cc):
+    the binding header against the archetype of the reference interfaces,
+    linked with liblfgpu.so and, as its checker, the oracle."""
+    return ["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", "-I", ROOT, "-I",
+            os.path.join(ROOT, "include"), BINDING_SRC, "-L", HERE, "-llfgpu", "-L", ORACLE_DIR,
+            "-llfref", "-Wl,-rpath,$ORIGIN/../../lehrfempp_amd:$ORIGIN/../../oracle", "-o", out]
 
 
 def build_cpp_demo() -> str:
-    """Compile the C++ facade driver (tests/cpp/homdir_demo.cc) with g++
-    against liblfgpu.so."""
+    """Compile the C++ drivers with g++ against liblfgpu.so: the facade
+    driver (tests/cpp/homdir_demo.cc) and, when the oracle library is built,
+    the LehrFEM++-binding driver (tests/cpp/binding_demo.cc)."""
     cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-I", ROOT, "-I", os.path.join(ROOT, "include"),
            DEMO_SRC, "-L", HERE, "-llfgpu", "-Wl,-rpath,$ORIGIN/../../lehrfempp_amd",
            "-o", DEMO_BIN]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("g++ failed for the C++ facade driver:\n%s" % res.stderr[-8000:])
+    if os.path.exists(os.path.join(ORACLE_DIR, "liblfref.so")):
+        res = subprocess.run(binding_demo_cmd(BINDING_BIN), capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("g++ failed for the binding driver:\n%s" % res.stderr[-8000:])
     return DEMO_BIN
 
 
