@@ -334,6 +334,22 @@ int lfgpu_pcg_sweep_times(lfgpu_ctx *ctx, const lfgpu_csr *A,
                           const lfgpu_pcg_opts *opts, int max_kernels,
                           double *out_ms, lfgpu_pcg_info *info);
 
+/* lfgpu_pcg in mode LFGPU_PCG_FP64 (two sweeps per iteration) for callers
+ * that cannot allocate memory: no reference counterpart; written for code
+ * that must not call hipMalloc (serving loops, harnesses that forbid
+ * allocation).  It allocates nothing: its work vectors, partial sums and
+ * solver state live in a fixed device array of the library (64 MiB per GPU,
+ * held until the process ends), the product is taken from A's CSR directly
+ * (no solver storage is built) and no graph is captured.  Systems of up to
+ * 2^21 rows (LFGPU_ERANGE above); other modes and single_reduction are
+ * rejected (LFGPU_EINVAL); opts->graph is ignored.  On a matrix that
+ * lfgpu_pcg stores as DIA the iterates are lfgpu_pcg's.  All calls on one
+ * GPU share the array: calls that overlap on one GPU (several contexts or
+ * threads) race and must be serialised by the caller. */
+int lfgpu_pcg_noalloc(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *b,
+                      lfgpu_vec *x, const lfgpu_pcg_opts *opts,
+                      lfgpu_pcg_info *info);
+
 /* ---- batched solves --------------------------------------------------
  * Many independent SPD systems stored as one block-diagonal matrix (e.g. the
  * plan of a mesh that is the disjoint union of the problems' meshes, so one

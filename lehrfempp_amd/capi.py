@@ -104,6 +104,7 @@ SIGNATURES = {
     "lfgpu_spmv": (C.c_int, [_P, _P, _P, _P]),
     "lfgpu_spmv_timed": (C.c_int, [_P, _P, _P, _P, C.c_int, _d]),
     "lfgpu_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo)]),
+    "lfgpu_pcg_noalloc": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo)]),
     "lfgpu_solver_storage": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                         C.POINTER(C.c_int)]),
     "lfgpu_pcg_sweep_times": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.c_int,
@@ -466,6 +467,20 @@ def pcg_sweep_times(ctx, A: Csr, b: Vec, x: Vec, iters=200, mode=PCG_FP64, max_k
     if rc != OK and rc != ENOCONV:
         check(rc)
     return list(out), info
+
+
+def pcg_noalloc(ctx, A: Csr, b: Vec, x: Vec, rtol=1e-10, maxit=None, check_every=64,
+                raise_on_noconv=True):
+    """lfgpu_pcg_noalloc: the FP64 two-sweep PCG without any device allocation
+    (fixed per-GPU work space, systems of up to 2^21 rows)."""
+    n = A.info()[0]
+    opts = PcgOpts(rtol, int(2 * n if maxit is None else maxit), PCG_FP64, check_every,
+                   0.0, 0, 0, 0)
+    info = PcgInfo()
+    rc = lib().lfgpu_pcg_noalloc(ctx.h, A.h, b.h, x.h, C.byref(opts), C.byref(info))
+    if rc != OK and (rc != ENOCONV or raise_on_noconv):
+        check(rc)
+    return info
 
 
 class Batch:

@@ -148,6 +148,7 @@ extern "C" int lfgpu_ctx_create(int device, lfgpu_ctx **out) {
   LFG_HIP(hipGetDeviceCount(&ndev));
   LFG_REQUIRE(device >= 0 && device < ndev, LFGPU_EINVAL, "device index out of range");
   LFG_HIP(hipSetDevice(device));
+  LFG_TRY(lfgpu::noalloc_scratch_init(device));
   lfgpu_ctx *c = new lfgpu_ctx();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -167,4 +167,7 @@ inline void csr_values_changed(lfgpu_csr *A) { A->val_version++; }
 // y = A x (FP64) on the context stream, without synchronisation; builds the
 // solver's storage of A on first use (lfgpu_solver.hip).
 int spmv_async(lfgpu_ctx *ctx, lfgpu_csr *A, const double *x, double *y);
+// Looks up this device's address of the allocation-free solver's work space
+// (lfgpu_solver.hip); called by lfgpu_ctx_create with the device current.
+int noalloc_scratch_init(int device);
 }  // namespace lfgpu

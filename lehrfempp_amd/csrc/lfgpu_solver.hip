@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -41,6 +42,9 @@ constexpr int FMT_ELL = 0, FMT_DIA = 1;
 // batches only: DIA storage whose offsets are chosen per problem (a block
 // diagonal matrix of structured meshes of different sizes)
 constexpr int FMT_BDIA = 2;
+// the CSR itself, no mirror (the allocation-free solve): the difference form
+// with the row sum accumulated during the product, in the mirror's order
+constexpr int FMT_CSR = 3;
 
 }  // namespace
 
@@ -142,10 +146,14 @@ struct MatView {
   const float *__restrict__ val32;
   const double *__restrict__ rowsum;
   const float *__restrict__ rowsum32;
+  const int32_t *__restrict__ row_ptr;  // FMT_CSR
+  const int32_t *__restrict__ col;      // FMT_CSR
 };
 
 MatView view_of(const lfgpu_solver_cache *c, int32_t n) {
   MatView m;
+  m.row_ptr = nullptr;
+  m.col = nullptr;
   m.n = n;
   m.lo = 0;
   m.ndiag = c->ndiag;
@@ -173,7 +181,21 @@ template <int FMT, bool HOIST = false, bool SYM = false, bool BLOCK = false>
 __device__ __forceinline__ double apply64(const MatView &m, const double *__restrict__ x,
                                           int32_t r, double xr) {
   double acc = 0.0;
-  if constexpr (FMT == FMT_DIA && HOIST) {
+  if constexpr (FMT == FMT_CSR) {
+    // off-diagonal terms in column order and the Neumaier-compensated row
+    // sum of all entries in CSR order: the same sums as the mirror's
+    // product (k_fill_vals and the DIA / ELL branches below)
+    double sum = 0.0, comp = 0.0;
+    for (int32_t k = m.row_ptr[r]; k < m.row_ptr[r + 1]; ++k) {
+      const double v = m.val[k];
+      const double t = sum + v;
+      comp += fabs(sum) >= fabs(v) ? (sum - t) + v : (v - t) + sum;
+      sum = t;
+      const int32_t c = m.col[k];
+      if (c != r) acc += v * (x[c] - xr);
+    }
+    return acc + (sum + comp) * xr;
+  } else if constexpr (FMT == FMT_DIA && HOIST) {
     double v[kMaxDiag], xc[kMaxDiag];
 #pragma unroll
     for (int d = 0; d < kMaxDiag; ++d) {
@@ -1971,7 +1993,7 @@ struct CgGraph {
 bool same_view(const MatView &x, const MatView &y) {
   if (x.n != y.n || x.lo != y.lo || x.ndiag != y.ndiag || x.slice_off != y.slice_off ||
       x.s_col != y.s_col || x.val != y.val || x.val32 != y.val32 || x.rowsum != y.rowsum ||
-      x.rowsum32 != y.rowsum32)
+      x.rowsum32 != y.rowsum32 || x.row_ptr != y.row_ptr || x.col != y.col)
     return false;
   for (int d = 0; d < kMaxDiag; ++d)
     if (x.off[d] != y.off[d]) return false;
@@ -3634,6 +3656,127 @@ extern "C" int lfgpu_pcg_sweep_times(lfgpu_ctx *ctx, const lfgpu_csr *A, const l
     out_ms[q] = m % 2 ? d[q][m / 2] : 0.5 * (d[q][m / 2 - 1] + d[q][m / 2]);
   }
   return rc;
+}
+
+// ===========================================================================
+// Allocation-free FP64 Jacobi-PCG (lfgpu_pcg_noalloc).  The work vectors,
+// the partial sums and the solver state live in one file-scope device array,
+// `scratch`, of fixed size; the product is taken from the CSR itself
+// (FMT_CSR), so no solver mirror is built.  The kernels are those of the
+// FP64 two-sweep solver with the 256-thread workgroup split lfgpu_pcg uses
+// below 2^21 rows, so on a DIA-eligible matrix the iterates equal
+// lfgpu_pcg's FP64 iterates.  Each GPU holds its own copy of the array for
+// the life of the process; its device address is looked up once per device
+// (lfgpu_ctx_create) and used instead of the array's name in host code.
+// ===========================================================================
+namespace {
+
+constexpr int32_t kNoAllocMaxRows = 1 << 21;
+constexpr int64_t kNoAllocVec = kNoAllocMaxRows;  // doubles per work vector
+constexpr int64_t kNoAllocPart = (int64_t)kPartSlots * kMaxGrid;
+constexpr int64_t kNoAllocState = 32;  // doubles reserved for the PcgState
+static_assert(sizeof(PcgState) <= kNoAllocState * sizeof(double), "PcgState too large");
+constexpr int64_t kNoAllocDoubles = 4 * kNoAllocVec + kNoAllocPart + kNoAllocState;
+constexpr int kNoAllocMaxDevices = 64;
+
+// dinv | r | p | q | partials | state  (64 MiB + 128 KiB + 256 B per GPU)
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+double *g_scratch_ptr[kNoAllocMaxDevices] = {nullptr};
+
+// Jacobi inverse diagonal from the CSR, as k_fill_vals: 1 / a_ii, or 1 where
+// the diagonal entry is missing or zero (Eigen's DiagonalPreconditioner)
+__global__ void k_csr_dinv(const int32_t *row_ptr, const int32_t *col, const double *val,
+                           int32_t n, double *dinv) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double d = 1.0;
+  for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+    if (col[k] == r && val[k] != 0.0) d = 1.0 / val[k];
+  dinv[r] = d;
+}
+
+// The solver state as lfgpu_pcg uploads it (thr = rtol^2, maxit); the array
+// keeps the previous call's contents, so every field is reset here.
+__global__ void k_noalloc_state(PcgState *st, double thr, int maxit) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *st = PcgState{};
+    st->thr = thr;
+    st->maxit = maxit;
+  }
+}
+
+}  // namespace
+
+namespace lfgpu {
+// Called by lfgpu_ctx_create with `device` current: records the device
+// address of this GPU's copy of `scratch` (once per device).
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return LFGPU_OK;  // entry point refuses
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (g_scratch_ptr[device]) return LFGPU_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr[device] = static_cast<double *>(ptr);
+  return LFGPU_OK;
+}
+}  // namespace lfgpu
+
+extern "C" int lfgpu_pcg_noalloc(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *b,
+                                 lfgpu_vec *x, const lfgpu_pcg_opts *opts,
+                                 lfgpu_pcg_info *info) {
+  LFG_REQUIRE(ctx && A && b && x && opts, LFGPU_EINVAL, "NULL argument to lfgpu_pcg_noalloc");
+  LFG_REQUIRE(A->nrows == A->ncols && b->n == A->nrows && x->n == A->nrows, LFGPU_EINVAL,
+              "PCG size mismatch");
+  LFG_REQUIRE(b->d != x->d, LFGPU_EINVAL, "PCG right-hand side and solution must differ");
+  LFG_REQUIRE(opts->rtol >= 0.0 && opts->maxit >= 0, LFGPU_EINVAL, "bad PCG options");
+  LFG_REQUIRE(opts->mode == LFGPU_PCG_FP64 && !opts->single_reduction, LFGPU_EINVAL,
+              "lfgpu_pcg_noalloc solves in FP64 with two sweeps per iteration only");
+  LFG_REQUIRE(A->nrows <= kNoAllocMaxRows, LFGPU_ERANGE,
+              "lfgpu_pcg_noalloc: the system has more rows than its fixed work space holds "
+              "(2^21 = 2097152)");
+  LFG_REQUIRE(ctx->device >= 0 && ctx->device < kNoAllocMaxDevices &&
+                  g_scratch_ptr[ctx->device] != nullptr,
+              LFGPU_EINVAL, "lfgpu_pcg_noalloc: work space of this device not set up");
+  double *base = g_scratch_ptr[ctx->device];
+  hipStream_t s = ctx->stream;
+  const int32_t n = A->nrows;
+  double *dinv = base, *r = base + kNoAllocVec, *p = base + 2 * kNoAllocVec,
+         *q = base + 3 * kNoAllocVec, *part = base + 4 * kNoAllocVec;
+  PcgState *st = reinterpret_cast<PcgState *>(base + 4 * kNoAllocVec + kNoAllocPart);
+
+  MatView m{};
+  m.n = n;
+  m.val = A->val;
+  m.row_ptr = A->row_ptr;
+  m.col = A->col;
+  const Sweep sw = make_sweep(n, kSB64);
+  const dim3 G(sw.ngroups), B(kSB64), BF(256);
+  const unsigned gflat = grid_for(n, 256);
+  int check = opts->check_every > 0 ? opts->check_every : 64;
+  check += check & 1;
+
+  LFG_HIP(hipEventRecord(ctx->ev0, s));
+  hipLaunchKernelGGL(k_noalloc_state, dim3(1), dim3(64), 0, s, st, opts->rtol * opts->rtol,
+                     opts->maxit);
+  if (n > 0)
+    hipLaunchKernelGGL(k_csr_dinv, dim3(gflat), BF, 0, s, A->row_ptr, A->col, A->val, n, dinv);
+  const CgArgs a{m, dinv, x->d, r, p, q, part, st, sw};
+  hipLaunchKernelGGL((k_cg_init<FMT_CSR, kSB64>), G, B, 0, s, a, b->d);
+  hipLaunchKernelGGL(k_cg_init_finish<kSB64>, dim3(1), B, 0, s, a);
+  if (n > 0) hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, st);
+  LFG_HIP(hipGetLastError());
+  PcgState *hs = reinterpret_cast<PcgState *>(ctx->pinned);
+  for (;;) {
+    launch_cg_chunk<FMT_CSR, kSB64>(s, sw, a, check, false);
+    LFG_HIP(hipGetLastError());
+    LFG_HIP(hipMemcpyAsync(hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    LFG_HIP(hipStreamSynchronize(s));
+    if (hs->done) break;
+  }
+  return pcg_finish(ctx, hs, false, info);
 }
 
 namespace lfgpu {
