@@ -121,13 +121,13 @@ def sweep_bytes(mode, n, sym):
       FP64 sweep 1 (k_cg_spmv):  p, r, D^-1, row sums and 3 (sym) or 6
                                  diagonals read, q written: 64N or 88N
       FP64 sweep 2 (k_cg_update): p, r, D^-1, x, q read, x, r, p written: 64N
-      FP32-Krylov sweep 1 (k_mx_a): p, r, D^-1, row sums, 6 diagonals read
-                                 (4 B each), q written: 44N
-      FP32-Krylov sweep 2 (k_mx_b): p, r, increment, q, D^-1 read,
-                                 increment, r, p written: 32N"""
+      FP32-Krylov sweep 1 (k_mx_a): p, r, row sums, 6 diagonals read (4 B
+                                 each), D^-1 read (bfloat16), q written: 42N
+      FP32-Krylov sweep 2 (k_mx_b): p, r, increment, q read, D^-1 read
+                                 (bfloat16), increment, r, p written: 30N"""
     if mode == "fp64":
         return (32 + 8 * (3 if sym else 6) + 8) * n, 64 * n
-    return 44 * n, 32 * n
+    return 42 * n, 30 * n
 
 
 def pmc_traffic(kernel, name="pmc_traffic_p1_2048.md"):
@@ -167,6 +167,36 @@ def roofline_entry(n, times_ms, sym, mode="fp64"):
                          "achieved": (b1 + b2) / (it_us * 1e-6) / 1e9,
                          "frac": (b1 + b2) / (it_us * 1e-6) / 1e9 / HBM_PEAK_GBS}}
     return out
+
+
+def ell_row_sweep_bytes(n, nnz, col_bytes):
+    """Bytes one FP64 row sweep of the sliced-ELL Jacobi-PCG must move (each
+    array once; the off-diagonal entries carry a value and a column, padding
+    slots are excluded; the gathered p values are re-reads the caches serve).
+      k_cgr_spmv:   p, r, D^-1, row sums read, q written (40 B per row) and
+                    8 + col_bytes per off-diagonal entry (nnz - N of them:
+                    every row stores its diagonal in the CSR)
+      k_cgr_update: p, r, D^-1, x, q read, x, r, p written: 64 B per row"""
+    return 40 * n + (8 + col_bytes) * (nnz - n), 64 * n
+
+
+def ell_roofline_entry(n, nnz, times_ms, col_bytes):
+    b1, b2 = ell_row_sweep_bytes(n, nnz, col_bytes)
+    it_us = 1e3 * (times_ms[0] + times_ms[1] + 2 * times_ms[2])
+    a1 = b1 / (times_ms[0] * 1e-3) / 1e9
+    a2 = b2 / (times_ms[1] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "k_cgr_spmv (sliced-ELL Jacobi-PCG row sweep 1: q = A p "
+                                      "and 3 dot products, %d-bit column offsets)"
+                                      % (8 * col_bytes),
+            "achieved": a1, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a1 / HBM_PEAK_GBS,
+            "algorithmic_bytes": b1, "avg_us": 1e3 * times_ms[0],
+            "timing": "median of per-launch hipEvent pairs on the solver stream",
+            "sweep2": {"kernel": "k_cgr_update", "algorithmic_bytes": b2,
+                       "avg_us": 1e3 * times_ms[1], "achieved": a2, "frac": a2 / HBM_PEAK_GBS},
+            "reductions_us": 2e3 * times_ms[2],
+            "iteration": {"us": it_us, "algorithmic_bytes": b1 + b2,
+                          "achieved": (b1 + b2) / (it_us * 1e-6) / 1e9,
+                          "frac": (b1 + b2) / (it_us * 1e-6) / 1e9 / HBM_PEAK_GBS}}
 
 
 def true_relres(capi, ctx, A, x, b):
@@ -527,7 +557,9 @@ def run_p1(args, capi, ctx):
     return out, checks
 
 
-def run_p3(args, capi, ctx, n=512, seed=0):
+def build_p3(capi, ctx, n=512, seed=0):
+    """The P3 workload: n x n jittered mesh, zero Dirichlet data on the
+    boundary DOFs, a = exp(N(0,1)) per cell, f = sin sin."""
     xy, cells, edges = capi.tp_triag_mesh(n)
     rng = np.random.default_rng(seed)
     h = 1.0 / n
@@ -553,6 +585,11 @@ def run_p3(args, capi, ctx, n=512, seed=0):
     b, x = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
     args_c = (fe, capi.coeff(capi.COEFF_EXPCELL, 1.0, theta), capi.coeff(capi.COEFF_CONST, 1.0),
               capi.coeff(capi.COEFF_SINSIN, 1.0), b)
+    return xy, cells, nd, nnz, mesh, fe, plan, theta, b, x, args_c
+
+
+def run_p3(args, capi, ctx, n=512, seed=0):
+    xy, cells, nd, nnz, mesh, fe, plan, theta, b, x, args_c = build_p3(capi, ctx, n, seed)
     plan.assemble(*args_c)
     asm = [plan.assemble(*args_c) for _ in range(3)]
     log("p3: N=%d nnz=%d assembly (exact element order) %.3f ms" % (nd, nnz, np.median(asm)))
@@ -583,6 +620,14 @@ def run_p3(args, capi, ctx, n=512, seed=0):
         log("p3 %s: %.1f ms, %d iters, true relres %.2e" % (name, inf.ms, inf.iters,
                                                             res[name]["true_relres"]))
     xs = {k: v.pop("x") for k, v in res.items()}
+    roof = None
+    if not args.no_solve:
+        # per-launch sweep durations of 200 FP64 iterations (plain launches)
+        t, _ = capi.pcg_sweep_times(ctx, plan.csr, b, x, iters=200, mode=capi.PCG_FP64)
+        st = capi.solver_storage(plan.csr)[0]
+        roof = ell_roofline_entry(nd, nnz, t, 2 if st == capi.STORAGE_ELL16 else 4)
+        log("p3 FP64 sweeps: %.1f + %.1f us (+ 2 x %.1f us reductions), %.0f GB/s per iteration"
+            % (1e3 * t[0], 1e3 * t[1], 1e3 * t[2], roof["iteration"]["achieved"]))
     out = {"metric": "P3 assembly Melem/s + PCG time to rtol 1e-10", "config": {
         "workload": "P3 reaction-diffusion, %dx%d jittered (0.2h) triangles, degree-6 "
                     "quadrature, a=exp(N(0,1)), f=sin sin" % (n, n),
@@ -594,7 +639,7 @@ def run_p3(args, capi, ctx, n=512, seed=0):
         "mfma_matrix_row_scaled_diff": mat_err, "mfma_rhs_bitwise_equal": rhs_equal,
         "pcg_matrix": "MFMA-contracted; err_vs_oracle against the oracle's solution of its "
                       "own reference-order system",
-        "pcg": res, "oracle": None,
+        "pcg": res, "roofline": roof, "oracle": None,
         "mixed_speedup_vs_fp64": {k: res["fp64"]["s"] / v["s"] for k, v in res.items()}
         if "fp64" in res else None}
     for hh in (b, x, theta, plan, fe, mesh):
@@ -663,14 +708,16 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
     per_mode = {}
     for name, md in modes.items():
         x.fill(0.0)
-        batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md)  # tunes / captures the graph
+        # tunes / captures the graph
+        batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md, graph=args.batch_graph)
         times = []
         for _ in range(max(1, args.batch_steps)):
             ctx.sync()
             t0 = time.perf_counter()
             asm_ms = plan.assemble(*acoef)
             x.fill(0.0)
-            summ, infos = batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md)
+            summ, infos = batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md,
+                                    graph=args.batch_graph)
             ctx.sync()
             times.append((time.perf_counter() - t0, asm_ms, summ.ms))
         sols[name] = x.numpy()
@@ -695,6 +742,7 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
                       "cells": int(ne), "dofs": int(nv), "nnz": int(plan.csr.info()[2])},
            "ms_per_step": sel["ms_per_step"], "modes": per_mode,
            "storage": {capi.STORAGE_ELL: "sliced ELL", capi.STORAGE_DIA: "DIA",
+                       capi.STORAGE_ELL16: "sliced ELL, 16-bit column offsets",
                        capi.STORAGE_BLOCK_DIA: "per-problem DIA"}.get(batch.storage()),
            "speedup_vs_fp64": ({k: per_mode["fp64"]["ms_per_step"] / v["ms_per_step"]
                                 for k, v in per_mode.items()} if "fp64" in per_mode else None),
@@ -970,28 +1018,51 @@ def run_c1(args, capi, ctx, n=64, reps=20):
 
 
 def run_traffic(args, capi, ctx):
-    """A short fixed sequence for HBM byte counters (rocprofv3 --pmc): the
-    FP64 SpMV of the 2048x2048 system, a few PCG iterations and a plain copy
-    kernel of known size.  Prints the dispatch list with algorithmic bytes."""
+    """A short fixed sequence for HBM byte counters (rocprofv3 --pmc) on the
+    2048x2048 system: the FP64 SpMV, 8 FP64 PCG iterations (k_cg_spmv /
+    k_cg_update), 8 FP32-Krylov iterations (k_mx_a / k_mx_b), the two
+    calibration kernels of known size (a plain copy and a non-temporal copy of
+    1 GiB each), then 8 FP64 iterations of the 512 x 512 P3 system (row
+    sweeps k_cgr_spmv / k_cgr_update on sliced ELL).  Returns the algorithmic
+    bytes per dispatch of each kernel."""
     P = Problem(capi, ctx, args.n, args.seed)
     P.assemble()
     xs, ys = capi.Vec(ctx, np.random.default_rng(1).standard_normal(P.nv)), capi.Vec(ctx, P.nv)
     capi.spmv_timed(ctx, P.plan.csr, xs, ys, 3)
-    P.x.fill(0.0)
-    capi.pcg(ctx, P.plan.csr, P.b, P.x, rtol=1e-10, maxit=4, check_every=2,
-             raise_on_noconv=False)
+    for md in (capi.PCG_FP64, capi.PCG_MIXED):
+        P.x.fill(0.0)
+        capi.pcg(ctx, P.plan.csr, P.b, P.x, rtol=1e-10, maxit=8, mode=md, check_every=8,
+                 raise_on_noconv=False)
+    _, _, sym = capi.solver_storage(P.plan.csr)
     ncopy = 1 << 27  # 1 GiB of doubles
     c0, c1 = capi.Vec(ctx, ncopy), capi.Vec(ctx, ncopy)
     c0.fill(1.0)
     for _ in range(3):
         capi.copy(ctx, c0, c1)
+    for _ in range(3):
+        capi.copy_nt(ctx, c0, c1)
     ctx.sync()
+    b64 = sweep_bytes("fp64", P.nv, sym)
+    b32 = sweep_bytes("mixed", P.nv, False)
     out = {"config": "traffic", "n": args.n, "dofs": P.nv, "nnz": P.nnz,
-           "spmv_dispatches": 3, "spmv_algorithmic_bytes": spmv_bytes(P.nnz, P.nv),
-           "copy_dispatches": 3, "copy_bytes_read": 8 * ncopy, "copy_bytes_written": 8 * ncopy}
+           "spmv_dispatches": 3, "spmv_csr_formula_bytes": spmv_bytes(P.nnz, P.nv),
+           "k_cg_spmv_bytes": b64[0], "k_cg_update_bytes": b64[1],
+           "k_mx_a_bytes": b32[0], "k_mx_b_bytes": b32[1],
+           "copy_dispatches": 3, "copy_nt_dispatches": 3,
+           "copy_bytes_read": 8 * ncopy, "copy_bytes_written": 8 * ncopy}
     for h in (c0, c1, xs, ys):
         h.close()
     P.close()
+    # the P3 row sweeps (sliced ELL, locality order, 16-bit column offsets):
+    # 8 FP64 iterations on the 512 x 512 P3 system
+    _, _, nd3, nnz3, mesh, fe, plan, theta, b, x, args_c = build_p3(capi, ctx)
+    plan.assemble(*args_c, mfma=True)
+    capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, maxit=8, check_every=8, raise_on_noconv=False)
+    st = capi.solver_storage(plan.csr)[0]
+    e1, e2 = ell_row_sweep_bytes(nd3, nnz3, 2 if st == capi.STORAGE_ELL16 else 4)
+    out.update({"p3_dofs": nd3, "p3_nnz": nnz3, "k_cgr_spmv_bytes": e1, "k_cgr_update_bytes": e2})
+    for h in (b, x, theta, plan, fe, mesh):
+        h.close()
     return out
 
 
@@ -1093,6 +1164,8 @@ def main():
                     help="skip the oracle's assembly and solves (parity at full size)")
     ap.add_argument("--c2-only", action="store_true",
                     help="default configuration: only the 2048^2 P1 problem (no P3 / batched)")
+    ap.add_argument("--batch-graph", type=int, default=0, choices=[0, -1],
+                    help="batched: 0 replays captured chunks, -1 plain launches")
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="batched: timed repetitions per precision mode")
     ap.add_argument("--sweep-iters", type=int, default=200,

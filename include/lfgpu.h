@@ -72,6 +72,10 @@ int lfgpu_vec_download(const lfgpu_vec *v, double *host, int64_t n);
 int lfgpu_vec_fill(lfgpu_vec *v, double value);
 /* y = x on the context stream (both of the same size, different vectors). */
 int lfgpu_vec_copy(lfgpu_ctx *ctx, const lfgpu_vec *x, lfgpu_vec *y);
+/* y = x with non-temporal (cache-bypassing) loads and stores: calibration of
+ * HBM byte counters for such streams (measurement support, no reference
+ * counterpart). */
+int lfgpu_vec_copy_nt(lfgpu_ctx *ctx, const lfgpu_vec *x, lfgpu_vec *y);
 
 /* ---- mesh + DOF handler ------------------------------------------------
  * Replaces the cell loop's view of the mesh and of UniformFEDofHandler
@@ -309,13 +313,15 @@ typedef struct lfgpu_pcg_info {
   int refinements; /* FP64 true-residual replacements (mixed)     */
   double relres;   /* final ||b - A x|| / ||b|| (FP64)            */
   double ms;       /* device time of the solve                    */
+  int syncs;       /* host waits on the solver state (polls)      */
 } lfgpu_pcg_info;
 int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu_vec *b,
               lfgpu_vec *x, const lfgpu_pcg_opts *opts, lfgpu_pcg_info *info);
 
 /* Storage of the solver's copy of A, built by the first SpMV / PCG call
- * (storage -1 before): LFGPU_STORAGE_DIA or LFGPU_STORAGE_ELL (defined with
- * the batched solves below), *ndiag the stored off-diagonals of DIA, *sym 1
+ * (storage -1 before): LFGPU_STORAGE_DIA, LFGPU_STORAGE_ELL or
+ * LFGPU_STORAGE_ELL16 (defined with the batched solves below), *ndiag the
+ * stored off-diagonals of DIA, *sym 1
  * when the DIA values are mirror-symmetric bit for bit, so that the FP64
  * product sweep reads the upper diagonals only (ndiag and sym may be NULL).
  * Introspection for measurement; no reference counterpart. */
@@ -383,6 +389,9 @@ int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec *b,
 #define LFGPU_STORAGE_ELL 0
 #define LFGPU_STORAGE_DIA 1
 #define LFGPU_STORAGE_BLOCK_DIA 2
+/* sliced ELL whose columns are stored as 16-bit offsets from the row
+ * (chosen when every stored column lies within 32767 rows of its row) */
+#define LFGPU_STORAGE_ELL16 3
 int lfgpu_batch_storage(const lfgpu_batch *bt, int *storage);
 
 /* ---- error norms ---------------------------------------------------------

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "liblfgpu.so")
 OK, EINVAL, EDEGENERATE, ENOCONV, EOOM, EHIP, ENORULE, ERANGE = range(8)
 COEFF_CONST, COEFF_CELL, COEFF_QP, COEFF_SINSIN, COEFF_EXPCELL = range(5)
 PCG_FP64, PCG_MIXED, PCG_MIXED_MATRIX = 0, 1, 2
-STORAGE_ELL, STORAGE_DIA, STORAGE_BLOCK_DIA = 0, 1, 2
+STORAGE_ELL, STORAGE_DIA, STORAGE_BLOCK_DIA, STORAGE_ELL16 = 0, 1, 2, 3
 
 _P = C.c_void_p
 _d = C.POINTER(C.c_double)
@@ -41,7 +41,7 @@ class PcgOpts(C.Structure):
 
 class PcgInfo(C.Structure):
     _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("refinements", C.c_int),
-                ("relres", C.c_double), ("ms", C.c_double)]
+                ("relres", C.c_double), ("ms", C.c_double), ("syncs", C.c_int)]
 
 
 class StrangInfo(C.Structure):
@@ -64,6 +64,7 @@ SIGNATURES = {
     "lfgpu_vec_download": (C.c_int, [_P, _d, C.c_int64]),
     "lfgpu_vec_fill": (C.c_int, [_P, C.c_double]),
     "lfgpu_vec_copy": (C.c_int, [_P, _P, _P]),
+    "lfgpu_vec_copy_nt": (C.c_int, [_P, _P, _P]),
     "lfgpu_mesh_create": (C.c_int, [_P, C.c_int64, _d, C.c_int64, _i32, C.c_int, _i64,
                                     C.c_int64, C.POINTER(_P)]),
     "lfgpu_mesh_destroy": (C.c_int, [_P]),
@@ -510,7 +511,7 @@ class Batch:
         return summ, list(infos)
 
     def storage(self):
-        """lfgpu_batch_storage: STORAGE_ELL / _DIA / _BLOCK_DIA (-1 before a solve)."""
+        """lfgpu_batch_storage: STORAGE_ELL / _ELL16 / _DIA / _BLOCK_DIA (-1 before a solve)."""
         v = C.c_int()
         check(lib().lfgpu_batch_storage(self.h, C.byref(v)))
         return v.value
@@ -590,6 +591,10 @@ class Strang:
 
 def copy(ctx, x: Vec, y: Vec):
     check(lib().lfgpu_vec_copy(ctx.h, x.h, y.h))
+
+
+def copy_nt(ctx, x: Vec, y: Vec):
+    check(lib().lfgpu_vec_copy_nt(ctx.h, x.h, y.h))
 
 
 def dot(ctx, x: Vec, y: Vec):

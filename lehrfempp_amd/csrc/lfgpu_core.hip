@@ -154,6 +154,8 @@ extern "C" int lfgpu_ctx_create(int device, lfgpu_ctx **out) {
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k)
+    e = hipEventCreateWithFlags(&c->evp[k], hipEventDisableTiming);
   if (e == hipSuccess) e = hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault);
   if (e != hipSuccess) {
     lfgpu_ctx_destroy(c);
@@ -171,6 +173,8 @@ extern "C" int lfgpu_ctx_destroy(lfgpu_ctx *c) {
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (hipEvent_t ev : c->evp)
+    if (ev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return LFGPU_OK;
@@ -263,6 +267,23 @@ extern "C" int lfgpu_vec_fill(lfgpu_vec *v, double value) {
 __global__ void k_copy(const double2 *__restrict__ x, double2 *__restrict__ y, int64_t n2) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n2) y[i] = x[i];
+}
+
+// y = x with non-temporal loads and stores: the calibration kernel of the HBM
+// byte counters for streams that bypass the caches
+__global__ void k_copy_nt(const double *__restrict__ x, double *__restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(&x[i]), &y[i]);
+}
+
+extern "C" int lfgpu_vec_copy_nt(lfgpu_ctx *ctx, const lfgpu_vec *x, lfgpu_vec *y) {
+  LFG_REQUIRE(ctx && x && y && x->n == y->n, LFGPU_EINVAL, "bad lfgpu_vec_copy_nt arguments");
+  LFG_REQUIRE(x->d != y->d, LFGPU_EINVAL, "lfgpu_vec_copy_nt needs two different vectors");
+  if (x->n == 0) return LFGPU_OK;
+  hipLaunchKernelGGL(k_copy_nt, dim3(grid_for(x->n, 256)), dim3(256), 0, ctx->stream, x->d, y->d,
+                     x->n);
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
 }
 
 extern "C" int lfgpu_vec_copy(lfgpu_ctx *ctx, const lfgpu_vec *x, lfgpu_vec *y) {

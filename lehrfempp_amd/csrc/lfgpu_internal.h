@@ -64,6 +64,7 @@ struct lfgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evp[2] = {nullptr, nullptr};  // solver polls (no timing)
   double last_ms = 0.0;
   // pinned host mirror of small solver status words
   void *pinned = nullptr;

@@ -16,6 +16,7 @@
 //             entries column-major inside a slice, padding value 0 and
 //             column = row.
 //   * Jacobi inverse diagonal (Eigen DiagonalPreconditioner semantics).
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -76,11 +77,13 @@ struct lfgpu_solver_cache {
   int32_t nslices = 0;
   int64_t mlen = 0;              // stored off-diagonal slots
   int32_t *slice_off = nullptr;  // ELL: [nslices + 1] element offsets
-  int32_t *s_col = nullptr;      // ELL columns
+  int32_t *s_col = nullptr;      // ELL columns (int32), or
+  int16_t *s_dcol = nullptr;     // ELL column minus row, when every offset fits
   double *val = nullptr;         // ELL / DIA values
   float *val32 = nullptr;
   double *dinv = nullptr;
-  float *dinv32 = nullptr;
+  float *dinv32 = nullptr;           // FP32-matrix mode: D^-1 in FP32
+  __hip_bfloat16 *dinv16 = nullptr;  // FP32-Krylov mode: D^-1 rounded to bfloat16
   double *rowsum = nullptr;
   float *rowsum32 = nullptr;
   uint64_t built_version = ~0ull;
@@ -118,8 +121,8 @@ void solver_cache_free(lfgpu_csr *A) {
   lfgpu_solver_cache *c = A->cache;
   if (!c) return;
   if (c->cg_graphs) cg_graphs_destroy(c->cg_graphs);
-  for (void *p : {(void *)c->slice_off, (void *)c->s_col, (void *)c->val, (void *)c->val32,
-                  (void *)c->dinv, (void *)c->dinv32, (void *)c->rowsum, (void *)c->rowsum32,
+  for (void *p : {(void *)c->slice_off, (void *)c->s_col, (void *)c->s_dcol, (void *)c->val, (void *)c->val32,
+                  (void *)c->dinv, (void *)c->dinv32, (void *)c->dinv16, (void *)c->rowsum, (void *)c->rowsum32,
                   (void *)c->r, (void *)c->p, (void *)c->q, (void *)c->r32, (void *)c->p32,
                   (void *)c->q32, (void *)c->xa32, (void *)c->xlo, (void *)c->r64, (void *)c->xa64,
                   (void *)c->part, (void *)c->cg1, (void *)c->rpart, (void *)c->st,
@@ -142,6 +145,7 @@ struct MatView {
   int off[kMaxDiag];
   const int32_t *__restrict__ slice_off;
   const int32_t *__restrict__ s_col;
+  const int16_t *__restrict__ s_dcol;  // non-null: 16-bit column offsets
   const double *__restrict__ val;
   const float *__restrict__ val32;
   const double *__restrict__ rowsum;
@@ -160,11 +164,73 @@ MatView view_of(const lfgpu_solver_cache *c, int32_t n) {
   for (int d = 0; d < kMaxDiag; ++d) m.off[d] = c->off[d];
   m.slice_off = c->slice_off;
   m.s_col = c->s_col;
+  m.s_dcol = c->s_dcol;
   m.val = c->val;
   m.val32 = c->val32;
   m.rowsum = c->rowsum;
   m.rowsum32 = c->rowsum32;
   return m;
+}
+
+// One sliced-ELL row: sum over its w slots k of v_k (x_{c_k} - xr), in
+// slot order (vp, cp point at the row's first slot; a row's slots are 64
+// elements apart).  CT = int32_t: stored columns; CT = int16_t: column minus
+// row.  Eight, then four, slots' value and column loads and gathers are
+// issued before their terms are added.
+template <class CT>
+__device__ __forceinline__ int32_t ell_col(const CT *__restrict__ cp, int32_t k, int32_t r) {
+  if constexpr (sizeof(CT) == 2)
+    return r + (int32_t)cp[64 * k];
+  else
+    return cp[64 * k];
+}
+
+template <class TV, class CT, class XF>
+__device__ __forceinline__ double ell_row_sum(const TV *__restrict__ vp, const CT *__restrict__ cp,
+                                              int32_t w, int32_t r, double xr, const XF &xf) {
+  using XT = decltype(xf(0));
+  double acc = 0.0;
+  int32_t k = 0;
+  for (; k + 8 <= w; k += 8) {
+    int32_t c[8];
+    TV vv[8];
+    XT xx[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      c[u] = ell_col(cp, k + u, r);
+      vv[u] = vp[64 * (k + u)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xx[u] = xf(c[u]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += (double)vv[u] * ((double)xx[u] - xr);
+  }
+  for (; k + 4 <= w; k += 4) {
+    int32_t c[4];
+    TV vv[4];
+    XT xx[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c[u] = ell_col(cp, k + u, r);
+      vv[u] = vp[64 * (k + u)];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xx[u] = xf(c[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += (double)vv[u] * ((double)xx[u] - xr);
+  }
+  for (; k < w; ++k) acc += (double)vp[64 * k] * ((double)xf(ell_col(cp, k, r)) - xr);
+  return acc;
+}
+
+template <class TV, class XF>
+__device__ __forceinline__ double ell_row(const MatView &m, const TV *__restrict__ val, int32_t r,
+                                          double xr, const XF &xf) {
+  const int32_t s = r >> 6, l = r & 63;
+  const int32_t o = m.slice_off[s];
+  const int32_t w = (m.slice_off[s + 1] - o) >> 6;
+  return m.s_dcol ? ell_row_sum(val + o + l, m.s_dcol + o + l, w, r, xr, xf)
+                  : ell_row_sum(val + o + l, m.s_col + o + l, w, r, xr, xf);
 }
 
 // (A x)_r in FP64 (difference form).  HOIST (DIA): issue every load of the
@@ -226,37 +292,7 @@ __device__ __forceinline__ double apply64(const MatView &m, const double *__rest
       }
     }
   } else {
-    const int32_t s = r >> 6, l = r & 63;
-    const int32_t o = m.slice_off[s];
-    const int32_t w = (m.slice_off[s + 1] - o) >> 6;
-    const double *vp = m.val + o + l;
-    const int32_t *cp = m.s_col + o + l;
-    int32_t k = 0;
-    for (; k + 8 <= w; k += 8) {
-      int32_t c[8];
-      double vv[8], xx[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        c[u] = cp[64 * (k + u)];
-        vv[u] = vp[64 * (k + u)];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) xx[u] = x[c[u]];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc += vv[u] * (xx[u] - xr);
-    }
-    for (; k + 4 <= w; k += 4) {
-      const int32_t c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)],
-                    c3 = cp[64 * (k + 3)];
-      const double v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)],
-                   v3 = vp[64 * (k + 3)];
-      const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-      acc += v0 * (x0 - xr);
-      acc += v1 * (x1 - xr);
-      acc += v2 * (x2 - xr);
-      acc += v3 * (x3 - xr);
-    }
-    for (; k < w; ++k) acc += vp[64 * k] * (x[cp[64 * k]] - xr);
+    acc = ell_row(m, m.val, r, xr, [&](int32_t c) { return x[c]; });
   }
   return acc + m.rowsum[r] * xr;
 }
@@ -293,37 +329,7 @@ __device__ __forceinline__ float apply32(const MatView &m, const float *__restri
       }
     }
   } else {
-    const int32_t s = r >> 6, l = r & 63;
-    const int32_t o = m.slice_off[s];
-    const int32_t w = (m.slice_off[s + 1] - o) >> 6;
-    const float *vp = m.val32 + o + l;
-    const int32_t *cp = m.s_col + o + l;
-    int32_t k = 0;
-    for (; k + 8 <= w; k += 8) {
-      int32_t c[8];
-      float vv[8], xx[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        c[u] = cp[64 * (k + u)];
-        vv[u] = vp[64 * (k + u)];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) xx[u] = x[c[u]];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc += (double)vv[u] * ((double)xx[u] - xd);
-    }
-    for (; k + 4 <= w; k += 4) {  // four independent gathers in flight
-      const int32_t c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)],
-                    c3 = cp[64 * (k + 3)];
-      const float v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)],
-                  v3 = vp[64 * (k + 3)];
-      const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-      acc += (double)v0 * ((double)x0 - xd);
-      acc += (double)v1 * ((double)x1 - xd);
-      acc += (double)v2 * ((double)x2 - xd);
-      acc += (double)v3 * ((double)x3 - xd);
-    }
-    for (; k < w; ++k) acc += (double)vp[64 * k] * ((double)x[cp[64 * k]] - xd);
+    acc = ell_row(m, m.val32, r, xd, [&](int32_t c) { return x[c]; });
   }
   return (float)(acc + (double)m.rowsum32[r] * xd);
 }
@@ -349,18 +355,38 @@ __global__ void k_slice_width(const int32_t *row_ptr, const int32_t *col, int32_
   w[s] = m * kSlice;
 }
 
+// flag |= 1 if a stored column lies more than 32767 rows from its stored row
+__global__ void k_ell_span(const int32_t *row_ptr, const int32_t *col, int32_t n,
+                           const int32_t *perm, const int32_t *iperm, int *flag) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int32_t o = perm ? perm[r] : r;
+  for (int32_t p = row_ptr[o]; p < row_ptr[o + 1]; ++p) {
+    if (col[p] == o) continue;
+    const int32_t d = (iperm ? iperm[col[p]] : col[p]) - r;
+    if (d > 32767 || d < -32768) {
+      atomicOr(flag, 1);
+      return;
+    }
+  }
+}
+
+// CT = int32_t: the column; CT = int16_t: column minus row.  Padding slots
+// point at the row itself (value 0, and x_r - x_r = 0 in the difference form).
+template <class CT>
 __global__ void k_fill_ell_cols(const int32_t *row_ptr, const int32_t *col, int32_t n,
                                 const int32_t *slice_off, const int32_t *perm,
-                                const int32_t *iperm, int32_t *s_col) {
+                                const int32_t *iperm, CT *s_col) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int32_t o = perm ? perm[r] : r;
   const int32_t s = r / kSlice, l = r % kSlice;
   const int32_t off = slice_off[s], w = (slice_off[s + 1] - off) / kSlice;
+  const int32_t base = sizeof(CT) == 2 ? r : 0;
   int32_t k = 0;
   for (int32_t p = row_ptr[o]; p < row_ptr[o + 1]; ++p)
-    if (col[p] != o) s_col[off + (k++) * kSlice + l] = iperm ? iperm[col[p]] : col[p];
-  for (; k < w; ++k) s_col[off + k * kSlice + l] = r;
+    if (col[p] != o) s_col[off + (k++) * kSlice + l] = (CT)((iperm ? iperm[col[p]] : col[p]) - base);
+  for (; k < w; ++k) s_col[off + k * kSlice + l] = (CT)(r - base);
 }
 
 // locality key of a row: its smallest column (the CSR columns are sorted)
@@ -409,7 +435,8 @@ __global__ void k_dia_check(const int32_t *row_ptr, const int32_t *col, int32_t 
 __global__ void k_fill_vals(const int32_t *row_ptr, const int32_t *col, const double *val,
                             int32_t n, int fmt, DiagSet ds, const int32_t *slice_off,
                             const int32_t *perm, double *mval, float *mval32, double *rowsum,
-                            float *rowsum32, double *dinv, float *dinv32) {
+                            float *rowsum32, double *dinv, float *dinv32,
+                            __hip_bfloat16 *dinv16) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int32_t o = perm ? perm[r] : r;  // the CSR row stored as row r
@@ -459,6 +486,7 @@ __global__ void k_fill_vals(const int32_t *row_ptr, const int32_t *col, const do
   dinv[r] = d;
   if (rowsum32) rowsum32[r] = (float)rs;
   if (dinv32) dinv32[r] = (float)d;
+  if (dinv16) dinv16[r] = __float2bfloat16((float)d);
 }
 
 // flag = 1 if a lower-diagonal value differs, bit for bit, from the
@@ -547,12 +575,24 @@ int detect_dia(lfgpu_ctx *ctx, const lfgpu_csr *A, DiagSet *ds) {
   return LFGPU_OK;
 }
 
-// Locality order of the rows of a sliced-ELL matrix: a stable counting sort
-// by each row's smallest column.  On a finite element matrix whose lowest
-// DOFs (the vertices) are numbered along the mesh, the rows of every DOF kind
-// (vertex, edge, cell DOFs) near one part of the mesh end up next to each
-// other, so a window of consecutive rows gathers from a compact window of
-// the (equally permuted) vector instead of from three distant ranges.
+// Locality order of the rows of a sliced-ELL matrix.  Rows are ordered by
+// their smallest column in windows of 2^kEllWindowBits columns and, inside a
+// window, by their length (then by index).  On a finite element matrix whose
+// lowest DOFs (the vertices) are numbered along the mesh, the rows of every
+// DOF kind (vertex, edge, cell DOFs) near one part of the mesh end up next to
+// each other, so a run of consecutive rows gathers from a compact window of
+// the (equally permuted) vector instead of from three distant ranges; the
+// length order keeps the 64-row slices of rows of one kind, so they carry
+// no more padding than in the original order.
+#ifndef LFGPU_ELL_WINDOW_BITS
+#define LFGPU_ELL_WINDOW_BITS 9
+#endif
+constexpr int kEllWindowBits = LFGPU_ELL_WINDOW_BITS;
+#ifndef LFGPU_ELL16
+#define LFGPU_ELL16 1
+#endif
+constexpr bool kEll16 = LFGPU_ELL16;
+
 int ell_locality_order(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_solver_cache *c) {
   hipStream_t s = ctx->stream;
   const int32_t n = A->nrows;
@@ -560,17 +600,21 @@ int ell_locality_order(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_solver_cache *c
   LFG_TRY(dalloc(&dkey, (size_t)n, "row keys"));
   hipLaunchKernelGGL(k_row_key, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col, n,
                      dkey);
-  std::vector<int32_t> key(n), perm(n), iperm(n);
+  std::vector<int32_t> key(n), rp((size_t)n + 1), perm(n), iperm(n);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
     e = hipMemcpyAsync(key.data(), dkey, (size_t)n * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(rp.data(), A->row_ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   dfree(dkey);
   if (e != hipSuccess) return hip_fail(e, "row keys");
-  std::vector<int32_t> start((size_t)n + 1, 0);
-  for (int32_t r = 0; r < n; ++r) start[(size_t)key[r] + 1]++;
-  for (int32_t k = 0; k < n; ++k) start[(size_t)k + 1] += start[k];
-  for (int32_t r = 0; r < n; ++r) perm[start[key[r]]++] = r;
+  std::vector<uint64_t> sk(n);
+  for (int32_t r = 0; r < n; ++r)
+    sk[r] = ((uint64_t)(uint32_t)(key[r] >> kEllWindowBits) << 32) |
+            (uint64_t)(uint32_t)(rp[(size_t)r + 1] - rp[r]);
+  for (int32_t r = 0; r < n; ++r) perm[r] = r;
+  std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return sk[a] < sk[b]; });
   for (int32_t r = 0; r < n; ++r) iperm[perm[r]] = r;
   LFG_TRY(dalloc(&c->perm, (size_t)n, "row order"));
   LFG_TRY(dalloc(&c->iperm, (size_t)n, "inverse row order"));
@@ -606,7 +650,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
       c->mlen = (int64_t)ds.ndiag * n;
     } else {
       c->fmt = FMT_ELL;
-      if (allow_perm && n > 0) LFG_TRY(ell_locality_order(ctx, A, c));
+      if (allow_perm && kEllWindowBits < 31 && n > 0) LFG_TRY(ell_locality_order(ctx, A, c));
       c->nslices = (n + kSlice - 1) / kSlice;
       int32_t *w = nullptr;
       LFG_TRY(dalloc(&w, (size_t)c->nslices, "slice widths"));
@@ -619,9 +663,31 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
       dfree(w);
       LFG_TRY(rc);
       c->mlen = len;
-      LFG_TRY(dalloc(&c->s_col, (size_t)std::max<int64_t>(len, 1), "ELL columns"));
-      hipLaunchKernelGGL(k_fill_ell_cols, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr,
-                         A->col, n, c->slice_off, c->perm, c->iperm, c->s_col);
+      // 16-bit column offsets when every stored column lies within 32767
+      // rows of its row (a locality-ordered FE matrix): 2 of the 12 bytes of
+      // an FP64 slot
+      int *flag = nullptr;
+      LFG_TRY(dalloc(&flag, 1, "ELL span flag"));
+      int hflag = 1;
+      hipError_t e = hipMemsetAsync(flag, 0, 4, s);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ell_span, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr,
+                           A->col, n, c->perm, c->iperm, flag);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      dfree(flag);
+      if (e != hipSuccess) return hip_fail(e, "ELL column span");
+      if (hflag == 0 && kEll16) {
+        LFG_TRY(dalloc(&c->s_dcol, (size_t)std::max<int64_t>(len, 1), "ELL column offsets"));
+        hipLaunchKernelGGL(k_fill_ell_cols<int16_t>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                           A->row_ptr, A->col, n, c->slice_off, c->perm, c->iperm, c->s_dcol);
+      } else {
+        LFG_TRY(dalloc(&c->s_col, (size_t)std::max<int64_t>(len, 1), "ELL columns"));
+        hipLaunchKernelGGL(k_fill_ell_cols<int32_t>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                           A->row_ptr, A->col, n, c->slice_off, c->perm, c->iperm, c->s_col);
+      }
       LFG_HIP(hipGetLastError());
     }
     LFG_TRY(dalloc(&c->val, (size_t)std::max<int64_t>(c->mlen, 1), "solver values"));
@@ -637,6 +703,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
     LFG_TRY(dalloc(&c->val32, (size_t)std::max<int64_t>(c->mlen, 1), "solver values FP32"));
     LFG_TRY(dalloc(&c->rowsum32, (size_t)n, "row sums FP32"));
     LFG_TRY(dalloc(&c->dinv32, (size_t)n, "inverse diagonal FP32"));
+    LFG_TRY(dalloc(&c->dinv16, (size_t)n, "inverse diagonal BF16"));
     LFG_TRY(dalloc(&c->r32, (size_t)n, "PCG r32"));
     LFG_TRY(dalloc(&c->p32, (size_t)n, "PCG p32"));
     LFG_TRY(dalloc(&c->q32, (size_t)n, "PCG q32"));
@@ -655,7 +722,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
                        A->val, n, c->fmt, ds, c->slice_off, c->perm, c->val,
                        c->have32 ? c->val32 : nullptr, c->rowsum,
                        c->have32 ? c->rowsum32 : nullptr, c->dinv,
-                       c->have32 ? c->dinv32 : nullptr);
+                       c->have32 ? c->dinv32 : nullptr, c->have32 ? c->dinv16 : nullptr);
     LFG_HIP(hipGetLastError());
     LFG_TRY(dia_symmetry(ctx, c, n));
     c->built_version = A->val_version;
@@ -740,6 +807,74 @@ inline void tmark_end(hipStream_t s) {
   if (e != hipSuccess && t->err == hipSuccess) t->err = e;
   t->used += 1;
 }
+
+// Chunks of iterations with the host one step behind the device.  A step
+// queues k chunks (launch() queues one), a copy of the solver state into
+// pinned host slot j (copy(j)) and an event.  The host queues step j + 1
+// before it waits for step j, so the device does not idle at a poll; once
+// step j shows that every problem has stopped, step j + 1 runs empty (each
+// kernel returns on its done flag) and completes before anything queued
+// after it.  remaining(j) reads slot j: < 0 once every problem has stopped,
+// else an estimate of the iterations still needed (0: unknown); the next
+// step queued takes half of that in chunks, between 1 and
+// kMaxChunksPerPoll.  *slot receives the slot of the final state, *syncs
+// counts the host waits.
+constexpr int kMaxChunksPerPoll = 8;
+
+template <class Launch, class Copy, class Remaining>
+int poll_chunks(lfgpu_ctx *ctx, int check, Launch &&launch, Copy &&copy, Remaining &&remaining,
+                int *slot, int *syncs) {
+  hipStream_t s = ctx->stream;
+  auto step = [&](int j, int chunks) -> int {
+    for (int q = 0; q < chunks; ++q) LFG_TRY(launch());
+    LFG_TRY(copy(j));
+    LFG_HIP(hipEventRecord(ctx->evp[j], s));
+    return LFGPU_OK;
+  };
+  int j = 0, k = 1;
+  LFG_TRY(step(0, 1));
+  for (;;) {
+    LFG_TRY(step(j ^ 1, k));
+    LFG_HIP(hipEventSynchronize(ctx->evp[j]));
+    ++*syncs;
+    const double est = remaining(j);
+    if (est < 0.0) break;
+    // the per-kernel timer keeps one chunk per step: few empty launches
+    k = t_timer ? 1
+                : (int)std::min<double>(kMaxChunksPerPoll, std::max(1.0, std::floor(0.5 * est / check)));
+    j ^= 1;
+  }
+  *slot = j;
+  return LFGPU_OK;
+}
+
+// Iterations still needed to bring ||r||^2 from rr down to thr at the rate
+// observed since the previous poll (rr0 at iteration it0), at most what the
+// cap leaves; 0 when the history gives no estimate.  With thr = 0 (a fixed
+// number of iterations) the cap alone.
+inline double remaining_iters(double rr0, int it0, double rr, int it, double thr, int maxit) {
+  const double cap = std::max(0.0, (double)maxit - it);
+  if (!(thr > 0.0)) return cap;
+  if (!(rr > thr) || !(rr < rr0) || it <= it0) return 0.0;
+  return std::min(cap, (it - it0) * (std::log(thr / rr) / std::log(rr / rr0)));
+}
+
+// Host-side history of one problem's polled residual.  rr_now is the
+// residual measured at iteration it_rr (every iteration in FP64; at the last
+// reliable update in the mixed modes), it_now the current iteration.
+struct PollHist {
+  double rr = 0.0;
+  int it = -1;
+  double est = 0.0;  // iterations still needed after iteration `it`
+  double next(double rr_now, int it_rr, int it_now, double thr, int maxit) {
+    if (it_rr != it) {
+      est = it >= 0 ? remaining_iters(rr, it, rr_now, it_rr, thr, maxit) : 0.0;
+      rr = rr_now;
+      it = it_rr;
+    }
+    return est > 0.0 ? std::max(1.0, est - (it_now - it)) : 0.0;
+  }
+};
 
 struct Sweep {
   int32_t n;
@@ -1278,8 +1413,13 @@ __device__ __forceinline__ double apply64_fn(const MatView &m, int32_t r, double
     const int32_t o = m.slice_off[sl];
     const int32_t wd = (m.slice_off[sl + 1] - o) >> 6;
     const double *vp = m.val + o + l;
-    const int32_t *cp = m.s_col + o + l;
-    for (int32_t k = 0; k < wd; ++k) acc += vp[64 * k] * (u(cp[64 * k]) - ur);
+    if (m.s_dcol) {
+      const int16_t *cp = m.s_dcol + o + l;
+      for (int32_t k = 0; k < wd; ++k) acc += vp[64 * k] * (u(ell_col(cp, k, r)) - ur);
+    } else {
+      const int32_t *cp = m.s_col + o + l;
+      for (int32_t k = 0; k < wd; ++k) acc += vp[64 * k] * (u(cp[64 * k]) - ur);
+    }
   }
   return acc + m.rowsum[r] * ur;
 }
@@ -1451,10 +1591,35 @@ void launch_cg1_chunk(hipStream_t s, const Sweep &sw, const Cg1Args &a, int coun
 // the recurrence residual has dropped by delta (or after a caller-set number
 // of iterations) and restart the FP32 recurrence from it.
 // ---------------------------------------------------------------------------
+// The FP32-Krylov mode preconditions with D^-1 rounded to bfloat16 (8
+// significant bits, FP32's exponent range, so no overflow or underflow for
+// any diagonal FP32 can hold).  Jacobi-PCG converges with any SPD diagonal;
+// a relative perturbation of at most 2^-9 changed the iteration count of the
+// 4.2M-DOF P1 system by 0.01% and saves 4 of the two sweeps' bytes per row.
+// The FP64 residual of the reliable updates, which decides convergence,
+// never sees it.
+__device__ __forceinline__ float dinv_bf16(const __hip_bfloat16 *__restrict__ d, int32_t i) {
+  return __bfloat162float(d[i]);
+}
+
+// D^-1_i in the recurrence precision: bfloat16 for FP32 Krylov vectors
+// (VT = float); FP32 for the FP32-matrix mode (VT = double), whose FP64
+// recurrence measured 8% more iterations on the 2.4M-DOF P3 system with the
+// bfloat16 diagonal
+template <class Args>
+__device__ __forceinline__ auto mix_dinv(const Args &a, int32_t i) {
+  using VT = std::remove_reference_t<decltype(*a.p)>;
+  if constexpr (std::is_same_v<VT, float>)
+    return dinv_bf16(a.dinv16, i);
+  else
+    return (double)a.dinv32[i];
+}
+
 template <class VT>
 struct MixArgs {
   MatView m;
   const float *__restrict__ dinv32;
+  const __hip_bfloat16 *__restrict__ dinv16;
   const double *__restrict__ dinv;
   double *__restrict__ x;    // high part of the solution (the caller's vector)
   double *__restrict__ xlo;  // low part
@@ -1479,7 +1644,7 @@ __global__ __launch_bounds__(256) void k_mix_setup(MixArgs<VT> a) {
   if (i >= a.sw.n) return;
   const VT ri = (VT)a.r64[i];
   a.r[i] = ri;
-  a.p[i] = (VT)a.dinv32[i] * ri;
+  a.p[i] = mix_dinv(a, i) * ri;
   a.xa[i] = (VT)0;
   a.xlo[i] = 0.0;
 }
@@ -1518,38 +1683,7 @@ __device__ __forceinline__ double apply_mixed(const MatView &m, const double *__
       }
     }
   } else {
-    const int32_t s = r >> 6, l = r & 63;
-    const int32_t o = m.slice_off[s];
-    const int32_t w = (m.slice_off[s + 1] - o) >> 6;
-    const float *vp = m.val32 + o + l;
-    const int32_t *cp = m.s_col + o + l;
-    int32_t k = 0;
-    for (; k + 8 <= w; k += 8) {
-      int32_t c[8];
-      float vv[8];
-      double xx[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        c[u] = cp[64 * (k + u)];
-        vv[u] = vp[64 * (k + u)];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) xx[u] = x[c[u]];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc += (double)vv[u] * (xx[u] - xr);
-    }
-    for (; k + 4 <= w; k += 4) {  // four independent gathers in flight
-      const int32_t c0 = cp[64 * k], c1 = cp[64 * (k + 1)], c2 = cp[64 * (k + 2)],
-                    c3 = cp[64 * (k + 3)];
-      const float v0 = vp[64 * k], v1 = vp[64 * (k + 1)], v2 = vp[64 * (k + 2)],
-                  v3 = vp[64 * (k + 3)];
-      const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-      acc += (double)v0 * (x0 - xr);
-      acc += (double)v1 * (x1 - xr);
-      acc += (double)v2 * (x2 - xr);
-      acc += (double)v3 * (x3 - xr);
-    }
-    for (; k < w; ++k) acc += (double)vp[64 * k] * (x[cp[64 * k]] - xr);
+    acc = ell_row(m, m.val32, r, xr, [&](int32_t c) { return x[c]; });
   }
   return acc + (double)m.rowsum32[r] * xr;
 }
@@ -1557,7 +1691,7 @@ __device__ __forceinline__ double apply_mixed(const MatView &m, const double *__
 // Two sweeps per iteration, as in the FP64 solver.  Sweep A decides, from
 // the previous sweep's partials (identically in every workgroup), whether
 // this slot is a CG step or a reliable update:
-//   step:    q = A32 p, partials p.q, (M q, r), (M q, q)          (M = D^-1 in FP32)
+//   step:    q = A32 p, partials p.q, (M q, r), (M q, q)          (M = D^-1 in BF16)
 //   update:  r64 = b - A x - A xa - A xlo in FP64 (x, xa, xlo as left by the
 //            previous slot), partial ||r64||^2
 // Sweep B completes the slot:
@@ -1603,7 +1737,7 @@ __global__ __launch_bounds__(SB) void k_mx_a(MixArgs<VT> a, int par) {
   if (!refine) {
     double pq = 0.0, qdr = 0.0, qdq = 0.0;
     for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
-      const VT pi = a.p[i], mi = (VT)a.dinv32[i], ri = a.r[i];
+      const VT pi = a.p[i], mi = mix_dinv(a, i), ri = a.r[i];
       const VT qi = apply_mixed<FMT, true>(a.m, a.p, i, pi);
       a.q[i] = qi;
       const double dq = (double)(mi * qi);
@@ -1652,7 +1786,7 @@ __global__ __launch_bounds__(SB) void k_mx_b(MixArgs<VT> a, int par) {
       // plain loads here: the non-temporal form measured 2% slower for
       // this chunked sweep (faster in the batched and row sweeps)
       const VT pi = a.p[i], xai = a.xa[i], r0 = a.r[i], qi = a.q[i];
-      const VT mi = (VT)a.dinv32[i];
+      const VT mi = mix_dinv(a, i);
       const VT ri = r0 - fa * qi;
       const VT zi = mi * ri;
       a.xa[i] = xai + fa * pi;
@@ -1692,7 +1826,7 @@ __global__ __launch_bounds__(SB) void k_mx_b(MixArgs<VT> a, int par) {
       a.xa[i] = (VT)0;
       if (!stop) {
         const VT ri = (VT)a.r64[i];
-        const VT mi = (VT)a.dinv32[i];
+        const VT mi = mix_dinv(a, i);
         a.p[i] += mi * (ri - a.r[i]);
         a.r[i] = ri;
         const VT zi = mi * ri;
@@ -1747,7 +1881,7 @@ __global__ __launch_bounds__(kRB) void k_mxr_init(MixArgs<VT> a) {
     a.r64[i] = ri;
     const VT rv = (VT)ri;
     a.r[i] = rv;
-    a.p[i] = (VT)a.dinv32[i] * rv;
+    a.p[i] = mix_dinv(a, i) * rv;
     a.xa[i] = (VT)0;
     a.xlo[i] = 0.0;
     v[0] = bi * bi;
@@ -1801,7 +1935,7 @@ __global__ __launch_bounds__(kRB) void k_mxr_a(MixArgs<VT> a) {
       const VT pi = a.p[i];
       const VT qi = apply_mixed<FMT>(a.m, a.p, i, pi);
       a.q[i] = qi;
-      const double dq = (double)((VT)a.dinv32[i] * qi);
+      const double dq = (double)(mix_dinv(a, i) * qi);
       v[0] = (double)pi * (double)qi;
       v[1] = dq * (double)a.r[i];
       v[2] = dq * (double)qi;
@@ -1873,7 +2007,7 @@ __global__ __launch_bounds__(kRB) void k_mxr_b(MixArgs<VT> a) {
       const VT pi = a.p[i], r0 = a.r[i];
       const VT xai = __builtin_nontemporal_load(&a.xa[i]);
       const VT qi = __builtin_nontemporal_load(&a.q[i]);
-      const VT mi = (VT)a.dinv32[i];
+      const VT mi = mix_dinv(a, i);
       const VT ri = r0 - fa * qi;
       const VT zi = mi * ri;
       __builtin_nontemporal_store(xai + fa * pi, &a.xa[i]);
@@ -1894,7 +2028,7 @@ __global__ __launch_bounds__(kRB) void k_mxr_b(MixArgs<VT> a) {
       a.xa[i] = (VT)0;
       if (!stop) {
         const VT ri = (VT)a.r64[i];
-        const VT mi = (VT)a.dinv32[i];
+        const VT mi = mix_dinv(a, i);
         a.p[i] += mi * (ri - a.r[i]);
         a.r[i] = ri;
         const VT zi = mi * ri;
@@ -1992,7 +2126,7 @@ struct CgGraph {
 
 bool same_view(const MatView &x, const MatView &y) {
   if (x.n != y.n || x.lo != y.lo || x.ndiag != y.ndiag || x.slice_off != y.slice_off ||
-      x.s_col != y.s_col || x.val != y.val || x.val32 != y.val32 || x.rowsum != y.rowsum ||
+      x.s_col != y.s_col || x.s_dcol != y.s_dcol || x.val != y.val || x.val32 != y.val32 || x.rowsum != y.rowsum ||
       x.rowsum32 != y.rowsum32 || x.row_ptr != y.row_ptr || x.col != y.col)
     return false;
   for (int d = 0; d < kMaxDiag; ++d)
@@ -2083,43 +2217,41 @@ void launch_mix_chunk(hipStream_t s, const Sweep &sw, const MixArgs<VT> &m, int 
   }
 }
 
-template <class VT>
+// poll(launch): poll_chunks with the solve's state copies (done 1, or 2:
+// every workgroup has folded its rows, both stop the polling)
+template <class VT, class Poll>
 int run_mixed(hipStream_t s, const lfgpu_solver_cache *c, const Sweep &sw, const MixArgs<VT> &m,
-              int check, PcgState *hs) {
+              int check, Poll &&poll) {
   const unsigned gflat = grid_for(sw.n, 256);
   hipLaunchKernelGGL(k_mix_setup<VT>, dim3(gflat), dim3(256), 0, s, m);
   LFG_HIP(hipGetLastError());
-  for (;;) {
+  LFG_TRY(poll([&]() -> int {
     if (c->fmt == FMT_DIA)
       launch_mix_chunk<FMT_DIA, VT>(s, sw, m, check);
     else
       launch_mix_chunk<FMT_ELL, VT>(s, sw, m, check);
     LFG_HIP(hipGetLastError());
-    LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    LFG_HIP(hipStreamSynchronize(s));
-    if (hs->done) break;  // 1, or 2: every workgroup has folded its rows
-  }
+    return LFGPU_OK;
+  }));
   hipLaunchKernelGGL(k_mix_final<VT>, dim3(gflat), dim3(256), 0, s, m);
   LFG_HIP(hipGetLastError());
   return LFGPU_OK;
 }
 
 // mixed precision on sliced ELL (k_mxr_*): set-up, slots until done, x += xlo
-template <class VT>
+template <class VT, class Poll>
 int run_mixed_rows(hipStream_t s, const lfgpu_solver_cache *c, const MixArgs<VT> &m, int32_t n,
-                   int check, PcgState *hs) {
+                   int check, Poll &&poll) {
   const unsigned gflat = grid_for(n, 256);
   hipLaunchKernelGGL((k_mxr_init<FMT_ELL, VT>), dim3(m.nwg), dim3(kRB), 0, s, m);
   hipLaunchKernelGGL(k_mxr_red_init<VT>, dim3(1), dim3(kRedB), 0, s, m);
   hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), dim3(256), 0, s, m.x, n, c->st);
   LFG_HIP(hipGetLastError());
-  for (;;) {
+  LFG_TRY(poll([&]() -> int {
     launch_mxr_chunk<FMT_ELL, VT>(s, m, check);
     LFG_HIP(hipGetLastError());
-    LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    LFG_HIP(hipStreamSynchronize(s));
-    if (hs->done) break;
-  }
+    return LFGPU_OK;
+  }));
   hipLaunchKernelGGL(k_mix_final<VT>, dim3(gflat), dim3(256), 0, s, m);
   LFG_HIP(hipGetLastError());
   return LFGPU_OK;
@@ -2377,6 +2509,7 @@ template <class VT>
 struct BMArgs {
   MatView m;
   const float *__restrict__ dinv32;
+  const __hip_bfloat16 *__restrict__ dinv16;
   const double *__restrict__ dinv;
   double *__restrict__ x;
   double *__restrict__ xlo;
@@ -2415,7 +2548,7 @@ __global__ __launch_bounds__(256) void k_bmix_init(BMArgs<VT> a) {
     a.r64[i] = ri;
     const VT rv = (VT)ri;
     a.r[i] = rv;
-    a.p[i] = (VT)a.dinv32[i] * rv;
+    a.p[i] = mix_dinv(a, i) * rv;
     a.xa[i] = (VT)0;
     a.xlo[i] = 0.0;
     bb += bi * bi;
@@ -2499,7 +2632,7 @@ __global__ __launch_bounds__(256) void k_bmx_a(BMArgs<VT> a, int par) {
   if (!refine) {
     double pq = 0.0, qdr = 0.0, qdq = 0.0;
     for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-      const VT pi = a.p[i], mi = (VT)a.dinv32[i], ri = a.r[i];
+      const VT pi = a.p[i], mi = mix_dinv(a, i), ri = a.r[i];
       const VT qi = apply_mixed<AF, true>(m, a.p, i, pi);
       a.q[i] = qi;
       const double dq = (double)(mi * qi);
@@ -2548,7 +2681,7 @@ __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
       const VT pi = a.p[i], r0 = a.r[i];
       const VT xai = __builtin_nontemporal_load(&a.xa[i]);
       const VT qi = __builtin_nontemporal_load(&a.q[i]);
-      const VT mi = (VT)a.dinv32[i];
+      const VT mi = mix_dinv(a, i);
       const VT ri = r0 - fa * qi;
       const VT zi = mi * ri;
       __builtin_nontemporal_store(xai + fa * pi, &a.xa[i]);
@@ -2583,7 +2716,7 @@ __global__ __launch_bounds__(256) void k_bmx_b(BMArgs<VT> a, int par) {
       a.xa[i] = (VT)0;
       if (!stop) {
         const VT ri = (VT)a.r64[i];
-        const VT mi = (VT)a.dinv32[i];
+        const VT mi = mix_dinv(a, i);
         a.p[i] += mi * (ri - a.r[i]);
         a.r[i] = ri;
         const VT zi = mi * ri;
@@ -2627,9 +2760,9 @@ struct lfgpu_batch {
   BProb *probs = nullptr;
   BState *st = nullptr;
   double *part = nullptr;
-  BState *h_st = nullptr;  // pinned
+  BState *h_st = nullptr;  // pinned, two slots of nprob states
   BMState *mst = nullptr;    // mixed-precision states (allocated on first use)
-  BMState *h_mst = nullptr;  // pinned
+  BMState *h_mst = nullptr;  // pinned, two slots of nprob states
   // one captured chunk of iterations per precision mode (index = mode), so
   // alternating modes replay their own graphs instead of re-capturing; a
   // slot is reused while every captured argument is unchanged
@@ -3064,6 +3197,7 @@ BMArgs<VT> batch_mixed_args(lfgpu_batch *bt, const double *b, double *x, double 
   a.boff = bt->boff;
   a.prow = bt->d_row_off;
   a.dinv32 = c->dinv32;
+  a.dinv16 = c->dinv16;
   a.dinv = c->dinv;
   a.x = x;
   a.xlo = c->xlo;
@@ -3104,7 +3238,7 @@ void batch_mixed_chunk(lfgpu_batch *bt, const BMArgs<VT> &a, int count) {
 // problem has stopped, then x += xlo.
 template <class VT>
 int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg_opts *opts,
-                    int check, int mode) {
+                    int check, int mode, int *slot, int *syncs) {
   hipStream_t s = bt->ctx->stream;
   const double delta2 =
       opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
@@ -3133,20 +3267,33 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
   if (use_graph)
     LFG_TRY(batch_capture(bt, mode, check, x, b, delta2, max_interval, false,
                           [&]() { batch_mixed_chunk<VT>(bt, a, check); }));
-  for (;;) {
-    if (use_graph) {
-      LFG_HIP(hipGraphLaunch(bt->slot[mode].exec, s));
-    } else {
-      batch_mixed_chunk<VT>(bt, a, check);
-      LFG_HIP(hipGetLastError());
-    }
-    LFG_HIP(hipMemcpyAsync(bt->h_mst, bt->mst, bt->nprob * sizeof(BMState),
-                           hipMemcpyDeviceToHost, s));
-    LFG_HIP(hipStreamSynchronize(s));
-    bool all = true;
-    for (int p = 0; p < bt->nprob && all; ++p) all = bt->h_mst[p].done != 0;
-    if (all) break;
-  }
+  std::vector<PollHist> hist(bt->nprob);
+  LFG_TRY(poll_chunks(
+      bt->ctx, check,
+      [&]() -> int {
+        if (use_graph) {
+          LFG_HIP(hipGraphLaunch(bt->slot[mode].exec, s));
+        } else {
+          batch_mixed_chunk<VT>(bt, a, check);
+          LFG_HIP(hipGetLastError());
+        }
+        return LFGPU_OK;
+      },
+      [&](int j) -> int {
+        LFG_HIP(hipMemcpyAsync(bt->h_mst + (size_t)j * bt->nprob, bt->mst,
+                               bt->nprob * sizeof(BMState), hipMemcpyDeviceToHost, s));
+        return LFGPU_OK;
+      },
+      [&](int j) -> double {
+        const BMState *h = bt->h_mst + (size_t)j * bt->nprob;
+        double est = -1.0;
+        for (int p = 0; p < bt->nprob; ++p)
+          if (!h[p].done)
+            est = std::max(est, hist[p].next(h[p].rr, h[p].last_refine, h[p].iter, h[p].thr,
+                                             h[p].maxit));
+        return est;
+      },
+      slot, syncs));
   hipLaunchKernelGGL(k_bmix_final<VT>, G, B, 0, s, a);
   LFG_HIP(hipGetLastError());
   return LFGPU_OK;
@@ -3216,6 +3363,7 @@ int pcg_finish(lfgpu_ctx *ctx, const PcgState *hs, bool mixed, lfgpu_pcg_info *i
     info->refinements = mixed ? hs->refinements : 0;
     info->relres = relres;
     info->ms = ms;
+    info->syncs = 0;
   }
   if (!hs->converged) {
     char buf[64];
@@ -3282,7 +3430,26 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   const int nwg = (int)((n + kRB - 1) / kRB);
   if (rowsweep && !c->rpart)
     LFG_TRY(dalloc(&c->rpart, (size_t)3 * std::max(nwg, 1), "row-sweep partials"));
-  PcgState *hs = reinterpret_cast<PcgState *>(ctx->pinned);
+  // two pinned state slots: the iterations are polled one step behind
+  static_assert(2 * sizeof(PcgState) <= 4096, "pinned context buffer holds two states");
+  PcgState *hslot = reinterpret_cast<PcgState *>(ctx->pinned);
+  int slot = 0, syncs = 0;
+  PollHist hist;
+  auto poll = [&](auto &&launch) {
+    return poll_chunks(
+        ctx, check, launch,
+        [&](int j) -> int {
+          LFG_HIP(hipMemcpyAsync(hslot + j, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+          return LFGPU_OK;
+        },
+        [&](int j) -> double {
+          const PcgState &h = hslot[j];
+          if (h.done) return -1.0;
+          return mixed ? hist.next(h.rr, h.last_refine, h.iter, h.thr, h.maxit)
+                       : hist.next(h.rr, h.iter, h.iter, h.thr, h.maxit);
+        },
+        &slot, &syncs);
+  };
   PcgState h{};
   h.thr = opts->rtol * opts->rtol;
   h.maxit = opts->maxit;
@@ -3306,7 +3473,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
       const hipError_t e = hipGetLastError();
       if (e != hipSuccess) return hip_fail(e, "PCG solution scatter");
     }
-    return pcg_finish(ctx, hs, mixed_mode, info);
+    const int rc = pcg_finish(ctx, hslot + slot, mixed_mode, info);
+    if (info) info->syncs = syncs;
+    return rc;
   };
 
   const MatView mv = view_of(c, n);
@@ -3335,29 +3504,27 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
     hipLaunchKernelGGL(k_cg1_init_finish<kSB64>, dim3(1), B, 0, s, a1);
     hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, xd, n, c->st);
     LFG_HIP(hipGetLastError());
-    for (;;) {
+    LFG_TRY(poll([&]() -> int {
       if (c->fmt == FMT_DIA)
         launch_cg1_chunk<FMT_DIA>(s, sw, a1, check);
       else
         launch_cg1_chunk<FMT_ELL>(s, sw, a1, check);
       LFG_HIP(hipGetLastError());
-      LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-      LFG_HIP(hipStreamSynchronize(s));
-      if (hs->done) break;
-    }
+      return LFGPU_OK;
+    }));
     return finish(false);
   }
   if (rowsweep && mixed) {
     const double d2 = opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
     const int mi = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
     if (opts->mode == LFGPU_PCG_MIXED) {
-      MixArgs<float> m{mv, c->dinv32, c->dinv, xd, c->xlo, bd, c->r, c->r32, c->p32,
+      MixArgs<float> m{mv, c->dinv32, c->dinv16, c->dinv, xd, c->xlo, bd, c->r, c->r32, c->p32,
                        c->q32, c->xa32, c->part, c->st, sw, d2, mi, c->rpart, nwg};
-      LFG_TRY(run_mixed_rows(s, c, m, n, check, hs));
+      LFG_TRY(run_mixed_rows(s, c, m, n, check, poll));
     } else {
-      MixArgs<double> m{mv, c->dinv32, c->dinv, xd, c->xlo, bd, c->r64, c->r, c->p,
+      MixArgs<double> m{mv, c->dinv32, c->dinv16, c->dinv, xd, c->xlo, bd, c->r64, c->r, c->p,
                         c->q, c->xa64, c->part, c->st, sw, d2, mi, c->rpart, nwg};
-      LFG_TRY(run_mixed_rows(s, c, m, n, check, hs));
+      LFG_TRY(run_mixed_rows(s, c, m, n, check, poll));
     }
     return finish(true);
   }
@@ -3367,12 +3534,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
     hipLaunchKernelGGL(k_cgr_red_init, dim3(1), dim3(kRedB), 0, s, ra);
     hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, xd, n, c->st);
     LFG_HIP(hipGetLastError());
-    for (;;) {
-      LFG_TRY(run_fp64_chunk(c, s, CG_ROWS_ELL, CgArgs{}, ra, check, false, use_graph));
-      LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-      LFG_HIP(hipStreamSynchronize(s));
-      if (hs->done) break;
-    }
+    LFG_TRY(poll([&]() -> int {
+      return run_fp64_chunk(c, s, CG_ROWS_ELL, CgArgs{}, ra, check, false, use_graph);
+    }));
     return finish(false);
   }
   // the setup sweep leaves r = b - A x in the buffer the chosen mode reads
@@ -3401,26 +3565,22 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
       opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
   const int max_interval = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
   if (opts->mode == LFGPU_PCG_MIXED) {
-    MixArgs<float> m{mv, c->dinv32, c->dinv, xd, c->xlo, bd, c->r, c->r32, c->p32, c->q32,
+    MixArgs<float> m{mv, c->dinv32, c->dinv16, c->dinv, xd, c->xlo, bd, c->r, c->r32, c->p32, c->q32,
                      c->xa32, c->part, c->st, sw, delta2, max_interval};
-    LFG_TRY(run_mixed(s, c, sw, m, check, hs));
+    LFG_TRY(run_mixed(s, c, sw, m, check, poll));
   } else if (opts->mode == LFGPU_PCG_MIXED_MATRIX) {
     // FP64 vectors: p, q and the increment accumulator reuse the FP64
     // work vectors; r64 needs its own buffer next to the recurrence r.
-    MixArgs<double> m{mv, c->dinv32, c->dinv, xd, c->xlo, bd, c->r64, c->r, c->p, c->q,
+    MixArgs<double> m{mv, c->dinv32, c->dinv16, c->dinv, xd, c->xlo, bd, c->r64, c->r, c->p, c->q,
                       c->xa64, c->part, c->st, sw, delta2, max_interval};
-    LFG_TRY(run_mixed(s, c, sw, m, check, hs));
+    LFG_TRY(run_mixed(s, c, sw, m, check, poll));
   } else {
-    for (;;) {
+    LFG_TRY(poll([&]() -> int {
       if (c->fmt == FMT_DIA)
-        LFG_TRY(run_fp64_chunk(c, s, wide ? CG_CHUNK_DIA_WIDE : CG_CHUNK_DIA, a, RowArgs{}, check,
-                               c->sym64, use_graph));
-      else
-        LFG_TRY(run_fp64_chunk(c, s, CG_CHUNK_ELL, a, RowArgs{}, check, false, use_graph));
-      LFG_HIP(hipMemcpyAsync(hs, c->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-      LFG_HIP(hipStreamSynchronize(s));
-      if (hs->done) break;
-    }
+        return run_fp64_chunk(c, s, wide ? CG_CHUNK_DIA_WIDE : CG_CHUNK_DIA, a, RowArgs{}, check,
+                              c->sym64, use_graph);
+      return run_fp64_chunk(c, s, CG_CHUNK_ELL, a, RowArgs{}, check, false, use_graph);
+    }));
   }
   return finish(mixed);
 }
@@ -3441,7 +3601,8 @@ extern "C" int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
   bt->row_off.assign(row_offsets, row_offsets + nprob + 1);
   int rc = dalloc(&bt->st, (size_t)nprob, "batch states");
   if (rc == LFGPU_OK) {
-    hipError_t e = hipHostMalloc(&bt->h_st, nprob * sizeof(BState), hipHostMallocDefault);
+    // two slots: the solve polls one step behind the device
+    hipError_t e = hipHostMalloc(&bt->h_st, 2 * nprob * sizeof(BState), hipHostMallocDefault);
     if (e != hipSuccess) rc = hip_fail(e, "pinned batch states");
   }
   if (rc == LFGPU_OK) rc = dalloc(&bt->d_row_off, (size_t)nprob + 1, "batch row offsets");
@@ -3483,7 +3644,8 @@ extern "C" int lfgpu_batch_storage(const lfgpu_batch *bt, int *storage) {
   *storage = bt->fmt < 0 ? -1
              : bt->fmt == FMT_BDIA ? LFGPU_STORAGE_BLOCK_DIA
              : bt->fmt == FMT_DIA  ? LFGPU_STORAGE_DIA
-                                   : LFGPU_STORAGE_ELL;
+             : bt->A->cache && bt->A->cache->s_dcol ? LFGPU_STORAGE_ELL16
+                                                    : LFGPU_STORAGE_ELL;
   return LFGPU_OK;
 }
 
@@ -3527,15 +3689,16 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
   if (bt->rows_per_group == 0) LFG_TRY(batch_tune(bt, b->d));
   if (mixed && !bt->mst) {
     LFG_TRY(dalloc(&bt->mst, (size_t)bt->nprob, "batch mixed states"));
-    LFG_HIP(hipHostMalloc(&bt->h_mst, bt->nprob * sizeof(BMState), hipHostMallocDefault));
+    LFG_HIP(hipHostMalloc(&bt->h_mst, 2 * bt->nprob * sizeof(BMState), hipHostMallocDefault));
   }
   int check = opts->check_every > 0 ? opts->check_every : 64;
   check += check & 1;
+  int slot = 0, syncs = 0;
   LFG_HIP(hipEventRecord(ctx->ev0, s));
   if (opts->mode == LFGPU_PCG_MIXED) {
-    LFG_TRY(batch_run_mixed<float>(bt, b->d, x->d, opts, check, opts->mode));
+    LFG_TRY(batch_run_mixed<float>(bt, b->d, x->d, opts, check, opts->mode, &slot, &syncs));
   } else if (opts->mode == LFGPU_PCG_MIXED_MATRIX) {
-    LFG_TRY(batch_run_mixed<double>(bt, b->d, x->d, opts, check, opts->mode));
+    LFG_TRY(batch_run_mixed<double>(bt, b->d, x->d, opts, check, opts->mode, &slot, &syncs));
   } else {
     LFG_TRY(batch_init(bt, b->d, x->d, opts->rtol, opts->maxit));
     const BArgs a = batch_args(bt, x->d);
@@ -3548,20 +3711,32 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
     if (use_graph)
       LFG_TRY(batch_capture(bt, LFGPU_PCG_FP64, check, x->d, nullptr, 0.0, 0, bt->bsym,
                             [&]() { batch_chunk(bt, a, check); }));
-    for (;;) {
-      if (use_graph) {
-        LFG_HIP(hipGraphLaunch(bt->slot[LFGPU_PCG_FP64].exec, s));
-      } else {
-        batch_chunk(bt, a, check);
-        LFG_HIP(hipGetLastError());
-      }
-      LFG_HIP(hipMemcpyAsync(bt->h_st, bt->st, bt->nprob * sizeof(BState),
-                             hipMemcpyDeviceToHost, s));
-      LFG_HIP(hipStreamSynchronize(s));
-      bool all = true;
-      for (int p = 0; p < bt->nprob && all; ++p) all = bt->h_st[p].done != 0;
-      if (all) break;
-    }
+    std::vector<PollHist> hist(bt->nprob);
+    LFG_TRY(poll_chunks(
+        ctx, check,
+        [&]() -> int {
+          if (use_graph) {
+            LFG_HIP(hipGraphLaunch(bt->slot[LFGPU_PCG_FP64].exec, s));
+          } else {
+            batch_chunk(bt, a, check);
+            LFG_HIP(hipGetLastError());
+          }
+          return LFGPU_OK;
+        },
+        [&](int j) -> int {
+          LFG_HIP(hipMemcpyAsync(bt->h_st + (size_t)j * bt->nprob, bt->st,
+                                 bt->nprob * sizeof(BState), hipMemcpyDeviceToHost, s));
+          return LFGPU_OK;
+        },
+        [&](int j) -> double {
+          const BState *h = bt->h_st + (size_t)j * bt->nprob;
+          double est = -1.0;
+          for (int p = 0; p < bt->nprob; ++p)
+            if (!h[p].done)
+              est = std::max(est, hist[p].next(h[p].rr, h[p].iter, h[p].iter, h[p].thr, h[p].maxit));
+          return est;
+        },
+        &slot, &syncs));
   }
   LFG_HIP(hipEventRecord(ctx->ev1, s));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
@@ -3574,10 +3749,10 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
     double bb, rr;
     int iter, converged, refinements = 0;
     if (mixed) {
-      const BMState &h = bt->h_mst[p];
+      const BMState &h = bt->h_mst[(size_t)slot * bt->nprob + p];
       bb = h.bb, rr = h.rr, iter = h.iter, converged = h.converged, refinements = h.refinements;
     } else {
-      const BState &h = bt->h_st[p];
+      const BState &h = bt->h_st[(size_t)slot * bt->nprob + p];
       bb = h.bb, rr = h.rr, iter = h.iter, converged = h.converged;
     }
     const double rel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
@@ -3587,6 +3762,7 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
       infos[p].refinements = refinements;
       infos[p].relres = rel;
       infos[p].ms = ms;
+      infos[p].syncs = syncs;
     }
     conv &= converged;
     maxit_seen = std::max(maxit_seen, iter);
@@ -3598,6 +3774,7 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
     summary->refinements = bt->rows_per_group;
     summary->relres = worst;
     summary->ms = ms;
+    summary->syncs = syncs;
   }
   if (!conv) {
     set_error("batched PCG: some problems did not reach rtol within maxit");
@@ -3609,7 +3786,10 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
 extern "C" int lfgpu_solver_storage(const lfgpu_csr *A, int *storage, int *ndiag, int *sym) {
   LFG_REQUIRE(A && storage, LFGPU_EINVAL, "NULL argument to lfgpu_solver_storage");
   const lfgpu_solver_cache *c = A->cache;
-  *storage = !c ? -1 : c->fmt == FMT_DIA ? LFGPU_STORAGE_DIA : LFGPU_STORAGE_ELL;
+  *storage = !c                  ? -1
+             : c->fmt == FMT_DIA ? LFGPU_STORAGE_DIA
+             : c->s_dcol         ? LFGPU_STORAGE_ELL16
+                                 : LFGPU_STORAGE_ELL;
   if (ndiag) *ndiag = c ? c->ndiag : 0;
   if (sym) *sym = c && c->sym64 ? 1 : 0;
   return LFGPU_OK;
@@ -3769,14 +3949,18 @@ extern "C" int lfgpu_pcg_noalloc(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu
   if (n > 0) hipLaunchKernelGGL(k_zero_if_bb0, dim3(gflat), BF, 0, s, x->d, n, st);
   LFG_HIP(hipGetLastError());
   PcgState *hs = reinterpret_cast<PcgState *>(ctx->pinned);
+  int syncs = 0;
   for (;;) {
     launch_cg_chunk<FMT_CSR, kSB64>(s, sw, a, check, false);
     LFG_HIP(hipGetLastError());
     LFG_HIP(hipMemcpyAsync(hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     LFG_HIP(hipStreamSynchronize(s));
+    ++syncs;
     if (hs->done) break;
   }
-  return pcg_finish(ctx, hs, false, info);
+  const int rc = pcg_finish(ctx, hs, false, info);
+  if (info) info->syncs = syncs;
+  return rc;
 }
 
 namespace lfgpu {

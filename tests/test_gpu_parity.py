@@ -492,6 +492,52 @@ def test_pcg_graph_replay_matches_plain_launches(ctx, degree):
         h.close()
 
 
+@pytest.mark.parametrize("degree", [1, 3])
+def test_polling_does_not_change_the_iterates(ctx, degree):
+    # The host polls the solver state one step behind the device and queues
+    # several chunks per poll once the residual history allows an estimate.
+    # Chunk length, steps and the empty chunk after the stop must leave the
+    # iterates and the iteration count unchanged in every mode (single and
+    # batched), and the number of host waits must fall with the estimate.
+    if degree == 1:
+        m, g, fe, plan, va, b = _c1_system(
+            ctx, 40, np.exp(np.random.default_rng(7).standard_normal(2 * 40 * 40)), 1.0)
+        nd = m.nv
+    else:
+        m, nd, g, fe, plan, va, b = _p3_system(ctx, 8)
+    x = capi.Vec(ctx, nd)
+    for mode in (capi.PCG_FP64, capi.PCG_MIXED, capi.PCG_MIXED_MATRIX):
+        runs = []
+        for check, graph in ((2, -1), (8, -1), (64, -1), (8, 1)):
+            x.fill(0.0)
+            info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, mode=mode, check_every=check,
+                            graph=graph if mode == capi.PCG_FP64 else 0)
+            assert info.converged
+            runs.append((info.iters, info.syncs, x.numpy()))
+        for it, _, xv in runs[1:]:
+            assert it == runs[0][0] and np.array_equal(xv, runs[0][2]), mode
+        # two iterations per chunk: far fewer waits than chunks once the
+        # history gives an estimate (FP64: every poll; mixed: per refinement)
+        assert 1 <= runs[0][1] < (runs[0][0] // 2 + 2), (mode, runs[0][:2])
+        if mode == capi.PCG_FP64:
+            assert runs[0][1] <= runs[0][0] // 6, runs[0][:2]
+    # batched: two copies of the problem, the same iterates as the single solve
+    x.fill(0.0)
+    capi.pcg(ctx, plan.csr, b, x, rtol=1e-10, check_every=64)
+    x_single = x.numpy()
+    batch = capi.Batch(ctx, plan.csr, np.array([0, nd], np.int32))
+    for check in (2, 64):
+        x.fill(0.0)
+        summ, infos = batch.pcg(b, x, rtol=1e-10, check_every=check)
+        assert summ.converged and summ.syncs >= 1
+        assert np.linalg.norm(x.numpy() - x_single) <= 1e-12 * np.linalg.norm(x_single)
+        if check == 2:
+            assert summ.syncs <= summ.iters // 6, (summ.syncs, summ.iters)
+    batch.close()
+    for h in (x, va, b, plan, fe, g):
+        h.close()
+
+
 # ------------------------------------------------------------ batched solves
 def test_batched_pcg_matches_per_problem_solves(ctx):
     sizes = [9, 17, 5, 31, 12, 24, 40, 8]
@@ -669,7 +715,8 @@ def test_batched_block_dia_storage_gives_the_ell_iterates(ctx, mode):
         its[-1].append(xs_storage)
         batch.close()
         x.close()
-    assert its[0][-1] == capi.STORAGE_BLOCK_DIA and its[1][-1] == capi.STORAGE_ELL
+    # (fewer than 32768 rows: the ELL columns are 16-bit offsets)
+    assert its[0][-1] == capi.STORAGE_BLOCK_DIA and its[1][-1] == capi.STORAGE_ELL16
     assert its[0][:-1] == its[1][:-1]
     assert np.array_equal(xs[0], xs[1])
     # and the solutions solve the systems
@@ -779,13 +826,14 @@ def test_ell_locality_order_single_and_batched_solves(ctx, degree):
             batch = capi.Batch(ctx, plan.csr, np.array([0, nd], np.int32))
             x.fill(0.0)
             summ, _ = batch.pcg(b, x, rtol=1e-12)
-            assert summ.converged and batch.storage() == capi.STORAGE_ELL
+            # fewer than 32768 rows: the unpermuted storage has 16-bit offsets too
+            assert summ.converged and batch.storage() == capi.STORAGE_ELL16
             batch.close()
         else:
             x.fill(0.0)
             info = capi.pcg(ctx, plan.csr, b, x, rtol=1e-12)
             assert info.converged
-            assert capi.solver_storage(plan.csr)[0] == capi.STORAGE_ELL
+            assert capi.solver_storage(plan.csr)[0] == capi.STORAGE_ELL16
         sols.append(x.numpy())
         assert np.linalg.norm(sols[-1] - xd) <= 1e-8 * np.linalg.norm(xd), stage
         x.upload(xv)
@@ -795,6 +843,43 @@ def test_ell_locality_order_single_and_batched_solves(ctx, degree):
         assert np.all(np.abs(y.numpy() - yo) <= 1e-14 * scale + 1e-300), stage
     for h in (y, x, b, va, plan, fe, g):
         h.close()
+
+
+def test_ell_column_offsets_out_of_16_bit_range(ctx):
+    # A chain of 70000 unknowns closed into a ring: rows 0 and n-1 couple,
+    # so no diagonal set covers the matrix (sliced ELL), and in the locality
+    # order row n-1 (smallest column 0) moves next to row 0 while row n-2
+    # stays at the end: one stored column lies ~70000 rows from its row and
+    # the storage keeps 32-bit columns.  The same chain without the ring
+    # coupling but with a coupling inside 16-bit range (rows 0 and 20000)
+    # takes 16-bit offsets.  Products and solutions match the oracle's.
+    n = 70000
+    rng = np.random.default_rng(41)
+    for far, expect in ((n - 1, capi.STORAGE_ELL), (20000, capi.STORAGE_ELL16)):
+        d = 3.5 + rng.random(n)  # diagonally dominant: SPD
+        e = -(0.25 + rng.random(n - 1))
+        rows = np.concatenate([np.arange(n), np.arange(n - 1), np.arange(1, n), [0, far]])
+        cols = np.concatenate([np.arange(n), np.arange(1, n), np.arange(n - 1), [far, 0]])
+        vals = np.concatenate([d, e, e, [-0.25, -0.25]])
+        A = capi.coo_to_csr(ctx, rows.astype(np.int32), cols.astype(np.int32), vals, n, n)
+        rp, col, val = A.download()
+        bh = rng.standard_normal(n)
+        b, x, y = capi.Vec(ctx, bh), capi.Vec(ctx, n), capi.Vec(ctx, n)
+        xv = rng.standard_normal(n)
+        x.upload(xv)
+        capi.spmv(ctx, A, x, y)
+        assert capi.solver_storage(A)[0] == expect
+        yo = L.csr_spmv(rp, col, val, xv)
+        scale = L.csr_spmv(rp, col, np.abs(val), np.abs(xv))
+        assert np.all(np.abs(y.numpy() - yo) <= 1e-14 * scale)
+        xo, _, _ = L.pcg(rp, col, val, bh, tol=1e-13)
+        for mode in (capi.PCG_FP64, capi.PCG_MIXED, capi.PCG_MIXED_MATRIX):
+            x.fill(0.0)
+            info = capi.pcg(ctx, A, b, x, rtol=1e-12, mode=mode)
+            assert info.converged, (far, mode)
+            assert np.linalg.norm(x.numpy() - xo) <= 1e-10 * np.linalg.norm(xo), (far, mode)
+        for h in (y, x, b, A):
+            h.close()
 
 
 # ---------------------------------------------- adjoint gradient end to end
