@@ -729,9 +729,36 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
             "pcg_ms": float(np.median([t[2] for t in times])), "iters_max": int(max(it)),
             "iters_min": int(min(it)), "iters_sum": int(sum(it)),
             "refinements_sum": int(sum(i.refinements for i in infos)),
-            "converged": bool(summ.converged), "worst_relres": summ.relres}
-        log("batched %s: %.1f ms/step (pcg %.1f ms), iters %d..%d"
-            % (name, 1e3 * wall, per_mode[name]["pcg_ms"], min(it), max(it)))
+            "converged": bool(summ.converged), "worst_relres": summ.relres,
+            "host_syncs": int(summ.syncs), "path": "separate calls"}
+        log("batched %s: %.1f ms/step (pcg %.1f ms), iters %d..%d, %d host syncs"
+            % (name, 1e3 * wall, per_mode[name]["pcg_ms"], min(it), max(it), summ.syncs))
+        if args.batch_graph == 0:
+            # the step as one captured graph (assembly, value refresh, set-up,
+            # first chunk) plus the captured chunks: lfgpu_batch_assemble_pcg
+            x.fill(0.0)
+            batch.assemble_pcg(plan, *acoef, x, rtol=1e-10, maxit=MAXIT, mode=md)  # captures
+            ftimes = []
+            for _ in range(max(1, args.batch_steps)):
+                ctx.sync()
+                t0 = time.perf_counter()
+                x.fill(0.0)
+                fs, finfos = batch.assemble_pcg(plan, *acoef, x, rtol=1e-10, maxit=MAXIT, mode=md)
+                ctx.sync()
+                ftimes.append((time.perf_counter() - t0, fs.ms))
+            fwall = float(np.median([t[0] for t in ftimes]))
+            same = bool(np.array_equal(x.numpy(), sols[name]) and
+                        [i.iters for i in finfos] == it)
+            sep = {k: per_mode[name][k] for k in ("melem_s", "ms_per_step", "host_syncs")}
+            per_mode[name].update({
+                "melem_s": ne / fwall / 1e6, "ms_per_step": 1e3 * fwall,
+                "device_ms": float(np.median([t[1] for t in ftimes])),
+                "host_syncs": int(fs.syncs), "path": "one captured graph per step "
+                "(lfgpu_batch_assemble_pcg) and the captured chunks",
+                "bitwise_equal_to_separate_calls": same, "separate_calls": sep})
+            log("batched %s one graph: %.1f ms/step (device %.1f ms), %d host syncs, "
+                "bitwise equal to the separate calls: %s"
+                % (name, 1e3 * fwall, per_mode[name]["device_ms"], fs.syncs, same))
     plain = (None if args.no_plain else
              plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff))
     sel = per_mode[args.batch_mode if args.batch_mode in per_mode else next(iter(per_mode))]

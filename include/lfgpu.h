@@ -288,7 +288,10 @@ typedef struct lfgpu_pcg_opts {
   double rtol;     /* relative residual target                    */
   int maxit;       /* iteration cap (Eigen: 2*n by default)       */
   int mode;        /* LFGPU_PCG_FP64 or LFGPU_PCG_MIXED            */
-  int check_every; /* iterations between host convergence checks */
+  int check_every; /* iterations per chunk; the host polls the state
+                    * one step behind the device, every 1 to 8 chunks
+                    * (sized from the residual history); chunks after
+                    * the stop run empty                             */
   /* mixed mode only (0 selects the default):                        */
   double refine_delta;     /* reliable update once ||r|| fell by this  */
                            /* factor since the last one (default 0.3)  */
@@ -393,6 +396,23 @@ int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec *b,
  * (chosen when every stored column lies within 32767 rows of its row) */
 #define LFGPU_STORAGE_ELL16 3
 int lfgpu_batch_storage(const lfgpu_batch *bt, int *storage);
+/* lfgpu_assemble_galerkin of plan p (without prescribed Dirichlet values)
+ * followed by lfgpu_batch_pcg on the batch created over p's matrix, with the
+ * same results bit for bit.  The first call with a set of arguments runs the
+ * two calls and captures a graph holding the assembly, the refresh of the
+ * solver's copy of the values, the solve's set-up and its first chunk of
+ * iterations; later calls with the same arguments (handles, coefficient
+ * descriptors, vectors and options; the coefficient data may change) replay
+ * it as one launch and continue with the captured chunks, polled as in
+ * lfgpu_batch_pcg (summary->syncs).  summary->ms covers assembly and solve.
+ * The batched form of the north-star pipeline (assemble, then solve, per
+ * problem; no single reference counterpart: AssembleMatrixLocally +
+ * AssembleVectorLocally + the solve of homDir_linfe_demo.cc:130-174). */
+int lfgpu_batch_assemble_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, lfgpu_plan *p,
+                             const lfgpu_fe *fe, const lfgpu_coeff *alpha,
+                             const lfgpu_coeff *gamma, const lfgpu_coeff *f,
+                             lfgpu_vec *rhs, lfgpu_vec *x, const lfgpu_pcg_opts *opts,
+                             lfgpu_pcg_info *infos, lfgpu_pcg_info *summary);
 
 /* ---- error norms ---------------------------------------------------------
  * Squared L2 and H1-seminorm errors of the finite element function with DOF

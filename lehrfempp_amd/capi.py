@@ -113,6 +113,9 @@ SIGNATURES = {
     "lfgpu_batch_create": (C.c_int, [_P, _P, C.c_int, _i32, C.POINTER(_P)]),
     "lfgpu_batch_destroy": (C.c_int, [_P]),
     "lfgpu_batch_storage": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "lfgpu_batch_assemble_pcg": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                           C.POINTER(PcgOpts), C.POINTER(PcgInfo),
+                                           C.POINTER(PcgInfo)]),
     "lfgpu_batch_pcg": (C.c_int, [_P, _P, _P, _P, C.POINTER(PcgOpts), C.POINTER(PcgInfo),
                                   C.POINTER(PcgInfo)]),
     "lfgpu_error_norms": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P, C.POINTER(C.c_double),
@@ -506,6 +509,23 @@ class Batch:
         summ = PcgInfo()
         rc = lib().lfgpu_batch_pcg(self.ctx.h, self.h, b.h, x.h, C.byref(opts), infos,
                                    C.byref(summ))
+        if rc != OK and (rc != ENOCONV or raise_on_noconv):
+            check(rc)
+        return summ, list(infos)
+
+    def assemble_pcg(self, plan, fe, alpha, gamma, f, rhs: Vec, x: Vec, rtol=1e-10,
+                     maxit=100000, check_every=64, raise_on_noconv=True, mode=PCG_FP64,
+                     refine_delta=0.0, refine_max_interval=0):
+        """lfgpu_batch_assemble_pcg: plan.assemble(fe, alpha, gamma, f, rhs) and
+        pcg(rhs, x), replayed as one captured graph from the second call with
+        the same arguments on."""
+        opts = PcgOpts(rtol, int(maxit), mode, check_every, refine_delta, refine_max_interval, 0,
+                       0)
+        infos = (PcgInfo * self.nprob)()
+        summ = PcgInfo()
+        rc = lib().lfgpu_batch_assemble_pcg(
+            self.ctx.h, self.h, plan.h, fe.h, C.byref(alpha), C.byref(gamma),
+            None if f is None else C.byref(f), rhs.h, x.h, C.byref(opts), infos, C.byref(summ))
         if rc != OK and (rc != ENOCONV or raise_on_noconv):
             check(rc)
         return summ, list(infos)

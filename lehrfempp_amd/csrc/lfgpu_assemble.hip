@@ -162,6 +162,7 @@ extern "C" int lfgpu_plan_destroy(lfgpu_plan *p) {
   dfree(p->fixed);
   dfree(p->ke);
   dfree(p->fe);
+  dfree(p->coef);
   plan_free_edges(p);
   delete p;
   return LFGPU_OK;
@@ -490,6 +491,59 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
   for (int t = threadIdx.x; t < cnt; t += BLOCK) a.val[base + t] = s_acc[t];
 }
 
+// Coefficients whose value costs transcendental functions (EXPCELL: exp per
+// cell; SINSIN: two sines per point) are evaluated once per cell, with the
+// same expressions as coeff_qp, into out (EXPCELL: one value per cell;
+// SINSIN: NQP per cell).  The row-owner kernel then reads them as a CELL /
+// QP coefficient of value 1, and 1 * v == v, so every value it uses is the
+// one it would have computed itself, for each of the cell's local rows.
+template <int NQP>
+__global__ void k_coeff_cells(CoeffD c, const double *xy, const int32_t *cell_v, int64_t ncells,
+                              const double *tab, double *out) {
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= ncells) return;
+  if (c.kind == LFGPU_COEFF_EXPCELL) {
+    out[cell] = c.value * exp(c.data[cell]);
+    return;
+  }
+  const CellXY p = load_cell(xy, cell_v, cell);
+  double v[NQP];
+  coeff_qp<NQP>(c, cell, p, tab, v);
+#pragma unroll
+  for (int k = 0; k < NQP; ++k) out[cell * NQP + k] = v[k];
+}
+
+bool costly(const CoeffD &c) {
+  return c.kind == LFGPU_COEFF_EXPCELL || c.kind == LFGPU_COEFF_SINSIN;
+}
+
+// Queues the per-cell evaluation of the costly coefficients of a into the
+// plan's buffer and points a at it.
+template <int NQP>
+int precompute_coeffs(hipStream_t s, lfgpu_plan *p, AsmArgs &a) {
+  CoeffD *cs[3] = {&a.alpha, &a.gamma, &a.f};
+  const int64_t nc = p->mesh->ncells;
+  const int64_t len = 3 * std::max<int64_t>(nc * NQP, 1);
+  if (!costly(a.alpha) && !costly(a.gamma) && !(a.has_f && costly(a.f))) return LFGPU_OK;
+  if (p->coef_len < len) {
+    dfree(p->coef);
+    p->coef = nullptr;
+    p->coef_len = 0;
+    LFG_TRY(dalloc(&p->coef, (size_t)len, "per-cell coefficient values"));
+    p->coef_len = len;
+  }
+  for (int q = 0; q < 3; ++q) {
+    CoeffD &c = *cs[q];
+    if (!costly(c) || (q == 2 && !a.has_f) || nc == 0) continue;
+    double *out = p->coef + q * (len / 3);
+    hipLaunchKernelGGL(k_coeff_cells<NQP>, dim3(grid_for(nc, 256)), dim3(256), 0, s, c, a.xy,
+                       a.cell_v, nc, a.tab, out);
+    c = CoeffD{c.kind == LFGPU_COEFF_EXPCELL ? LFGPU_COEFF_CELL : LFGPU_COEFF_QP, 1.0, out};
+  }
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
+}
+
 template <int NLOC, int NQP, int MS, bool EDGES>
 int launch_assemble(hipStream_t s, const lfgpu_plan *p, const AsmArgs &a) {
   const size_t lds = (size_t)std::max(p->max_block_nnz, 1) * sizeof(double);
@@ -681,13 +735,25 @@ int asm_args(lfgpu_plan *p, const lfgpu_fe *fe, const lfgpu_coeff *alpha, const 
   return LFGPU_OK;
 }
 
+#ifndef LFGPU_COEF_PRE
+#define LFGPU_COEF_PRE 1
+#endif
+
+// the per-cell coefficient pass, then the row-owner kernel
+template <bool EDGES>
+int queue_assembly(hipStream_t s, lfgpu_plan *p, AsmArgs a) {
+  if (p->mesh->nloc == 3) {
+    if (LFGPU_COEF_PRE) LFG_TRY(precompute_coeffs<3>(s, p, a));
+    return launch_assemble<3, 3, 4, EDGES>(s, p, a);
+  }
+  if (LFGPU_COEF_PRE) LFG_TRY(precompute_coeffs<12>(s, p, a));
+  return launch_assemble<10, 12, 12, EDGES>(s, p, a);
+}
+
 template <bool EDGES>
 int run_assembly(lfgpu_ctx *ctx, lfgpu_plan *p, const AsmArgs &a) {
   LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-  if (p->mesh->nloc == 3)
-    LFG_TRY((launch_assemble<3, 3, 4, EDGES>(ctx->stream, p, a)));
-  else
-    LFG_TRY((launch_assemble<10, 12, 12, EDGES>(ctx->stream, p, a)));
+  LFG_TRY(queue_assembly<EDGES>(ctx->stream, p, a));
   LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0.f;
@@ -698,6 +764,16 @@ int run_assembly(lfgpu_ctx *ctx, lfgpu_plan *p, const AsmArgs &a) {
 }
 
 }  // namespace
+
+namespace lfgpu {
+int assemble_enqueue(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
+                     const lfgpu_coeff *alpha, const lfgpu_coeff *gamma, const lfgpu_coeff *f,
+                     lfgpu_vec *rhs) {
+  AsmArgs a;
+  LFG_TRY(asm_args(p, fe, alpha, gamma, f, nullptr, rhs, a));
+  return queue_assembly<false>(ctx->stream, p, a);
+}
+}  // namespace lfgpu
 
 extern "C" int lfgpu_assemble_galerkin(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
                                        const lfgpu_coeff *alpha, const lfgpu_coeff *gamma,

@@ -140,6 +140,43 @@ __device__ __forceinline__ void element_row(const Tab<NLOC, NQP> &T, const Geo &
   }
 }
 
+// P1: the reference gradients of the linear Lagrange functions are the same
+// at every quadrature point (the table holds equal values), so each
+// G_j = J^{-T} grad N_j is formed once, with the products of the general
+// loop, and reused at every point: the same values, one third of the work.
+template <>
+__device__ __forceinline__ void element_row<3, 3>(const Tab<3, 3> &T, const Geo &g,
+                                                  const double *ak, const double *gk, int i,
+                                                  double *row) {
+  double G0[3], G1[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double gxj = T.dx(j, 0), gyj = T.dy(j, 0);
+    G0[j] = g.jit00 * gxj + g.jit01 * gyj;
+    G1[j] = g.jit10 * gxj + g.jit11 * gyj;
+    row[j] = 0.0;
+  }
+  double Gi0 = G0[0], Gi1 = G1[0];
+#pragma unroll
+  for (int j = 1; j < 3; ++j)
+    if (i == j) {
+      Gi0 = G0[j];
+      Gi1 = G1[j];
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double w = T.w(k) * g.det;
+    const double a = ak[k], c = gk[k];
+    const double cNi = c * T.N(i, k);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double s = Gi0 * (a * G0[j]) + Gi1 * (a * G1[j]);
+      const double m = cNi * T.N(j, k);
+      row[j] += w * (s + m);
+    }
+  }
+}
+
 // Entry i of ScalarLoadElementVectorProvider::Eval (loc_comp_ellbvp.h:733-740).
 template <int NLOC, int NQP>
 __device__ __forceinline__ double element_load(const Tab<NLOC, NQP> &T, const Geo &g,

@@ -131,6 +131,10 @@ struct lfgpu_plan {
   int block = 256;             // rows per workgroup of the assembly kernel
   int max_block_nnz = 0;       // LDS doubles needed by one workgroup
   double *ke = nullptr;        // [ncells x nloc^2] element matrices (contraction path)
+  // coefficient values evaluated once per cell before the row-owner
+  // assembly (exp per cell, sin sin per point): three slots of ncells x nqp
+  double *coef = nullptr;
+  int64_t coef_len = 0;
   double *fe = nullptr;        // [ncells x nloc] element vectors (contraction path)
   // boundary edge terms (lfgpu_plan_set_edges)
   const lfgpu_edges *edges = nullptr;
@@ -168,6 +172,13 @@ inline void csr_values_changed(lfgpu_csr *A) { A->val_version++; }
 // y = A x (FP64) on the context stream, without synchronisation; builds the
 // solver's storage of A on first use (lfgpu_solver.hip).
 int spmv_async(lfgpu_ctx *ctx, lfgpu_csr *A, const double *x, double *y);
+// Queues the fused row-owner assembly of lfgpu_assemble_galerkin on the
+// context stream: no events, no synchronisation, no host bookkeeping of the
+// CSR's value version (so it can be captured into a graph;
+// lfgpu_assemble.hip).
+int assemble_enqueue(lfgpu_ctx *ctx, lfgpu_plan *p, const lfgpu_fe *fe,
+                     const lfgpu_coeff *alpha, const lfgpu_coeff *gamma, const lfgpu_coeff *f,
+                     lfgpu_vec *rhs);
 // Looks up this device's address of the allocation-free solver's work space
 // (lfgpu_solver.hip); called by lfgpu_ctx_create with the device current.
 int noalloc_scratch_init(int device);

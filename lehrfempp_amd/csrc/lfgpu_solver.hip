@@ -389,11 +389,26 @@ __global__ void k_fill_ell_cols(const int32_t *row_ptr, const int32_t *col, int3
   for (; k < w; ++k) s_col[off + k * kSlice + l] = (CT)(r - base);
 }
 
-// locality key of a row: its smallest column (the CSR columns are sorted)
+// first locality key of a row: its smallest column (the CSR columns are
+// sorted)
 __global__ void k_row_key(const int32_t *row_ptr, const int32_t *col, int32_t n, int32_t *key) {
   const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   key[r] = row_ptr[r + 1] > row_ptr[r] ? col[row_ptr[r]] : r;
+}
+
+// locality key of a row: the smallest first key among its columns (its own
+// included).  A row whose own smallest column is far from the mesh position
+// of its neighbours (a P3 cell DOF of a corner cell, all of whose vertices
+// carry Dirichlet values, couples only to edge DOFs) takes its neighbours'
+// key and lands next to them.
+__global__ void k_row_key2(const int32_t *row_ptr, const int32_t *col, int32_t n,
+                           const int32_t *key1, int32_t *key) {
+  const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int32_t k = key1[r];
+  for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p) k = min(k, key1[col[p]]);
+  key[r] = k;
 }
 
 __global__ void k_gather(const double *__restrict__ src, const int32_t *__restrict__ perm,
@@ -576,7 +591,8 @@ int detect_dia(lfgpu_ctx *ctx, const lfgpu_csr *A, DiagSet *ds) {
 }
 
 // Locality order of the rows of a sliced-ELL matrix.  Rows are ordered by
-// their smallest column in windows of 2^kEllWindowBits columns and, inside a
+// their locality key (k_row_key2: the smallest column among the row and its
+// neighbours' rows) in windows of 2^kEllWindowBits columns and, inside a
 // window, by their length (then by index).  On a finite element matrix whose
 // lowest DOFs (the vertices) are numbered along the mesh, the rows of every
 // DOF kind (vertex, edge, cell DOFs) near one part of the mesh end up next to
@@ -597,9 +613,11 @@ int ell_locality_order(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_solver_cache *c
   hipStream_t s = ctx->stream;
   const int32_t n = A->nrows;
   int32_t *dkey = nullptr;
-  LFG_TRY(dalloc(&dkey, (size_t)n, "row keys"));
+  LFG_TRY(dalloc(&dkey, 2 * (size_t)n, "row keys"));
   hipLaunchKernelGGL(k_row_key, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col, n,
-                     dkey);
+                     dkey + n);
+  hipLaunchKernelGGL(k_row_key2, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col, n,
+                     dkey + n, dkey);
   std::vector<int32_t> key(n), rp((size_t)n + 1), perm(n), iperm(n);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
@@ -624,6 +642,18 @@ int ell_locality_order(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_solver_cache *c
   LFG_HIP(hipMemcpyAsync(c->iperm, iperm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
   LFG_HIP(hipStreamSynchronize(s));
   return LFGPU_OK;
+}
+
+// Queues the refresh of the storage's values, row sums and inverse diagonal
+// (and their FP32 / bfloat16 copies) from A's current values.
+void fill_vals_enqueue(hipStream_t s, const lfgpu_csr *A, lfgpu_solver_cache *c) {
+  DiagSet ds{};
+  ds.ndiag = c->ndiag;
+  for (int d = 0; d < kMaxDiag; ++d) ds.off[d] = c->off[d];
+  hipLaunchKernelGGL(k_fill_vals, dim3(grid_for(A->nrows, 256)), dim3(256), 0, s, A->row_ptr,
+                     A->col, A->val, A->nrows, c->fmt, ds, c->slice_off, c->perm, c->val,
+                     c->have32 ? c->val32 : nullptr, c->rowsum, c->have32 ? c->rowsum32 : nullptr,
+                     c->dinv, c->have32 ? c->dinv32 : nullptr, c->have32 ? c->dinv16 : nullptr);
 }
 
 // Builds (or refreshes) the solver's storage of A.  allow_perm: a new
@@ -715,14 +745,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
     c->built_version = ~0ull;
   }
   if (c->built_version != A->val_version) {
-    DiagSet ds{};
-    ds.ndiag = c->ndiag;
-    for (int d = 0; d < kMaxDiag; ++d) ds.off[d] = c->off[d];
-    hipLaunchKernelGGL(k_fill_vals, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col,
-                       A->val, n, c->fmt, ds, c->slice_off, c->perm, c->val,
-                       c->have32 ? c->val32 : nullptr, c->rowsum,
-                       c->have32 ? c->rowsum32 : nullptr, c->dinv,
-                       c->have32 ? c->dinv32 : nullptr, c->have32 ? c->dinv16 : nullptr);
+    fill_vals_enqueue(s, A, c);
     LFG_HIP(hipGetLastError());
     LFG_TRY(dia_symmetry(ctx, c, n));
     c->built_version = A->val_version;
@@ -2294,6 +2317,7 @@ struct BArgs {
   BState *st;
   const int32_t *boff;  // FMT_BDIA: [nprob x kMaxDiag] offsets
   const int32_t *prow;  // FMT_BDIA: [nprob + 1] first rows
+  const int *symflag;   // FMT_BDIA: 0 when the values are mirror-symmetric bit for bit
 };
 
 // The matrix view of a workgroup of problem prob: FMT_BDIA takes that
@@ -2399,13 +2423,27 @@ __global__ __launch_bounds__(256) void k_bcg_init_finish(BArgs a) {
   }
 }
 
+// every problem's state before a solve (thr holds tol^2 until the set-up
+// scales it by ||b||^2); set on the device, so a captured solve needs no copy
+__global__ void k_bstate_init(BState *st, int nprob, double tol2, int maxit) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  BState v{};
+  v.thr = tol2;
+  v.maxit = maxit;
+  st[p] = v;
+}
+
 __global__ void k_bcg_zero_if_bb0(BArgs a) {
   const BGroup G = a.groups[blockIdx.x];
   if (a.st[G.prob].bb != 0.0) return;
   for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) a.x[i] = 0.0;
 }
 
-template <int FMT, bool SYM = false>
+// SYMMODE 0: every diagonal read; 1: upper diagonals only (values
+// mirror-symmetric bit for bit); 2: chosen on the device from *a.symflag,
+// for chunks captured before the values they run on are known
+template <int FMT, int SYMMODE = 0>
 __global__ __launch_bounds__(256) void k_bcg_spmv(BArgs a, int par) {
   __shared__ double sh[16];
   const BGroup G = a.groups[blockIdx.x];
@@ -2430,14 +2468,25 @@ __global__ __launch_bounds__(256) void k_bcg_spmv(BArgs a, int par) {
   }
   const MatView m = batch_view<FMT>(a.m, a.boff, a.prow, G.prob);
   double pq = 0.0, qdr = 0.0, qdq = 0.0;
-  for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
-    const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
-    const double qi = apply64<applied_fmt<FMT>(), true, SYM, true>(m, a.p, i, pi);
-    __builtin_nontemporal_store(qi, &a.q[i]);  // as in k_cg_spmv
-    const double dq = di * qi;
-    pq += pi * qi;
-    qdr += dq * ri;
-    qdq += dq * qi;
+  auto rows = [&](auto sym) {
+    for (int32_t i = G.r0 + threadIdx.x; i < G.r1; i += 256) {
+      const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
+      const double qi =
+          apply64<applied_fmt<FMT>(), true, decltype(sym)::value, true>(m, a.p, i, pi);
+      __builtin_nontemporal_store(qi, &a.q[i]);  // as in k_cg_spmv
+      const double dq = di * qi;
+      pq += pi * qi;
+      qdr += dq * ri;
+      qdq += dq * qi;
+    }
+  };
+  if constexpr (SYMMODE == 2) {
+    if (*a.symflag == 0)
+      rows(std::true_type{});
+    else
+      rows(std::false_type{});
+  } else {
+    rows(std::integral_constant<bool, SYMMODE == 1>{});
   }
   double v[3] = {pq, qdr, qdq};
   double *const sl[3] = {bslot(a, P_PQ), bslot(a, P_QDR), bslot(a, P_QDQ)};
@@ -2477,11 +2526,13 @@ __global__ __launch_bounds__(256) void k_bcg_update(BArgs a, int par) {
 }
 
 template <int FMT>
-void launch_bcg_chunk(hipStream_t s, const BArgs &a, int count, bool sym) {
+void launch_bcg_chunk(hipStream_t s, const BArgs &a, int count, int symmode) {
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    if (FMT == FMT_BDIA && sym)
-      hipLaunchKernelGGL((k_bcg_spmv<FMT, true>), dim3(a.ngroups), dim3(256), 0, s, a, par);
+    if (FMT == FMT_BDIA && symmode == 1)
+      hipLaunchKernelGGL((k_bcg_spmv<FMT, 1>), dim3(a.ngroups), dim3(256), 0, s, a, par);
+    else if (FMT == FMT_BDIA && symmode == 2)
+      hipLaunchKernelGGL((k_bcg_spmv<FMT, 2>), dim3(a.ngroups), dim3(256), 0, s, a, par);
     else
       hipLaunchKernelGGL(k_bcg_spmv<FMT>, dim3(a.ngroups), dim3(256), 0, s, a, par);
     hipLaunchKernelGGL(k_bcg_update, dim3(a.ngroups), dim3(256), 0, s, a, par);
@@ -2593,6 +2644,15 @@ __global__ __launch_bounds__(256) void k_bmix_init_finish(BMArgs<VT> a) {
       st->iter = 0;
     }
   }
+}
+
+__global__ void k_bmstate_init(BMState *st, int nprob, double tol2, int maxit) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  BMState v{};
+  v.thr = tol2;
+  v.maxit = maxit;
+  st[p] = v;
 }
 
 template <class VT>
@@ -2774,8 +2834,25 @@ struct lfgpu_batch {
     const double *b = nullptr;
     double delta2 = 0.0;
     int max_interval = 0;
-    bool sym = false;
+    int sym = 0;  // symmode of the captured FP64 product (k_bcg_spmv)
   } slot[3];
+  // lfgpu_batch_assemble_pcg: assembly, value refill, state set-up and the
+  // first chunk captured as one graph per mode, keyed on every argument
+  struct FusedKey {
+    const lfgpu_plan *plan = nullptr;
+    const lfgpu_fe *fe = nullptr;
+    lfgpu_coeff alpha{}, gamma{}, f{};
+    bool has_f = false;
+    const double *rhs = nullptr, *x = nullptr;
+    double rtol = 0.0, delta2 = 0.0;
+    int maxit = 0, check = 0, max_interval = 0, rows_per_group = 0;
+    const float *bval32 = nullptr;
+  };
+  struct Fused {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    FusedKey key;
+  } fused[3];
   double tune_ms[4] = {0, 0, 0, 0};
   // storage used by the batched sweeps: the matrix's own mirror (DIA or
   // ELL) or per-problem DIA (FMT_BDIA); -1 until the first solve decides
@@ -2790,6 +2867,7 @@ struct lfgpu_batch {
   // every problem has the same mirror-symmetric offsets and its FP64 values
   // are symmetric bit for bit: the FP64 SpMV sweep reads upper diagonals only
   bool bsymoff = false, bsym = false;
+  int *d_symflag = nullptr;  // device copy of the symmetry test (0: symmetric)
 };
 
 namespace {
@@ -2985,6 +3063,28 @@ int batch_choose_format(lfgpu_batch *bt) {
   return LFGPU_OK;
 }
 
+__global__ void k_set_flag(int *flag, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *flag = v;
+}
+
+// Queues the per-problem DIA value fill and, when the offsets are mirror-
+// symmetric, the bitwise symmetry check into bt->d_symflag (0: symmetric;
+// 1 otherwise).  Kernels only (capturable).
+int batch_fill_enqueue(lfgpu_batch *bt) {
+  lfgpu_csr *A = bt->A;
+  const int32_t n = A->nrows;
+  hipStream_t s = bt->ctx->stream;
+  hipLaunchKernelGGL(k_bdia_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col,
+                     A->val, n, bt->d_row_off, bt->nprob, bt->boff, bt->bndiag, bt->bval,
+                     bt->bval32);
+  hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, s, bt->d_symflag, bt->bsymoff ? 0 : 1);
+  if (bt->bsymoff)
+    hipLaunchKernelGGL(k_bdia_sym, dim3(grid_for(n, 256)), dim3(256), 0, s, bt->bval, n,
+                       bt->d_row_off, bt->nprob, bt->boff, bt->bndiag, bt->d_symflag);
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
+}
+
 // (Re)fills the per-problem DIA values after the matrix values changed.
 int batch_fill(lfgpu_batch *bt, bool need32) {
   if (bt->fmt != FMT_BDIA) return LFGPU_OK;
@@ -2996,28 +3096,12 @@ int batch_fill(lfgpu_batch *bt, bool need32) {
   }
   const bool with32 = bt->bval32 != nullptr;
   if (bt->bbuilt == A->val_version && bt->bbuilt32 == with32) return LFGPU_OK;
+  LFG_TRY(batch_fill_enqueue(bt));
+  int h = 1;
   hipStream_t s = bt->ctx->stream;
-  hipLaunchKernelGGL(k_bdia_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, A->row_ptr, A->col,
-                     A->val, n, bt->d_row_off, bt->nprob, bt->boff, bt->bndiag, bt->bval,
-                     bt->bval32);
-  LFG_HIP(hipGetLastError());
-  bt->bsym = false;
-  if (bt->bsymoff) {
-    int *flag = nullptr;
-    LFG_TRY(dalloc(&flag, 1, "batch symmetry flag"));
-    hipError_t e = hipMemsetAsync(flag, 0, 4, s);
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_bdia_sym, dim3(grid_for(n, 256)), dim3(256), 0, s, bt->bval, n,
-                         bt->d_row_off, bt->nprob, bt->boff, bt->bndiag, flag);
-      e = hipGetLastError();
-    }
-    int h = 1;
-    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    dfree(flag);
-    if (e != hipSuccess) return hip_fail(e, "batch symmetry check");
-    bt->bsym = h == 0;
-  }
+  LFG_HIP(hipMemcpyAsync(&h, bt->d_symflag, 4, hipMemcpyDeviceToHost, s));
+  LFG_HIP(hipStreamSynchronize(s));
+  bt->bsym = h == 0;
   bt->bbuilt = A->val_version;
   bt->bbuilt32 = with32;
   return LFGPU_OK;
@@ -3040,6 +3124,11 @@ void batch_release_graph(lfgpu_batch *bt) {
     if (g.graph) (void)hipGraphDestroy(g.graph);
     g = lfgpu_batch::Slot{};
   }
+  for (auto &g : bt->fused) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g = lfgpu_batch::Fused{};
+  }
 }
 
 // Captures one chunk of `count` iterations into slot `mode` unless the slot
@@ -3048,7 +3137,7 @@ void batch_release_graph(lfgpu_batch *bt) {
 // runs.
 template <class Launch>
 int batch_capture(lfgpu_batch *bt, int mode, int count, const double *x, const double *b,
-                  double delta2, int max_interval, bool sym, Launch &&launch) {
+                  double delta2, int max_interval, int sym, Launch &&launch) {
   lfgpu_batch::Slot &g = bt->slot[mode];
   if (g.exec && g.count == count && g.x == x && g.b == b && g.delta2 == delta2 &&
       g.max_interval == max_interval && g.sym == sym)
@@ -3113,18 +3202,14 @@ int batch_layout(lfgpu_batch *bt, int R) {
 BArgs batch_args(lfgpu_batch *bt, double *x) {
   lfgpu_solver_cache *c = bt->A->cache;
   return BArgs{batch_matview(bt), c->dinv, x, c->r, c->p, c->q, bt->part, bt->ngroups,
-               bt->groups, bt->probs, bt->st, bt->boff, bt->d_row_off};
+               bt->groups, bt->probs, bt->st, bt->boff, bt->d_row_off, bt->d_symflag};
 }
 
+// states, r = b - A x, p = D^-1 r and Eigen's early exits: kernels only
 int batch_init(lfgpu_batch *bt, const double *b, double *x, double rtol, int maxit) {
   hipStream_t s = bt->ctx->stream;
-  for (int p = 0; p < bt->nprob; ++p) {
-    bt->h_st[p] = BState{};
-    bt->h_st[p].thr = rtol * rtol;
-    bt->h_st[p].maxit = maxit;
-  }
-  LFG_HIP(hipMemcpyAsync(bt->st, bt->h_st, bt->nprob * sizeof(BState), hipMemcpyHostToDevice, s));
-  LFG_HIP(hipStreamSynchronize(s));
+  hipLaunchKernelGGL(k_bstate_init, dim3(grid_for(bt->nprob, 256)), dim3(256), 0, s, bt->st,
+                     bt->nprob, rtol * rtol, maxit);
   const BArgs a = batch_args(bt, x);
   if (bt->fmt == FMT_DIA)
     hipLaunchKernelGGL(k_bcg_init<FMT_DIA>, dim3(a.ngroups), dim3(256), 0, s, a, b);
@@ -3138,13 +3223,16 @@ int batch_init(lfgpu_batch *bt, const double *b, double *x, double rtol, int max
   return LFGPU_OK;
 }
 
-void batch_chunk(lfgpu_batch *bt, const BArgs &a, int count) {
+// symmode of the per-problem DIA product as k_bcg_spmv; -1: from the
+// host's symmetry flag of the current values
+void batch_chunk(lfgpu_batch *bt, const BArgs &a, int count, int symmode = -1) {
+  if (symmode < 0) symmode = bt->bsym ? 1 : 0;
   if (bt->fmt == FMT_DIA)
-    launch_bcg_chunk<FMT_DIA>(bt->ctx->stream, a, count, false);
+    launch_bcg_chunk<FMT_DIA>(bt->ctx->stream, a, count, 0);
   else if (bt->fmt == FMT_BDIA)
-    launch_bcg_chunk<FMT_BDIA>(bt->ctx->stream, a, count, bt->bsym);
+    launch_bcg_chunk<FMT_BDIA>(bt->ctx->stream, a, count, symmode);
   else
-    launch_bcg_chunk<FMT_ELL>(bt->ctx->stream, a, count, false);
+    launch_bcg_chunk<FMT_ELL>(bt->ctx->stream, a, count, 0);
 }
 
 // Autotuner: times a few iterations for each candidate workgroup tile (on a
@@ -3234,24 +3322,43 @@ void batch_mixed_chunk(lfgpu_batch *bt, const BMArgs<VT> &a, int count) {
     launch_bmix_chunk<FMT_ELL, VT>(bt->ctx->stream, a, count);
 }
 
-// Mixed-precision batch solve: set-up, graph-replayed chunks until every
-// problem has stopped, then x += xlo.
-template <class VT>
-int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg_opts *opts,
-                    int check, int mode, int *slot, int *syncs) {
-  hipStream_t s = bt->ctx->stream;
-  const double delta2 =
-      opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
-  const int max_interval = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
-  for (int p = 0; p < bt->nprob; ++p) {
-    bt->h_mst[p] = BMState{};
-    bt->h_mst[p].thr = opts->rtol * opts->rtol;
-    bt->h_mst[p].maxit = opts->maxit;
+// Polling callbacks of a batch solve (poll_chunks): copies of the device
+// states into the two pinned host slots, and the largest estimate of the
+// iterations the problems still running need.  The residual of a mixed
+// state was measured at its last reliable update.
+inline int rr_iter(const BState &h) { return h.iter; }
+inline int rr_iter(const BMState &h) { return h.last_refine; }
+
+template <class ST>
+struct BatchPoll {
+  lfgpu_batch *bt;
+  const ST *dev;
+  ST *host;  // two slots of nprob states
+  std::vector<PollHist> hist;
+  BatchPoll(lfgpu_batch *b, const ST *d, ST *h) : bt(b), dev(d), host(h), hist(b->nprob) {}
+  int copy(int j) {
+    LFG_HIP(hipMemcpyAsync(host + (size_t)j * bt->nprob, dev, bt->nprob * sizeof(ST),
+                           hipMemcpyDeviceToHost, bt->ctx->stream));
+    return LFGPU_OK;
   }
-  LFG_HIP(hipMemcpyAsync(bt->mst, bt->h_mst, bt->nprob * sizeof(BMState), hipMemcpyHostToDevice,
-                         s));
-  const BMArgs<VT> a = batch_mixed_args<VT>(bt, b, x, delta2, max_interval);
+  double remaining(int j) {
+    const ST *h = host + (size_t)j * bt->nprob;
+    double est = -1.0;
+    for (int p = 0; p < bt->nprob; ++p)
+      if (!h[p].done)
+        est = std::max(est, hist[p].next(h[p].rr, rr_iter(h[p]), h[p].iter, h[p].thr, h[p].maxit));
+    return est;
+  }
+};
+
+// states, r64 = b - A x, the FP32 recurrence start and Eigen's early exits:
+// kernels only
+template <class VT>
+int batch_mixed_init(lfgpu_batch *bt, const BMArgs<VT> &a, double rtol, int maxit) {
+  hipStream_t s = bt->ctx->stream;
   const dim3 G(a.ngroups), B(256);
+  hipLaunchKernelGGL(k_bmstate_init, dim3(grid_for(bt->nprob, 256)), B, 0, s, bt->mst, bt->nprob,
+                     rtol * rtol, maxit);
   if (bt->fmt == FMT_DIA)
     hipLaunchKernelGGL((k_bmix_init<FMT_DIA, VT>), G, B, 0, s, a);
   else if (bt->fmt == FMT_BDIA)
@@ -3261,13 +3368,28 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
   hipLaunchKernelGGL(k_bmix_init_finish<VT>, dim3(bt->nprob), B, 0, s, a);
   hipLaunchKernelGGL(k_bmix_zero_if_bb0<VT>, G, B, 0, s, a);
   LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
+}
+
+// Mixed-precision batch solve: set-up, graph-replayed chunks until every
+// problem has stopped, then x += xlo.
+template <class VT>
+int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg_opts *opts,
+                    int check, int mode, int *slot, int *syncs) {
+  hipStream_t s = bt->ctx->stream;
+  const double delta2 =
+      opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
+  const int max_interval = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
+  const BMArgs<VT> a = batch_mixed_args<VT>(bt, b, x, delta2, max_interval);
+  const dim3 G(a.ngroups), B(256);
+  LFG_TRY(batch_mixed_init(bt, a, opts->rtol, opts->maxit));
   // the captured arguments depend on the layout (a new layout releases every
   // slot), x, b and the refinement options
   const bool use_graph = opts->graph >= 0;
   if (use_graph)
-    LFG_TRY(batch_capture(bt, mode, check, x, b, delta2, max_interval, false,
+    LFG_TRY(batch_capture(bt, mode, check, x, b, delta2, max_interval, 0,
                           [&]() { batch_mixed_chunk<VT>(bt, a, check); }));
-  std::vector<PollHist> hist(bt->nprob);
+  BatchPoll<BMState> bp(bt, bt->mst, bt->h_mst);
   LFG_TRY(poll_chunks(
       bt->ctx, check,
       [&]() -> int {
@@ -3279,21 +3401,7 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
         }
         return LFGPU_OK;
       },
-      [&](int j) -> int {
-        LFG_HIP(hipMemcpyAsync(bt->h_mst + (size_t)j * bt->nprob, bt->mst,
-                               bt->nprob * sizeof(BMState), hipMemcpyDeviceToHost, s));
-        return LFGPU_OK;
-      },
-      [&](int j) -> double {
-        const BMState *h = bt->h_mst + (size_t)j * bt->nprob;
-        double est = -1.0;
-        for (int p = 0; p < bt->nprob; ++p)
-          if (!h[p].done)
-            est = std::max(est, hist[p].next(h[p].rr, h[p].last_refine, h[p].iter, h[p].thr,
-                                             h[p].maxit));
-        return est;
-      },
-      slot, syncs));
+      [&](int j) { return bp.copy(j); }, [&](int j) { return bp.remaining(j); }, slot, syncs));
   hipLaunchKernelGGL(k_bmix_final<VT>, G, B, 0, s, a);
   LFG_HIP(hipGetLastError());
   return LFGPU_OK;
@@ -3606,6 +3714,7 @@ extern "C" int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
     if (e != hipSuccess) rc = hip_fail(e, "pinned batch states");
   }
   if (rc == LFGPU_OK) rc = dalloc(&bt->d_row_off, (size_t)nprob + 1, "batch row offsets");
+  if (rc == LFGPU_OK) rc = dalloc(&bt->d_symflag, 1, "batch symmetry flag");
   // every problem is solved on its own rows only: refuse entries that couple
   // two problems
   int *flag = nullptr;
@@ -3659,6 +3768,7 @@ extern "C" int lfgpu_batch_destroy(lfgpu_batch *bt) {
   dfree(bt->st);
   dfree(bt->mst);
   dfree(bt->d_row_off);
+  dfree(bt->d_symflag);
   dfree(bt->boff);
   dfree(bt->bval);
   dfree(bt->bval32);
@@ -3667,6 +3777,11 @@ extern "C" int lfgpu_batch_destroy(lfgpu_batch *bt) {
   delete bt;
   return LFGPU_OK;
 }
+
+namespace {
+int batch_report(lfgpu_batch *bt, bool mixed, int slot, int syncs, lfgpu_pcg_info *infos,
+                 lfgpu_pcg_info *summary);
+}  // namespace
 
 extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec *b, lfgpu_vec *x,
                                const lfgpu_pcg_opts *opts, lfgpu_pcg_info *infos,
@@ -3709,9 +3824,9 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
     // recomputed whenever the values change)
     const bool use_graph = opts->graph >= 0;
     if (use_graph)
-      LFG_TRY(batch_capture(bt, LFGPU_PCG_FP64, check, x->d, nullptr, 0.0, 0, bt->bsym,
+      LFG_TRY(batch_capture(bt, LFGPU_PCG_FP64, check, x->d, nullptr, 0.0, 0, bt->bsym ? 1 : 0,
                             [&]() { batch_chunk(bt, a, check); }));
-    std::vector<PollHist> hist(bt->nprob);
+    BatchPoll<BState> bp(bt, bt->st, bt->h_st);
     LFG_TRY(poll_chunks(
         ctx, check,
         [&]() -> int {
@@ -3723,21 +3838,19 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
           }
           return LFGPU_OK;
         },
-        [&](int j) -> int {
-          LFG_HIP(hipMemcpyAsync(bt->h_st + (size_t)j * bt->nprob, bt->st,
-                                 bt->nprob * sizeof(BState), hipMemcpyDeviceToHost, s));
-          return LFGPU_OK;
-        },
-        [&](int j) -> double {
-          const BState *h = bt->h_st + (size_t)j * bt->nprob;
-          double est = -1.0;
-          for (int p = 0; p < bt->nprob; ++p)
-            if (!h[p].done)
-              est = std::max(est, hist[p].next(h[p].rr, h[p].iter, h[p].iter, h[p].thr, h[p].maxit));
-          return est;
-        },
-        &slot, &syncs));
+        [&](int j) { return bp.copy(j); }, [&](int j) { return bp.remaining(j); }, &slot,
+        &syncs));
   }
+  return batch_report(bt, mixed, slot, syncs, infos, summary);
+}
+
+namespace {
+// Waits for the solve queued on the context stream since ev0 and fills
+// infos / summary from the final state slot.
+int batch_report(lfgpu_batch *bt, bool mixed, int slot, int syncs, lfgpu_pcg_info *infos,
+                 lfgpu_pcg_info *summary) {
+  lfgpu_ctx *ctx = bt->ctx;
+  hipStream_t s = ctx->stream;
   LFG_HIP(hipEventRecord(ctx->ev1, s));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0.f;
@@ -3781,6 +3894,180 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
     return LFGPU_ENOCONV;
   }
   return LFGPU_OK;
+}
+}  // namespace
+
+namespace {
+bool same_coeff(const lfgpu_coeff &a, const lfgpu_coeff &b) {
+  return a.kind == b.kind && a.value == b.value && a.data == b.data;
+}
+
+bool same_key(const lfgpu_batch::FusedKey &a, const lfgpu_batch::FusedKey &b) {
+  return a.plan == b.plan && a.fe == b.fe && same_coeff(a.alpha, b.alpha) &&
+         same_coeff(a.gamma, b.gamma) && a.has_f == b.has_f &&
+         (!a.has_f || same_coeff(a.f, b.f)) && a.rhs == b.rhs && a.x == b.x && a.rtol == b.rtol &&
+         a.delta2 == b.delta2 && a.maxit == b.maxit && a.check == b.check &&
+         a.max_interval == b.max_interval && a.rows_per_group == b.rows_per_group &&
+         a.bval32 == b.bval32;
+}
+
+// product variant of the chunks that run on values assembled inside the
+// same graph: per-problem DIA with mirror-symmetric offsets reads the
+// device's symmetry flag
+int fused_symmode(const lfgpu_batch *bt) { return bt->fmt == FMT_BDIA && bt->bsymoff ? 2 : 0; }
+
+// Assembly, the solver storage's value refresh, the solve's set-up and its
+// first chunk, as kernels only
+int fused_enqueue(lfgpu_ctx *ctx, lfgpu_batch *bt, lfgpu_plan *plan, const lfgpu_fe *fe,
+                  const lfgpu_coeff *alpha, const lfgpu_coeff *gamma, const lfgpu_coeff *f,
+                  lfgpu_vec *rhs, double *x, const lfgpu_pcg_opts *opts, int check, double delta2,
+                  int max_interval) {
+  LFG_TRY(assemble_enqueue(ctx, plan, fe, alpha, gamma, f, rhs));
+  fill_vals_enqueue(ctx->stream, bt->A, bt->A->cache);
+  if (bt->fmt == FMT_BDIA) LFG_TRY(batch_fill_enqueue(bt));
+  if (opts->mode == LFGPU_PCG_FP64) {
+    LFG_TRY(batch_init(bt, rhs->d, x, opts->rtol, opts->maxit));
+    batch_chunk(bt, batch_args(bt, x), check, fused_symmode(bt));
+  } else if (opts->mode == LFGPU_PCG_MIXED) {
+    const BMArgs<float> a = batch_mixed_args<float>(bt, rhs->d, x, delta2, max_interval);
+    LFG_TRY(batch_mixed_init(bt, a, opts->rtol, opts->maxit));
+    batch_mixed_chunk<float>(bt, a, check);
+  } else {
+    const BMArgs<double> a = batch_mixed_args<double>(bt, rhs->d, x, delta2, max_interval);
+    LFG_TRY(batch_mixed_init(bt, a, opts->rtol, opts->maxit));
+    batch_mixed_chunk<double>(bt, a, check);
+  }
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
+}
+
+// The replayed solve: the captured graph, then chunk graphs polled as in
+// lfgpu_batch_pcg, then (mixed) x += xlo.
+template <class VT>
+int fused_replay(lfgpu_batch *bt, const lfgpu_batch::Fused &F, const double *b, double *x,
+                 const lfgpu_pcg_opts *opts, int check, double delta2, int max_interval,
+                 lfgpu_pcg_info *infos, lfgpu_pcg_info *summary) {
+  lfgpu_ctx *ctx = bt->ctx;
+  hipStream_t s = ctx->stream;
+  const int mode = opts->mode;
+  const bool mixed = mode != LFGPU_PCG_FP64;
+  BMArgs<VT> am{};
+  if (mixed) {
+    am = batch_mixed_args<VT>(bt, b, x, delta2, max_interval);
+    LFG_TRY(batch_capture(bt, mode, check, x, b, delta2, max_interval, 0,
+                          [&]() { batch_mixed_chunk<VT>(bt, am, check); }));
+  } else {
+    const int symmode = fused_symmode(bt);
+    const BArgs a = batch_args(bt, x);
+    LFG_TRY(batch_capture(bt, mode, check, x, nullptr, 0.0, 0, symmode,
+                          [&]() { batch_chunk(bt, a, check, symmode); }));
+  }
+  // the replay assembles new values: the host's record of which values the
+  // storage and the symmetry flag hold is out of date from here on
+  csr_values_changed(bt->A);
+  bt->bbuilt = ~0ull;
+  int slot = 0, syncs = 0;
+  bool first = true;
+  auto launch = [&]() -> int {
+    LFG_HIP(hipGraphLaunch(first ? F.exec : bt->slot[mode].exec, s));
+    first = false;
+    return LFGPU_OK;
+  };
+  LFG_HIP(hipEventRecord(ctx->ev0, s));
+  if (mixed) {
+    BatchPoll<BMState> bp(bt, bt->mst, bt->h_mst);
+    LFG_TRY(poll_chunks(ctx, check, launch, [&](int j) { return bp.copy(j); },
+                        [&](int j) { return bp.remaining(j); }, &slot, &syncs));
+    hipLaunchKernelGGL(k_bmix_final<VT>, dim3(am.ngroups), dim3(256), 0, s, am);
+    LFG_HIP(hipGetLastError());
+  } else {
+    BatchPoll<BState> bp(bt, bt->st, bt->h_st);
+    LFG_TRY(poll_chunks(ctx, check, launch, [&](int j) { return bp.copy(j); },
+                        [&](int j) { return bp.remaining(j); }, &slot, &syncs));
+  }
+  return batch_report(bt, mixed, slot, syncs, infos, summary);
+}
+}  // namespace
+
+extern "C" int lfgpu_batch_assemble_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, lfgpu_plan *plan,
+                                        const lfgpu_fe *fe, const lfgpu_coeff *alpha,
+                                        const lfgpu_coeff *gamma, const lfgpu_coeff *f,
+                                        lfgpu_vec *rhs, lfgpu_vec *x, const lfgpu_pcg_opts *opts,
+                                        lfgpu_pcg_info *infos, lfgpu_pcg_info *summary) {
+  LFG_REQUIRE(ctx && bt && plan && fe && alpha && gamma && rhs && x && opts, LFGPU_EINVAL,
+              "NULL argument to lfgpu_batch_assemble_pcg");
+  LFG_REQUIRE(ctx == bt->ctx, LFGPU_EINVAL, "the batch belongs to another context");
+  LFG_REQUIRE(&plan->csr == bt->A, LFGPU_EINVAL,
+              "the batch was not created on this plan's matrix");
+  LFG_REQUIRE(opts->mode == LFGPU_PCG_FP64 || opts->mode == LFGPU_PCG_MIXED ||
+                  opts->mode == LFGPU_PCG_MIXED_MATRIX,
+              LFGPU_EINVAL, "unknown PCG mode");
+  LFG_REQUIRE(rhs->n == bt->A->nrows && x->n == bt->A->nrows && rhs->d != x->d, LFGPU_EINVAL,
+              "batch PCG size mismatch");
+  int check = opts->check_every > 0 ? opts->check_every : 64;
+  check += check & 1;
+  const double delta2 =
+      opts->refine_delta > 0.0 ? opts->refine_delta * opts->refine_delta : 0.09;
+  const int max_interval = opts->refine_max_interval > 0 ? opts->refine_max_interval : 2147483647;
+  lfgpu_batch::FusedKey key;
+  key.plan = plan;
+  key.fe = fe;
+  key.alpha = *alpha;
+  key.gamma = *gamma;
+  key.has_f = f != nullptr;
+  if (f) key.f = *f;
+  key.rhs = rhs->d;
+  key.x = x->d;
+  key.rtol = opts->rtol;
+  key.delta2 = delta2;
+  key.maxit = opts->maxit;
+  key.check = check;
+  key.max_interval = max_interval;
+  key.rows_per_group = bt->rows_per_group;
+  key.bval32 = bt->bval32;
+  lfgpu_batch::Fused &F = bt->fused[opts->mode];
+  if (F.exec && same_key(F.key, key)) {
+    if (opts->mode == LFGPU_PCG_MIXED)
+      return fused_replay<float>(bt, F, rhs->d, x->d, opts, check, delta2, max_interval, infos,
+                                 summary);
+    return fused_replay<double>(bt, F, rhs->d, x->d, opts, check, delta2, max_interval, infos,
+                                summary);
+  }
+  // First call with these arguments: the separate calls (which choose the
+  // storage, tune the layout and allocate what the mode needs), then the
+  // capture of the graph the next calls replay.
+  LFG_TRY(lfgpu_assemble_galerkin(ctx, plan, fe, alpha, gamma, f, nullptr, rhs));
+  const double asm_ms = ctx->last_ms;
+  const int rc = lfgpu_batch_pcg(ctx, bt, rhs, x, opts, infos, summary);
+  if (rc != LFGPU_OK && rc != LFGPU_ENOCONV) return rc;
+  if (summary) summary->ms += asm_ms;
+  key.rows_per_group = bt->rows_per_group;
+  key.bval32 = bt->bval32;
+  hipStream_t s = ctx->stream;
+  if (F.exec) (void)hipGraphExecDestroy(F.exec);
+  if (F.graph) (void)hipGraphDestroy(F.graph);
+  F = lfgpu_batch::Fused{};
+  LFG_HIP(hipStreamSynchronize(s));
+  LFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int erc = fused_enqueue(ctx, bt, plan, fe, alpha, gamma, f, rhs, x->d, opts, check,
+                                delta2, max_interval);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &graph);
+  if (erc != LFGPU_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return erc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture (assemble + PCG)");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(ei, "hipGraphInstantiate (assemble + PCG)");
+  }
+  F.graph = graph;
+  F.exec = exec;
+  F.key = key;
+  return rc;
 }
 
 extern "C" int lfgpu_solver_storage(const lfgpu_csr *A, int *storage, int *ndiag, int *sym) {

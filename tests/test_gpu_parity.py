@@ -735,6 +735,57 @@ def test_batched_block_dia_storage_gives_the_ell_iterates(ctx, mode):
             h.close()
 
 
+@pytest.mark.parametrize("extra", [False, True])
+def test_batched_assemble_pcg_graph_matches_separate_calls(ctx, extra):
+    # lfgpu_batch_assemble_pcg: the first call runs assembly + batched PCG,
+    # later calls replay one captured graph (assembly, value refresh,
+    # set-up, first chunk) and continue with the chunk graphs.  Every call
+    # must equal plan.assemble + batch.pcg bit for bit, in every mode, also
+    # when the coefficient data change between replays and with them the
+    # bitwise symmetry of the per-problem DIA values (alpha = 1: symmetric;
+    # alpha per cell: not), which the replayed product reads on the device.
+    # extra: one unstructured problem more, which makes the batch sliced ELL.
+    sizes = [9, 17, 5, 24, 12]
+    xy, cells, flags, voff, coff = capi.batched_tp_mesh(sizes)
+    if extra:
+        em = delaunay_mesh(80, 5)
+        nv0 = xy.shape[0]
+        xy = np.vstack([xy, em.xy])
+        cells = np.vstack([cells, em.cells + nv0]).astype(np.int32)
+        flags = np.concatenate([flags, em.dirichlet_flags(1)])
+        voff = np.append(voff, xy.shape[0])
+    g = capi.Mesh(ctx, xy, cells)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, flags)
+    nd = g.ndofs
+    theta = capi.Vec(ctx, np.zeros(cells.shape[0]))
+    coefs = (capi.coeff(capi.COEFF_EXPCELL, 1.0, theta), capi.coeff(capi.COEFF_CONST, 0.5),
+             capi.coeff(capi.COEFF_SINSIN, 1.0))
+    b_sep, x_sep = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
+    b, x = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
+    # one batch for both paths: the same tuned layout, so the same sums
+    batch = capi.Batch(ctx, plan.csr, voff)
+    rng = np.random.default_rng(3)
+    thetas = [np.zeros(cells.shape[0]), rng.standard_normal(cells.shape[0]),
+              np.zeros(cells.shape[0]), rng.standard_normal(cells.shape[0])]
+    for mode in (capi.PCG_FP64, capi.PCG_MIXED, capi.PCG_MIXED_MATRIX):
+        for call, th in enumerate(thetas):
+            theta.upload(th)
+            plan.assemble(fe, *coefs, b_sep)
+            x_sep.fill(0.0)
+            s_ref, i_ref = batch.pcg(b_sep, x_sep, rtol=1e-10, mode=mode, check_every=16)
+            xr, br = x_sep.numpy(), b_sep.numpy()
+            x.fill(0.0)
+            s, inf = batch.assemble_pcg(plan, fe, *coefs, b, x, rtol=1e-10, mode=mode,
+                                        check_every=16)
+            assert s.converged and s.syncs >= 1, (mode, call)
+            assert np.array_equal(b.numpy(), br), (mode, call)
+            assert [i.iters for i in inf] == [i.iters for i in i_ref], (mode, call)
+            assert np.array_equal(x.numpy(), xr), (mode, call)
+    for h in (batch, x, b, x_sep, b_sep, theta, plan, fe, g):
+        h.close()
+
+
 def test_batched_fp64_symmetry_change_between_solves(ctx):
     # one batch, one solution vector, the same chunk length: the values first
     # bitwise symmetric (alpha = 1 on a jittered mesh: G_i . G_j commutes), so
