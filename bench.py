@@ -189,10 +189,12 @@ def ell_roofline_entry(n, nnz, times_ms, col_bytes):
                                       "and 3 dot products, %d-bit column offsets)"
                                       % (8 * col_bytes),
             "achieved": a1, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a1 / HBM_PEAK_GBS,
+            "traffic": pmc_traffic("k_cgr_spmv") if col_bytes == 2 else None,
             "algorithmic_bytes": b1, "avg_us": 1e3 * times_ms[0],
             "timing": "median of per-launch hipEvent pairs on the solver stream",
             "sweep2": {"kernel": "k_cgr_update", "algorithmic_bytes": b2,
-                       "avg_us": 1e3 * times_ms[1], "achieved": a2, "frac": a2 / HBM_PEAK_GBS},
+                       "avg_us": 1e3 * times_ms[1], "achieved": a2, "frac": a2 / HBM_PEAK_GBS,
+                       "traffic": pmc_traffic("k_cgr_update")},
             "reductions_us": 2e3 * times_ms[2],
             "iteration": {"us": it_us, "algorithmic_bytes": b1 + b2,
                           "achieved": (b1 + b2) / (it_us * 1e-6) / 1e9,
@@ -557,6 +559,18 @@ def run_p1(args, capi, ctx):
     return out, checks
 
 
+def p3_dirichlet_flags(nv, ev, ce, nd):
+    """flagEntitiesOnBoundary for P3 (special_entity_sets.cc:40-86): edges
+    of one cell, their endpoints, and the interior DOFs of those edges
+    (numbered nv + 2e, nv + 2e + 1)."""
+    be = np.nonzero(np.bincount(ce.ravel(), minlength=len(ev)) == 1)[0]
+    fl = np.zeros(nd, np.uint8)
+    fl[ev[be].ravel()] = 1
+    fl[nv + 2 * be] = 1
+    fl[nv + 2 * be + 1] = 1
+    return fl
+
+
 def build_p3(capi, ctx, n=512, seed=0):
     """The P3 workload: n x n jittered mesh, zero Dirichlet data on the
     boundary DOFs, a = exp(N(0,1)) per cell, f = sin sin."""
@@ -568,15 +582,7 @@ def build_p3(capi, ctx, n=512, seed=0):
     xy[inner] += rng.uniform(-0.2 * h, 0.2 * h, size=(int(inner.sum()), 2))
     ev, ce, co = capi.mesh_edges(xy.shape[0], cells, edges)
     cd, nd = capi.dof_table(xy.shape[0], cells, len(ev), ce, co, 3)
-    # boundary DOFs: boundary vertices and the interior DOFs of boundary edges
-    vb = ((xy[:, 0] == 0) | (xy[:, 0] == 1) | (xy[:, 1] == 0) | (xy[:, 1] == 1))
-    eb = vb[ev[:, 0]] & vb[ev[:, 1]] & (
-        (xy[ev[:, 0], 0] == xy[ev[:, 1], 0]) | (xy[ev[:, 0], 1] == xy[ev[:, 1], 1]))
-    flags = np.zeros(nd, np.uint8)
-    flags[: xy.shape[0]] = vb
-    be = np.nonzero(eb)[0]
-    flags[xy.shape[0] + 2 * be] = 1
-    flags[xy.shape[0] + 2 * be + 1] = 1
+    flags = p3_dirichlet_flags(xy.shape[0], ev, ce, nd)
     mesh = capi.Mesh(ctx, xy, cells, cd, nd)
     fe = capi.FE(ctx, 3)
     plan = capi.Plan(ctx, mesh, flags)
@@ -796,6 +802,142 @@ def run_batched(args, capi, ctx, nprob=256, seed=4):
             tgt = out["plain_fp64_per_problem"] if name == "plain_fp64" else per_mode[name]
             tgt["worst_err_vs_oracle"] = float(max(errs))
             log("batched %s: worst error vs the oracle %.2e" % (name, max(errs)))
+
+    return out, checks
+
+
+def run_batched_p3(args, capi, ctx, nprob=256, seed=6):
+    """C4 at degree 3: nprob P3 problems on n x n meshes, sides log-uniform
+    in [8, 64], each with its own DOF numbering (vertices, edges, cells) and
+    zero Dirichlet data on its boundary DOFs, stacked so that every problem
+    owns a contiguous row range; a = exp(N(0,1)) per cell, f = 1.  The
+    batch storage is sliced ELL (the P3 patterns are not banded), tuned by
+    the autotuner for this degree.  Each step: the separate calls
+    (assembly, batched PCG) and one captured graph per step
+    (lfgpu_batch_assemble_pcg)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(8), np.log(64), nprob))).astype(int)
+    xs, cs, cds, fls, doff, coff, vo = [], [], [], [], [0], [0], 0
+    for n in sizes:
+        xy, cells, edges = capi.tp_triag_mesh(int(n))
+        ev, ce, co = capi.mesh_edges(xy.shape[0], cells, edges)
+        cd, nd = capi.dof_table(xy.shape[0], cells, len(ev), ce, co, 3)
+        fl = p3_dirichlet_flags(xy.shape[0], ev, ce, nd)
+        xs.append(xy)
+        cs.append(cells + vo)
+        cds.append(cd + doff[-1])
+        fls.append(fl)
+        vo += xy.shape[0]
+        doff.append(doff[-1] + nd)
+        coff.append(coff[-1] + cells.shape[0])
+    xy = np.concatenate(xs)
+    cells = np.ascontiguousarray(np.concatenate(cs), np.int32)
+    cd = np.concatenate(cds)
+    nd, ne = int(doff[-1]), int(coff[-1])
+    doff = np.array(doff, np.int64)
+    log("batched P3: %d problems, %d cells, %d DOFs" % (nprob, ne, nd))
+    g = rng.standard_normal(ne)
+    mesh = capi.Mesh(ctx, xy, cells, cd, nd)
+    fe = capi.FE(ctx, 3)
+    plan = capi.Plan(ctx, mesh, np.concatenate(fls))
+    theta = capi.Vec(ctx, g)
+    b, x = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
+    acoef = (fe, capi.coeff(capi.COEFF_EXPCELL, 1.0, theta), capi.coeff(capi.COEFF_CONST, 0.0),
+             capi.coeff(capi.COEFF_CONST, 1.0), b)
+    plan.assemble(*acoef)
+    batch = capi.Batch(ctx, plan.csr, doff)
+    rp, col, val = plan.csr.download()
+    bh = b.numpy()
+    modes = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
+             "mixed_matrix": capi.PCG_MIXED_MATRIX}
+    per_mode, sols = {}, {}
+    for name, md in modes.items():
+        x.fill(0.0)
+        batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md)  # tunes, captures the chunk
+        times = []
+        for _ in range(max(1, args.batch_steps)):
+            ctx.sync()
+            t0 = time.perf_counter()
+            plan.assemble(*acoef)
+            x.fill(0.0)
+            summ, infos = batch.pcg(b, x, rtol=1e-10, maxit=MAXIT, mode=md)
+            ctx.sync()
+            times.append(time.perf_counter() - t0)
+        sols[name] = x.numpy()
+        it = [i.iters for i in infos]
+        x.fill(0.0)
+        batch.assemble_pcg(plan, *acoef, x, rtol=1e-10, maxit=MAXIT, mode=md)  # captures
+        ftimes = []
+        for _ in range(max(1, args.batch_steps)):
+            ctx.sync()
+            t0 = time.perf_counter()
+            x.fill(0.0)
+            fs, finfos = batch.assemble_pcg(plan, *acoef, x, rtol=1e-10, maxit=MAXIT, mode=md)
+            ctx.sync()
+            ftimes.append(time.perf_counter() - t0)
+        same = bool(np.array_equal(x.numpy(), sols[name]) and [i.iters for i in finfos] == it)
+        wall, fwall = float(np.median(times)), float(np.median(ftimes))
+        per_mode[name] = {
+            "melem_s": ne / fwall / 1e6, "ms_per_step": 1e3 * fwall, "host_syncs": int(fs.syncs),
+            "path": "one captured graph per step (lfgpu_batch_assemble_pcg) and the captured "
+                    "chunks",
+            "bitwise_equal_to_separate_calls": same,
+            "separate_calls": {"ms_per_step": 1e3 * wall, "host_syncs": int(summ.syncs)},
+            "iters_max": int(max(it)), "iters_min": int(min(it)),
+            "refinements_sum": int(sum(i.refinements for i in infos)),
+            "converged": bool(fs.converged), "worst_relres": fs.relres}
+        log("batched P3 %s: %.1f ms/step one graph (%d syncs), %.1f ms separate (%d syncs), "
+            "iters %d..%d, bitwise equal %s" % (name, 1e3 * fwall, fs.syncs, 1e3 * wall,
+                                                summ.syncs, min(it), max(it), same))
+    out = {"metric": "batched P3 assemble + PCG, Melem/s",
+           "config": {"workload": "%d P3 problems, sides log-uniform in [8,64], a=exp(N(0,1)), "
+                                  "f=1" % nprob, "cells": ne, "dofs": nd,
+                      "nnz": int(plan.csr.info()[2])},
+           "storage": {capi.STORAGE_ELL: "sliced ELL", capi.STORAGE_DIA: "DIA",
+                       capi.STORAGE_ELL16: "sliced ELL, 16-bit column offsets",
+                       capi.STORAGE_BLOCK_DIA: "per-problem DIA"}.get(batch.storage()),
+           "tuned_rows_per_workgroup": summ.refinements, "modes": per_mode,
+           "speedup_vs_fp64": {k: per_mode["fp64"]["ms_per_step"] / v["ms_per_step"]
+                               for k, v in per_mode.items()}, "oracle": None}
+    for hh in (batch, b, x, theta, plan, fe, mesh):
+        hh.close()
+
+    def checks():
+        # each problem assembled by the oracle's reference-order path on its own
+        # mesh and solved by its PCG (problems in parallel, one thread each)
+        if args.no_oracle:
+            return
+        import concurrent.futures as cf
+        L = oracle_lib()
+
+        def one(k):
+            n = int(sizes[k])
+            m = L.tp_triag_mesh(n)
+            cdo, ndo = m.dofs(3)
+            rpo, colo, valo, bo = L.cpu_path_assemble(m, 3, cdo, ndo,
+                                                      np.exp(g[coff[k]:coff[k + 1]]), 0.0,
+                                                      L.F_CONST, 1.0, m.dirichlet_flags(3))
+            lo, hi = int(doff[k]), int(doff[k + 1])
+            cmp = oracle_compare_csr(rpo, colo, valo, bo, rp[lo:hi + 1] - rp[lo],
+                                     col[rp[lo]:rp[hi]] - lo, val[rp[lo]:rp[hi]], bh[lo:hi])
+            xo, _ = oracle_solve(L, rpo, colo, valo, bo, threads=False)
+            return xo, cmp
+
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            res = list(ex.map(one, range(nprob)))
+        x_ref = np.concatenate([r[0] for r in res])
+        diffs = [r[1]["matrix_row_scaled_diff"] for r in res
+                 if r[1]["matrix_row_scaled_diff"] is not None]
+        out["oracle"] = {"s": time.perf_counter() - t0,
+                         "pattern_bitwise_equal": all(r[1]["pattern_bitwise_equal"] for r in res),
+                         "matrix_row_scaled_diff": float(max(diffs)) if diffs else None,
+                         "rhs_max_rel_diff": float(max(r[1]["rhs_max_rel_diff"] for r in res))}
+        for name, xv in sols.items():
+            errs = [rel_err(xv[doff[k]:doff[k + 1]], x_ref[doff[k]:doff[k + 1]])
+                    for k in range(nprob)]
+            per_mode[name]["worst_err_vs_oracle"] = float(max(errs))
+            log("batched P3 %s: worst error vs the oracle %.2e" % (name, max(errs)))
 
     return out, checks
 
@@ -1174,8 +1316,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--config", choices=["p1", "p3", "batched", "adjoint", "traffic", "fkpp",
-                                         "strip", "c1"],
+    ap.add_argument("--config", choices=["p1", "p3", "batched", "batched_p3", "adjoint", "traffic",
+                                         "fkpp", "strip", "c1"],
                     default="p1")
     ap.add_argument("--mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
     ap.add_argument("--batch-mode", choices=["fp64", "mixed", "mixed_matrix"], default="mixed")
@@ -1223,11 +1365,16 @@ def main():
             checks.append(chk)
             out["c4"], chk = run_batched(args, capi, ctx)
             checks.append(chk)
+            out["c4_p3"], chk = run_batched_p3(args, capi, ctx)
+            checks.append(chk)
     elif args.config == "p3":
         out, chk = run_p3(args, capi, ctx)
         checks.append(chk)
     elif args.config == "batched":
         out, chk = run_batched(args, capi, ctx)
+        checks.append(chk)
+    elif args.config == "batched_p3":
+        out, chk = run_batched_p3(args, capi, ctx)
         checks.append(chk)
     elif args.config == "traffic":
         out = run_traffic(args, capi, ctx)

@@ -735,18 +735,15 @@ int asm_args(lfgpu_plan *p, const lfgpu_fe *fe, const lfgpu_coeff *alpha, const 
   return LFGPU_OK;
 }
 
-#ifndef LFGPU_COEF_PRE
-#define LFGPU_COEF_PRE 1
-#endif
-
-// the per-cell coefficient pass, then the row-owner kernel
+// The row-owner kernel, at P3 after the per-cell coefficient pass: a P3
+// cell is visited by up to 10 rows, and the pass took the 512^2 P3
+// assembly from 1.32 to 1.08 ms.  A P1 cell is visited by 3 rows, and the
+// pass measured no gain there (0.80 vs 0.81 ms at 2048^2): that kernel waits
+// on its dependent record loads more than on the transcendental functions.
 template <bool EDGES>
 int queue_assembly(hipStream_t s, lfgpu_plan *p, AsmArgs a) {
-  if (p->mesh->nloc == 3) {
-    if (LFGPU_COEF_PRE) LFG_TRY(precompute_coeffs<3>(s, p, a));
-    return launch_assemble<3, 3, 4, EDGES>(s, p, a);
-  }
-  if (LFGPU_COEF_PRE) LFG_TRY(precompute_coeffs<12>(s, p, a));
+  if (p->mesh->nloc == 3) return launch_assemble<3, 3, 4, EDGES>(s, p, a);
+  LFG_TRY(precompute_coeffs<12>(s, p, a));
   return launch_assemble<10, 12, 12, EDGES>(s, p, a);
 }
 

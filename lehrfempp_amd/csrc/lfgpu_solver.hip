@@ -2823,9 +2823,10 @@ struct lfgpu_batch {
   BState *h_st = nullptr;  // pinned, two slots of nprob states
   BMState *mst = nullptr;    // mixed-precision states (allocated on first use)
   BMState *h_mst = nullptr;  // pinned, two slots of nprob states
-  // one captured chunk of iterations per precision mode (index = mode), so
-  // alternating modes replay their own graphs instead of re-capturing; a
-  // slot is reused while every captured argument is unchanged
+  // one captured chunk of iterations per precision mode and FP64 product
+  // variant (batch_slot), so alternating modes replay their own graphs
+  // instead of re-capturing; a slot is reused while every captured argument
+  // is unchanged
   struct Slot {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -2835,7 +2836,8 @@ struct lfgpu_batch {
     double delta2 = 0.0;
     int max_interval = 0;
     int sym = 0;  // symmode of the captured FP64 product (k_bcg_spmv)
-  } slot[3];
+  } slot[5];
+  int *h_symflag = nullptr;  // pinned host copy of d_symflag (after h_st)
   // lfgpu_batch_assemble_pcg: assembly, value refill, state set-up and the
   // first chunk captured as one graph per mode, keyed on every argument
   struct FusedKey {
@@ -3129,6 +3131,12 @@ void batch_release_graph(lfgpu_batch *bt) {
     if (g.graph) (void)hipGraphDestroy(g.graph);
     g = lfgpu_batch::Fused{};
   }
+}
+
+// slot of a captured chunk: the mode for the mixed modes; for FP64 one slot
+// per product variant (symmode 0, 1, 2)
+inline int batch_slot(int mode, int sym) {
+  return mode != LFGPU_PCG_FP64 ? mode : (sym == 0 ? 0 : sym == 1 ? 3 : 4);
 }
 
 // Captures one chunk of `count` iterations into slot `mode` unless the slot
@@ -3710,7 +3718,8 @@ extern "C" int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
   int rc = dalloc(&bt->st, (size_t)nprob, "batch states");
   if (rc == LFGPU_OK) {
     // two slots: the solve polls one step behind the device
-    hipError_t e = hipHostMalloc(&bt->h_st, 2 * nprob * sizeof(BState), hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(&bt->h_st, 2 * nprob * sizeof(BState) + 64, hipHostMallocDefault);
+    if (e == hipSuccess) bt->h_symflag = reinterpret_cast<int *>(bt->h_st + 2 * nprob);
     if (e != hipSuccess) rc = hip_fail(e, "pinned batch states");
   }
   if (rc == LFGPU_OK) rc = dalloc(&bt->d_row_off, (size_t)nprob + 1, "batch row offsets");
@@ -3824,14 +3833,15 @@ extern "C" int lfgpu_batch_pcg(lfgpu_ctx *ctx, lfgpu_batch *bt, const lfgpu_vec 
     // recomputed whenever the values change)
     const bool use_graph = opts->graph >= 0;
     if (use_graph)
-      LFG_TRY(batch_capture(bt, LFGPU_PCG_FP64, check, x->d, nullptr, 0.0, 0, bt->bsym ? 1 : 0,
+      LFG_TRY(batch_capture(bt, batch_slot(LFGPU_PCG_FP64, bt->bsym ? 1 : 0), check, x->d,
+                            nullptr, 0.0, 0, bt->bsym ? 1 : 0,
                             [&]() { batch_chunk(bt, a, check); }));
     BatchPoll<BState> bp(bt, bt->st, bt->h_st);
     LFG_TRY(poll_chunks(
         ctx, check,
         [&]() -> int {
           if (use_graph) {
-            LFG_HIP(hipGraphLaunch(bt->slot[LFGPU_PCG_FP64].exec, s));
+            LFG_HIP(hipGraphLaunch(bt->slot[batch_slot(LFGPU_PCG_FP64, bt->bsym ? 1 : 0)].exec, s));
           } else {
             batch_chunk(bt, a, check);
             LFG_HIP(hipGetLastError());
@@ -3957,10 +3967,16 @@ int fused_replay(lfgpu_batch *bt, const lfgpu_batch::Fused &F, const double *b, 
     LFG_TRY(batch_capture(bt, mode, check, x, b, delta2, max_interval, 0,
                           [&]() { batch_mixed_chunk<VT>(bt, am, check); }));
   } else {
-    const int symmode = fused_symmode(bt);
+    // the chunks read the symmetry flag on the device (symmode 2) until the
+    // host has read it with the first state copy; then the variant it names
     const BArgs a = batch_args(bt, x);
-    LFG_TRY(batch_capture(bt, mode, check, x, nullptr, 0.0, 0, symmode,
+    const int symmode = fused_symmode(bt);
+    LFG_TRY(batch_capture(bt, batch_slot(mode, symmode), check, x, nullptr, 0.0, 0, symmode,
                           [&]() { batch_chunk(bt, a, check, symmode); }));
+    if (symmode == 2)
+      for (int v = 0; v < 2; ++v)
+        LFG_TRY(batch_capture(bt, batch_slot(mode, v), check, x, nullptr, 0.0, 0, v,
+                              [&]() { batch_chunk(bt, a, check, v); }));
   }
   // the replay assembles new values: the host's record of which values the
   // storage and the symmetry flag hold is out of date from here on
@@ -3968,8 +3984,15 @@ int fused_replay(lfgpu_batch *bt, const lfgpu_batch::Fused &F, const double *b, 
   bt->bbuilt = ~0ull;
   int slot = 0, syncs = 0;
   bool first = true;
+  // FP64 on per-problem DIA with mirror-symmetric offsets: -1 until the host
+  // has read the symmetry flag the graph computed, then 0 (symmetric) or 1
+  int hflag = mixed || fused_symmode(bt) != 2 ? 1 : -1;
+  const bool devsym = hflag < 0;
   auto launch = [&]() -> int {
-    LFG_HIP(hipGraphLaunch(first ? F.exec : bt->slot[mode].exec, s));
+    hipGraphExec_t ex = F.exec;
+    if (!first)
+      ex = bt->slot[mixed ? mode : batch_slot(mode, hflag < 0 ? 2 : (hflag == 0 ? 1 : 0))].exec;
+    LFG_HIP(hipGraphLaunch(ex, s));
     first = false;
     return LFGPU_OK;
   };
@@ -3982,8 +4005,24 @@ int fused_replay(lfgpu_batch *bt, const lfgpu_batch::Fused &F, const double *b, 
     LFG_HIP(hipGetLastError());
   } else {
     BatchPoll<BState> bp(bt, bt->st, bt->h_st);
-    LFG_TRY(poll_chunks(ctx, check, launch, [&](int j) { return bp.copy(j); },
-                        [&](int j) { return bp.remaining(j); }, &slot, &syncs));
+    bool flag_queued = false;
+    LFG_TRY(poll_chunks(
+        ctx, check, launch,
+        [&](int j) -> int {
+          LFG_TRY(bp.copy(j));
+          if (devsym && !flag_queued) {
+            LFG_HIP(hipMemcpyAsync(bt->h_symflag, bt->d_symflag, sizeof(int),
+                                   hipMemcpyDeviceToHost, s));
+            flag_queued = true;
+          }
+          return LFGPU_OK;
+        },
+        [&](int j) {
+          // the first step's copies have completed once any step is waited on
+          if (devsym && hflag < 0) hflag = *bt->h_symflag != 0 ? 1 : 0;
+          return bp.remaining(j);
+        },
+        &slot, &syncs));
   }
   return batch_report(bt, mixed, slot, syncs, infos, summary);
 }

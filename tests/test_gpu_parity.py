@@ -786,6 +786,67 @@ def test_batched_assemble_pcg_graph_matches_separate_calls(ctx, extra):
         h.close()
 
 
+def test_batched_p3_problems_against_the_oracle(ctx):
+    # a batch of P3 problems, each with its own DOF numbering stacked into
+    # one contiguous row range: the device CSR block of every problem equals
+    # the oracle's reference-order system on its own mesh (pattern bit for
+    # bit, values to 1e-15 of the row), and every mode's solution solves it
+    # (oracle banded Cholesky, 1e-8); the one-graph step gives the same bits
+    sizes = [5, 9, 3, 12, 7]
+    xs, cs, cds, fls, doff, vo = [], [], [], [], [0], 0
+    for n in sizes:
+        m = L.tp_triag_mesh(n)
+        cd, nd = m.dofs(3)
+        xs.append(m.xy)
+        cs.append(m.cells + vo)
+        cds.append(cd + doff[-1])
+        fls.append(m.dirichlet_flags(3))
+        vo += m.nv
+        doff.append(doff[-1] + nd)
+    nd = doff[-1]
+    cells = np.ascontiguousarray(np.concatenate(cs), np.int32)
+    g = capi.Mesh(ctx, np.concatenate(xs), cells, np.concatenate(cds), nd)
+    fe = capi.FE(ctx, 3)
+    plan = capi.Plan(ctx, g, np.concatenate(fls))
+    th = np.random.default_rng(8).standard_normal(cells.shape[0])
+    theta = capi.Vec(ctx, th)
+    b, x = capi.Vec(ctx, nd), capi.Vec(ctx, nd)
+    coefs = (capi.coeff(capi.COEFF_EXPCELL, 1.0, theta), capi.coeff(capi.COEFF_CONST, 0.0),
+             capi.coeff(capi.COEFF_CONST, 1.0))
+    plan.assemble(fe, *coefs, b)
+    rp, col, val = plan.csr.download()
+    bh = b.numpy()
+    batch = capi.Batch(ctx, plan.csr, np.array(doff, np.int32))
+    ref = []
+    c0 = 0
+    for k, n in enumerate(sizes):
+        m = L.tp_triag_mesh(n)
+        cdo, ndo = m.dofs(3)
+        rpo, colo, valo, bo = L.cpu_path_assemble(m, 3, cdo, ndo, np.exp(th[c0:c0 + m.ne]), 0.0,
+                                                  L.F_CONST, 1.0, m.dirichlet_flags(3))
+        c0 += m.ne
+        lo, hi = doff[k], doff[k + 1]
+        assert np.array_equal(rp[lo:hi + 1] - rp[lo], rpo), k
+        assert np.array_equal(col[rp[lo]:rp[hi]] - lo, colo), k
+        assert row_scaled_err(rpo, val[rp[lo]:rp[hi]], valo) <= 1e-15, k
+        assert np.abs(bh[lo:hi] - bo).max() <= 1e-15 * np.abs(bo).max(), k
+        ref.append(L.band_cholesky_solve(rpo, colo, valo, bo))
+    for mode in (capi.PCG_FP64, capi.PCG_MIXED, capi.PCG_MIXED_MATRIX):
+        x.fill(0.0)
+        summ, infos = batch.pcg(b, x, rtol=1e-12, mode=mode)
+        assert summ.converged, mode
+        xv = x.numpy()
+        for k in range(len(sizes)):
+            xk = xv[doff[k]:doff[k + 1]]
+            assert np.linalg.norm(xk - ref[k]) <= 1e-8 * np.linalg.norm(ref[k]), (mode, k)
+        for _ in range(2):  # capture, then replay
+            x.fill(0.0)
+            batch.assemble_pcg(plan, fe, *coefs, b, x, rtol=1e-12, mode=mode)
+            assert np.array_equal(x.numpy(), xv), mode
+    for h in (batch, x, b, theta, plan, fe, g):
+        h.close()
+
+
 def test_batched_fp64_symmetry_change_between_solves(ctx):
     # one batch, one solution vector, the same chunk length: the values first
     # bitwise symmetric (alpha = 1 on a jittered mesh: G_i . G_j commutes), so
