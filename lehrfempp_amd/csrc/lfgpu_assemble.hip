@@ -358,7 +358,8 @@ __device__ __forceinline__ void edge_record_row(const AsmArgs &a, const ETab<NLE
   edge_row<NLE, NQE>(ET, seg_length(pa, pb), gk, i, row);
 }
 
-template <int NLOC, int NQP, int MS, int BLOCK, bool EDGES>
+// PRE: the cell coefficients went through the per-cell pass
+template <int NLOC, int NQP, int MS, int BLOCK, bool EDGES, bool PRE>
 __global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
   constexpr int NLE = NLOC == 3 ? 2 : 4, NQE = NLE;
   extern __shared__ double s_acc[];
@@ -392,20 +393,29 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
         const CellXY c = load_cell(a.xy, a.cell_v, cell);
         Geo g;
         tria_geo(c.x0, c.y0, c.x1, c.y1, c.x2, c.y2, g);
-        double ak[NQP], gk[NQP];
-        coeff_qp<NQP>(a.alpha, cell, c, s_tab, ak);
-        coeff_qp<NQP>(a.gamma, cell, c, s_tab, gk);
         double row[NLOC];
-        element_row<NLOC, NQP>(T, g, ak, gk, i, row);
+        if constexpr (PRE) {
+          element_row_f<NLOC, NQP>(T, g, PlainCoef(a.alpha, cell, NQP),
+                                   PlainCoef(a.gamma, cell, NQP), i, row);
+        } else {
+          double ak[NQP], gk[NQP];
+          coeff_qp<NQP>(a.alpha, cell, c, s_tab, ak);
+          coeff_qp<NQP>(a.gamma, cell, c, s_tab, gk);
+          element_row<NLOC, NQP>(T, g, ak, gk, i, row);
+        }
 #pragma unroll
         for (int j = 0; j < NLOC; ++j) {
           const uint8_t s = mt[1 + j];
           if (s != kDrop) s_acc[off + s] += row[j];
         }
         if (a.has_f) {
-          double fk[NQP];
-          coeff_qp<NQP>(a.f, cell, c, s_tab, fk);
-          b += element_load<NLOC, NQP>(T, g, fk, i);
+          if constexpr (PRE) {
+            b += element_load_f<NLOC, NQP>(T, g, PlainCoef(a.f, cell, NQP), i);
+          } else {
+            double fk[NQP];
+            coeff_qp<NQP>(a.f, cell, c, s_tab, fk);
+            b += element_load<NLOC, NQP>(T, g, fk, i);
+          }
         }
       }
       if constexpr (EDGES) {
@@ -454,11 +464,16 @@ __global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
           const CellXY c = load_cell(a.xy, a.cell_v, cell);
           Geo g;
           tria_geo(c.x0, c.y0, c.x1, c.y1, c.x2, c.y2, g);
-          double ak[NQP], gk[NQP];
-          coeff_qp<NQP>(a.alpha, cell, c, s_tab, ak);
-          coeff_qp<NQP>(a.gamma, cell, c, s_tab, gk);
           double row[NLOC];
-          element_row<NLOC, NQP>(T, g, ak, gk, mt[0], row);
+          if constexpr (PRE) {
+            element_row_f<NLOC, NQP>(T, g, PlainCoef(a.alpha, cell, NQP),
+                                     PlainCoef(a.gamma, cell, NQP), mt[0], row);
+          } else {
+            double ak[NQP], gk[NQP];
+            coeff_qp<NQP>(a.alpha, cell, c, s_tab, ak);
+            coeff_qp<NQP>(a.gamma, cell, c, s_tab, gk);
+            element_row<NLOC, NQP>(T, g, ak, gk, mt[0], row);
+          }
 #pragma unroll
           for (int j = 0; j < NLOC; ++j)
             if (mt[1 + j] == kDrop)
@@ -544,21 +559,21 @@ int precompute_coeffs(hipStream_t s, lfgpu_plan *p, AsmArgs &a) {
   return LFGPU_OK;
 }
 
-template <int NLOC, int NQP, int MS, bool EDGES>
+template <int NLOC, int NQP, int MS, bool EDGES, bool PRE>
 int launch_assemble(hipStream_t s, const lfgpu_plan *p, const AsmArgs &a) {
   const size_t lds = (size_t)std::max(p->max_block_nnz, 1) * sizeof(double);
   const unsigned grid = grid_for(a.nrows, p->block);
   switch (p->block) {
     case 256:
-      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 256, EDGES>), dim3(grid), dim3(256), lds,
+      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 256, EDGES, PRE>), dim3(grid), dim3(256), lds,
                          s, a);
       break;
     case 128:
-      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 128, EDGES>), dim3(grid), dim3(128), lds,
+      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 128, EDGES, PRE>), dim3(grid), dim3(128), lds,
                          s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 64, EDGES>), dim3(grid), dim3(64), lds, s,
+      hipLaunchKernelGGL((k_assemble_rows<NLOC, NQP, MS, 64, EDGES, PRE>), dim3(grid), dim3(64), lds, s,
                          a);
       break;
   }
@@ -735,16 +750,20 @@ int asm_args(lfgpu_plan *p, const lfgpu_fe *fe, const lfgpu_coeff *alpha, const 
   return LFGPU_OK;
 }
 
-// The row-owner kernel, at P3 after the per-cell coefficient pass: a P3
-// cell is visited by up to 10 rows, and the pass took the 512^2 P3
-// assembly from 1.32 to 1.08 ms.  A P1 cell is visited by 3 rows, and the
-// pass measured no gain there (0.80 vs 0.81 ms at 2048^2): that kernel waits
-// on its dependent record loads more than on the transcendental functions.
+// P3: the per-cell coefficient pass, then the row-owner kernel built for
+// the plain coefficient kinds it leaves.  A P3 cell is visited by up to 10
+// rows; and without the costly kinds' code the kernel needs 135 registers
+// instead of 256 plus 1.5 KB of spills per lane (512^2 P3: 1.32 ms without
+// the pass, 1.08 ms with the pass and the generic kernel, 0.83 ms now).
+// P1: the generic kernel.  A P1 cell is visited by 3 rows, and the pass
+// with the plain kernel (102 instead of 169 registers, twice the resident
+// waves) measured slower at 2048^2 (0.89 vs 0.79 ms): the pass's own
+// traffic and the extra gathers outweigh the work saved.
 template <bool EDGES>
 int queue_assembly(hipStream_t s, lfgpu_plan *p, AsmArgs a) {
-  if (p->mesh->nloc == 3) return launch_assemble<3, 3, 4, EDGES>(s, p, a);
+  if (p->mesh->nloc == 3) return launch_assemble<3, 3, 4, EDGES, false>(s, p, a);
   LFG_TRY(precompute_coeffs<12>(s, p, a));
-  return launch_assemble<10, 12, 12, EDGES>(s, p, a);
+  return launch_assemble<10, 12, 12, EDGES, true>(s, p, a);
 }
 
 template <bool EDGES>

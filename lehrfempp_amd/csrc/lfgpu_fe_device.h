@@ -114,77 +114,105 @@ struct Tab {
 // Row i of ReactionDiffusionElementMatrixProvider::Eval
 // (uscalfe/loc_comp_ellbvp.h:324-337):
 //   mat(i,j) = sum_k w_k ( G_i . (alpha_k G_j) + (gamma_k N_i) N_j ),
-//   w_k = omega_k |det J|, G = J^{-T} grad N^T.
+//   w_k = omega_k |det J|, G = J^{-T} grad N^T,
+// with alpha_k = ak(k), gamma_k = gk(k).
+// P1: the reference gradients of the linear Lagrange functions are the same
+// at every quadrature point (the table holds equal values), so each
+// G_j = J^{-T} grad N_j is formed once, with the products of the general
+// loop, and reused at every point: the same values, a third of the work.
+template <int NLOC, int NQP, class AK, class GK>
+__device__ __forceinline__ void element_row_f(const Tab<NLOC, NQP> &T, const Geo &g,
+                                              const AK &ak, const GK &gk, int i, double *row) {
+#pragma unroll
+  for (int j = 0; j < NLOC; ++j) row[j] = 0.0;
+  if constexpr (NLOC == 3 && NQP == 3) {
+    double G0[3], G1[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double gxj = T.dx(j, 0), gyj = T.dy(j, 0);
+      G0[j] = g.jit00 * gxj + g.jit01 * gyj;
+      G1[j] = g.jit10 * gxj + g.jit11 * gyj;
+    }
+    double Gi0 = G0[0], Gi1 = G1[0];
+#pragma unroll
+    for (int j = 1; j < 3; ++j)
+      if (i == j) {
+        Gi0 = G0[j];
+        Gi1 = G1[j];
+      }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double w = T.w(k) * g.det;
+      const double a = ak(k), c = gk(k);
+      const double cNi = c * T.N(i, k);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double s = Gi0 * (a * G0[j]) + Gi1 * (a * G1[j]);
+        const double m = cNi * T.N(j, k);
+        row[j] += w * (s + m);
+      }
+    }
+  } else {
+    for (int k = 0; k < NQP; ++k) {
+      const double w = T.w(k) * g.det;
+      const double gxi = T.dx(i, k), gyi = T.dy(i, k);
+      const double Gi0 = g.jit00 * gxi + g.jit01 * gyi;
+      const double Gi1 = g.jit10 * gxi + g.jit11 * gyi;
+      const double a = ak(k), c = gk(k);
+      const double cNi = c * T.N(i, k);
+#pragma unroll
+      for (int j = 0; j < NLOC; ++j) {
+        const double gxj = T.dx(j, k), gyj = T.dy(j, k);
+        const double Gj0 = g.jit00 * gxj + g.jit01 * gyj;
+        const double Gj1 = g.jit10 * gxj + g.jit11 * gyj;
+        const double s = Gi0 * (a * Gj0) + Gi1 * (a * Gj1);
+        const double m = cNi * T.N(j, k);
+        row[j] += w * (s + m);
+      }
+    }
+  }
+}
+
 template <int NLOC, int NQP>
 __device__ __forceinline__ void element_row(const Tab<NLOC, NQP> &T, const Geo &g,
                                             const double *ak, const double *gk, int i,
                                             double *row) {
-#pragma unroll
-  for (int j = 0; j < NLOC; ++j) row[j] = 0.0;
-  for (int k = 0; k < NQP; ++k) {
-    const double w = T.w(k) * g.det;
-    const double gxi = T.dx(i, k), gyi = T.dy(i, k);
-    const double Gi0 = g.jit00 * gxi + g.jit01 * gyi;
-    const double Gi1 = g.jit10 * gxi + g.jit11 * gyi;
-    const double a = ak[k], c = gk[k];
-    const double cNi = c * T.N(i, k);
-#pragma unroll
-    for (int j = 0; j < NLOC; ++j) {
-      const double gxj = T.dx(j, k), gyj = T.dy(j, k);
-      const double Gj0 = g.jit00 * gxj + g.jit01 * gyj;
-      const double Gj1 = g.jit10 * gxj + g.jit11 * gyj;
-      const double s = Gi0 * (a * Gj0) + Gi1 * (a * Gj1);
-      const double m = cNi * T.N(j, k);
-      row[j] += w * (s + m);
-    }
-  }
+  element_row_f<NLOC, NQP>(T, g, [&](int k) { return ak[k]; }, [&](int k) { return gk[k]; }, i,
+                           row);
 }
 
-// P1: the reference gradients of the linear Lagrange functions are the same
-// at every quadrature point (the table holds equal values), so each
-// G_j = J^{-T} grad N_j is formed once, with the products of the general
-// loop, and reused at every point: the same values, one third of the work.
-template <>
-__device__ __forceinline__ void element_row<3, 3>(const Tab<3, 3> &T, const Geo &g,
-                                                  const double *ak, const double *gk, int i,
-                                                  double *row) {
-  double G0[3], G1[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const double gxj = T.dx(j, 0), gyj = T.dy(j, 0);
-    G0[j] = g.jit00 * gxj + g.jit01 * gyj;
-    G1[j] = g.jit10 * gxj + g.jit11 * gyj;
-    row[j] = 0.0;
-  }
-  double Gi0 = G0[0], Gi1 = G1[0];
-#pragma unroll
-  for (int j = 1; j < 3; ++j)
-    if (i == j) {
-      Gi0 = G0[j];
-      Gi1 = G1[j];
-    }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const double w = T.w(k) * g.det;
-    const double a = ak[k], c = gk[k];
-    const double cNi = c * T.N(i, k);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const double s = Gi0 * (a * G0[j]) + Gi1 * (a * G1[j]);
-      const double m = cNi * T.N(j, k);
-      row[j] += w * (s + m);
-    }
-  }
+// Entry i of ScalarLoadElementVectorProvider::Eval (loc_comp_ellbvp.h:733-740),
+// f_k = fk(k).
+template <int NLOC, int NQP, class FK>
+__device__ __forceinline__ double element_load_f(const Tab<NLOC, NQP> &T, const Geo &g,
+                                                 const FK &fk, int i) {
+  double v = 0.0;
+  for (int k = 0; k < NQP; ++k) v += (T.w(k) * g.det * fk(k)) * T.N(i, k);
+  return v;
 }
 
-// Entry i of ScalarLoadElementVectorProvider::Eval (loc_comp_ellbvp.h:733-740).
 template <int NLOC, int NQP>
 __device__ __forceinline__ double element_load(const Tab<NLOC, NQP> &T, const Geo &g,
                                                const double *fk, int i) {
-  double v = 0.0;
-  for (int k = 0; k < NQP; ++k) v += (T.w(k) * g.det * fk[k]) * T.N(i, k);
-  return v;
+  return element_load_f<NLOC, NQP>(T, g, [&](int k) { return fk[k]; }, i);
 }
+
+// Value at point k of a coefficient of kind CONST, CELL or QP on one cell
+// (the expressions of coeff_at: value, value * data[cell],
+// value * data[cell * nqp + k]).
+struct PlainCoef {
+  double v0;
+  const double *q;  // the cell's point values (QP), else null
+  __device__ __forceinline__ PlainCoef(const CoeffD &c, int64_t cell, int nqp) {
+    v0 = c.value;
+    q = nullptr;
+    if (c.kind == LFGPU_COEFF_CELL)
+      v0 = c.value * c.data[cell];
+    else if (c.kind == LFGPU_COEFF_QP)
+      q = c.data + cell * nqp;
+  }
+  __device__ __forceinline__ double operator()(int k) const { return q ? v0 * q[k] : v0; }
+};
 
 // Cell data gathered once per (cell) visit.
 struct CellXY {
