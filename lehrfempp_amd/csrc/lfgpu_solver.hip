@@ -600,14 +600,7 @@ int detect_dia(lfgpu_ctx *ctx, const lfgpu_csr *A, DiagSet *ds) {
 // the (equally permuted) vector instead of from three distant ranges; the
 // length order keeps the 64-row slices of rows of one kind, so they carry
 // no more padding than in the original order.
-#ifndef LFGPU_ELL_WINDOW_BITS
-#define LFGPU_ELL_WINDOW_BITS 9
-#endif
-constexpr int kEllWindowBits = LFGPU_ELL_WINDOW_BITS;
-#ifndef LFGPU_ELL16
-#define LFGPU_ELL16 1
-#endif
-constexpr bool kEll16 = LFGPU_ELL16;
+constexpr int kEllWindowBits = 9;
 
 int ell_locality_order(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_solver_cache *c) {
   hipStream_t s = ctx->stream;
@@ -680,7 +673,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
       c->mlen = (int64_t)ds.ndiag * n;
     } else {
       c->fmt = FMT_ELL;
-      if (allow_perm && kEllWindowBits < 31 && n > 0) LFG_TRY(ell_locality_order(ctx, A, c));
+      if (allow_perm && n > 0) LFG_TRY(ell_locality_order(ctx, A, c));
       c->nslices = (n + kSlice - 1) / kSlice;
       int32_t *w = nullptr;
       LFG_TRY(dalloc(&w, (size_t)c->nslices, "slice widths"));
@@ -709,7 +702,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
       if (e == hipSuccess) e = hipStreamSynchronize(s);
       dfree(flag);
       if (e != hipSuccess) return hip_fail(e, "ELL column span");
-      if (hflag == 0 && kEll16) {
+      if (hflag == 0) {
         LFG_TRY(dalloc(&c->s_dcol, (size_t)std::max<int64_t>(len, 1), "ELL column offsets"));
         hipLaunchKernelGGL(k_fill_ell_cols<int16_t>, dim3(grid_for(n, 256)), dim3(256), 0, s,
                            A->row_ptr, A->col, n, c->slice_off, c->perm, c->iperm, c->s_dcol);
