@@ -358,9 +358,14 @@ __device__ __forceinline__ void edge_record_row(const AsmArgs &a, const ETab<NLE
   edge_row<NLE, NQE>(ET, seg_length(pa, pb), gk, i, row);
 }
 
-// PRE: the cell coefficients went through the per-cell pass
+// PRE: the cell coefficients went through the per-cell pass.  The P1
+// kernel with the costly coefficient kinds is compute-bound (exp and sines
+// recomputed on every visit of a cell): asking for 3 waves per SIMD caps it
+// at 168 registers (from 169 it would use, with 2 spilled) and measured 13%
+// faster on the 2048x2048 mesh than 2 waves, with the same bits.
 template <int NLOC, int NQP, int MS, int BLOCK, bool EDGES, bool PRE>
-__global__ __launch_bounds__(BLOCK) void k_assemble_rows(AsmArgs a) {
+__global__ __launch_bounds__(BLOCK, (NLOC == 3 && !PRE && BLOCK == 256 ? 3 : 1)) void
+k_assemble_rows(AsmArgs a) {
   constexpr int NLE = NLOC == 3 ? 2 : 4, NQE = NLE;
   extern __shared__ double s_acc[];
   __shared__ double s_tab[Tab<NLOC, NQP>::kLen];

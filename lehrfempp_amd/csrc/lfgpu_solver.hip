@@ -2522,12 +2522,16 @@ template <int FMT>
 void launch_bcg_chunk(hipStream_t s, const BArgs &a, int count, int symmode) {
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
-    if (FMT == FMT_BDIA && symmode == 1)
-      hipLaunchKernelGGL((k_bcg_spmv<FMT, 1>), dim3(a.ngroups), dim3(256), 0, s, a, par);
-    else if (FMT == FMT_BDIA && symmode == 2)
-      hipLaunchKernelGGL((k_bcg_spmv<FMT, 2>), dim3(a.ngroups), dim3(256), 0, s, a, par);
-    else
+    if constexpr (FMT == FMT_BDIA) {
+      if (symmode == 1)
+        hipLaunchKernelGGL((k_bcg_spmv<FMT, 1>), dim3(a.ngroups), dim3(256), 0, s, a, par);
+      else if (symmode == 2)
+        hipLaunchKernelGGL((k_bcg_spmv<FMT, 2>), dim3(a.ngroups), dim3(256), 0, s, a, par);
+      else
+        hipLaunchKernelGGL(k_bcg_spmv<FMT>, dim3(a.ngroups), dim3(256), 0, s, a, par);
+    } else {
       hipLaunchKernelGGL(k_bcg_spmv<FMT>, dim3(a.ngroups), dim3(256), 0, s, a, par);
+    }
     hipLaunchKernelGGL(k_bcg_update, dim3(a.ngroups), dim3(256), 0, s, a, par);
   }
 }

@@ -667,6 +667,65 @@ def test_pcg_fp64_on_symmetric_and_asymmetric_dia_values(ctx, jitter):
         h.close()
 
 
+def _grid_laplacian(m, seed, zero_pair, lower_zero):
+    """COO of an SPD 5-point operator on an m x m grid (edge weights
+    exp(N(0,1)), a small shift on the diagonal) plus one explicit zero pair
+    (i, i+1) / (i+1, i) between the last node of a grid line and the first
+    of the next, stored as +0.0 above the diagonal and `lower_zero` below."""
+    rng = np.random.default_rng(seed)
+    n = m * m
+    rows, cols, vals = [], [], []
+    diag = np.full(n, 1e-3)
+    for di, dj in ((0, 1), (1, 0)):
+        ii, jj = np.meshgrid(np.arange(m - di), np.arange(m - dj), indexing="ij")
+        a = (ii * m + jj).ravel()
+        b = ((ii + di) * m + jj + dj).ravel()
+        w = np.exp(rng.standard_normal(a.size))
+        rows += [a, b]
+        cols += [b, a]
+        vals += [-w, -w]
+        np.add.at(diag, a, w)
+        np.add.at(diag, b, w)
+    i = zero_pair
+    rows += [np.arange(n), np.array([i, i + 1])]
+    cols += [np.arange(n), np.array([i + 1, i])]
+    vals += [diag, np.array([0.0, lower_zero])]
+    return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals), n
+
+
+@pytest.mark.parametrize("mode", [capi.PCG_FP64, capi.PCG_MIXED, capi.PCG_MIXED_MATRIX])
+def test_dia_upper_diagonal_reads_give_the_full_read_iterates(ctx, mode):
+    # on DIA values that are symmetric bit for bit the FP64 product sweep
+    # reads only the upper diagonals (the lower ones mirrored); one explicit
+    # zero stored as -0.0 below the diagonal and +0.0 above makes the values
+    # asymmetric bit for bit, so that copy takes the full read, with the same
+    # terms in the same order: the iterates must agree bit for bit, iteration
+    # counts included.  (The mixed modes read every diagonal of either copy:
+    # the same comparison holds for them.)
+    m = 65  # odd: the middle row, from which the DIA offsets are read, is interior
+    i = 5 * m - 1
+    out = []
+    for lower_zero in (0.0, -0.0):
+        r, c, v, n = _grid_laplacian(m, 31, i, lower_zero)
+        A = capi.coo_to_csr(ctx, r, c, v, n, n)
+        b = capi.Vec(ctx, np.random.default_rng(32).standard_normal(n))
+        x = capi.Vec(ctx, n)
+        info = capi.pcg(ctx, A, b, x, rtol=1e-10, mode=mode)
+        st, nd, sym = capi.solver_storage(A)
+        assert st == capi.STORAGE_DIA and nd == 4, (st, nd)
+        assert sym == (np.signbit(lower_zero) == False), (sym, lower_zero)
+        assert info.converged and info.iters > 100, info.iters
+        out.append((info.iters, info.refinements, x.numpy().copy()))
+        if mode != capi.PCG_FP64:
+            rp, col, val = A.download()
+            res = b.numpy() - L.csr_spmv(rp, col, val, out[-1][2])
+            assert np.linalg.norm(res) <= 1.05e-10 * np.linalg.norm(b.numpy())
+        for h in (x, b, A):
+            h.close()
+    assert out[0][:2] == out[1][:2], (out[0][:2], out[1][:2])
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 def _batch_system(ctx, sizes, extra=None, seed=23):
     """Block-diagonal P1 system of tp meshes of the given sizes, optionally
     followed by one unstructured mesh; returns handles and offsets."""
