@@ -873,6 +873,125 @@ int lfref_edge_vectors(int degree, int nedges, const double *xy, const int32_t *
   return 0;
 }
 
+// FisherKPP's nonlocal boundary kernel (projects/fisher_kpp/
+// modelproblem_island_main.cc:76-80, strangsplitting_main.cc:160-165,
+// convergence_tau_main.cc:108-113): g(x, y) = 1 / (1 + |x - y|^2), and with
+// rmax > 0 zero unless rmin <= |x - y| <= rmax.
+static double nl_kernel(const double *x, const double *y, double rmin, double rmax) {
+  const double dx = x[0] - y[0], dy = x[1] - y[1];
+  const double s = dx * dx + dy * dy;  // (x - y).squaredNorm()
+  if (rmax > 0.0) {
+    const double d = std::sqrt(s);  // (x - y).norm()
+    if (!(d >= rmin && d <= rmax)) return 0.0;
+  }
+  return 1.0 / (1.0 + s);
+}
+
+// The gain h(x) = localQuadFunction(mesh, {segment rule}, g(x, .), 1,
+// edge_pred) (strangsplitting.h:78-115): sum over the flagged edges in index
+// order and their points l of (w_l g(x, y_l)) det.
+static double nl_gain_at(const SegTables &t, int nedges, const double *xy, const int32_t *edge_v,
+                         const uint8_t *flags, double rmin, double rmax, const double *x) {
+  double sum = 0.0;
+  for (int e = 0; e < nedges; ++e) {
+    if (!flags[e]) continue;
+    const double *a = P(xy, edge_v[2 * e]), *b = P(xy, edge_v[2 * e + 1]);
+    const double det = seg_length(a, b);
+    for (int l = 0; l < t.nqp; ++l) {
+      const double y[2] = {b[0] * t.pts[l] + a[0] * (1 - t.pts[l]),
+                           b[1] * t.pts[l] + a[1] * (1 - t.pts[l])};
+      sum += t.w[l] * nl_kernel(x, y, rmin, rmax) * det;
+    }
+  }
+  return sum;
+}
+
+int lfref_nonlocal_gain(int degree, int nedges, const double *xy, const int32_t *edge_v,
+                        const uint8_t *flags, double rmin, double rmax, double *out) {
+  if (degree != 1 && degree != 3) return -1000;
+  SegTables t;
+  make_seg_tables(degree, t);
+  for (int e = 0; e < nedges; ++e) {
+    const double *a = P(xy, edge_v[2 * e]), *b = P(xy, edge_v[2 * e + 1]);
+    for (int k = 0; k < t.nqp; ++k) {
+      // MassEdgeMatrixProvider evaluates h at SegmentO1::Global of its points
+      const double x[2] = {b[0] * t.pts[k] + a[0] * (1 - t.pts[k]),
+                           b[1] * t.pts[k] + a[1] * (1 - t.pts[k])};
+      out[(int64_t)e * t.nqp + k] =
+          flags[e] ? nl_gain_at(t, nedges, xy, edge_v, flags, rmin, rmax, x) : 0.0;
+    }
+  }
+  return 0;
+}
+
+// The loss matrix L of modelproblem_island_main.cc:92-126: column j (a DOF
+// of a flagged edge) is AssembleVectorLocally(1, ...) of the edge load with
+// f = L_j, L_j(x) = [AssembleVectorLocally(1, ...) of the edge load with
+// f = g(x, .)]_j; both loads are ScalarLoadEdgeVectorProvider::Eval
+// (uscalfe/loc_comp_ellbvp.h:905-916) over the flagged edges in index order.
+// rows_out receives the DOFs of the flagged edges in ascending order; L_out
+// the nb x nb block on them (row-major); returns nb, or -1 if it exceeds
+// max_nb.
+int lfref_nonlocal_loss(int degree, int nedges, const double *xy, const int32_t *edge_v,
+                        const int64_t *edge_dofs, const uint8_t *flags, double rmin, double rmax,
+                        int max_nb, int32_t *rows_out, double *L_out) {
+  if (degree != 1 && degree != 3) return -1000;
+  SegTables t;
+  make_seg_tables(degree, t);
+  const int nle = t.nsf, nq = t.nqp;
+  // the flagged edges of every DOF, in index order, with its local index
+  std::map<int64_t, std::vector<std::pair<int, int>>> dof_edges;
+  for (int e = 0; e < nedges; ++e)
+    if (flags[e])
+      for (int m = 0; m < nle; ++m) dof_edges[edge_dofs[(int64_t)e * nle + m]].push_back({e, m});
+  const int nb = (int)dof_edges.size();
+  if (nb > max_nb) return -1;
+  std::vector<int64_t> rows;
+  for (const auto &kv : dof_edges) rows.push_back(kv.first);
+  for (int b = 0; b < nb; ++b) rows_out[b] = (int32_t)rows[b];
+  auto qp = [&](int e, int k, double *x) {
+    const double *a = P(xy, edge_v[2 * e]), *b = P(xy, edge_v[2 * e + 1]);
+    x[0] = b[0] * t.pts[k] + a[0] * (1 - t.pts[k]);
+    x[1] = b[1] * t.pts[k] + a[1] * (1 - t.pts[k]);
+  };
+  auto det_of = [&](int e) { return seg_length(P(xy, edge_v[2 * e]), P(xy, edge_v[2 * e + 1])); };
+  // L_j(x): entry j of the assembled edge load of g(x, .)
+  auto Lj = [&](int bj, const double *x) {
+    double acc = 0.0;
+    for (const auto &em : dof_edges[rows[bj]]) {
+      const int e = em.first, m = em.second;
+      const double det = det_of(e);
+      double v = 0.0;
+      for (int l = 0; l < nq; ++l) {
+        double y[2];
+        qp(e, l, y);
+        const double w = (t.w[l] * det) * nl_kernel(x, y, rmin, rmax);
+        v += t.N[nle * l + m] * w;
+      }
+      acc += v;
+    }
+    return acc;
+  };
+  for (int bi = 0; bi < nb; ++bi)
+    for (int bj = 0; bj < nb; ++bj) {
+      double acc = 0.0;
+      for (const auto &em : dof_edges[rows[bi]]) {
+        const int e = em.first, m = em.second;
+        const double det = det_of(e);
+        double v = 0.0;
+        for (int k = 0; k < nq; ++k) {
+          double x[2];
+          qp(e, k, x);
+          const double w = (t.w[k] * det) * Lj(bj, x);
+          v += t.N[nle * k + m] * w;
+        }
+        acc += v;
+      }
+      L_out[(int64_t)bi * nb + bj] = acc;
+    }
+  return nb;
+}
+
 int lfref_error_norms(int degree, int ncells, const double *xy,
                       const int32_t *cell_v, const int64_t *cell_dofs,
                       const double *uh, int exact_kind, const double *u_qp,

@@ -191,6 +191,26 @@ int lfref_edge_vectors(int degree, int nedges, const double *xy,
                        const int32_t *edge_v, int g_kind, double g_const,
                        const double *g_data, double *out);
 
+/* FisherKPP's nonlocal boundary terms on the edges with flags[e] != 0, for
+ * the kernel g(x, y) = 1 / (1 + |x - y|^2), zero outside rmin <= |x - y| <=
+ * rmax when rmax > 0 (projects/fisher_kpp/modelproblem_island_main.cc:70-126
+ * and the annulus variants of strangsplitting_main.cc, convergence_*_main.cc):
+ *   gain: h(x) = localQuadFunction(mesh, {segment rule}, g(x, .), 1,
+ *     edge_pred) (strangsplitting.h:78-115) at the segment rule's points of
+ *     every flagged edge, out[e * nqp + k] (0 on the other edges);
+ *   loss: the matrix L, column j = AssembleVectorLocally(1, ...) of the edge
+ *     load of L_j(x) = [AssembleVectorLocally(1, ...) of the edge load of
+ *     g(x, .)]_j, on the DOFs of the flagged edges (edge_dofs [nedges x
+ *     (degree + 1)]): their ascending list in rows_out and the nb x nb block
+ *     (row-major) in L_out; returns nb (-1 above max_nb). */
+int lfref_nonlocal_gain(int degree, int nedges, const double *xy,
+                        const int32_t *edge_v, const uint8_t *flags, double rmin,
+                        double rmax, double *out);
+int lfref_nonlocal_loss(int degree, int nedges, const double *xy,
+                        const int32_t *edge_v, const int64_t *edge_dofs,
+                        const uint8_t *flags, double rmin, double rmax, int max_nb,
+                        int32_t *rows_out, double *L_out);
+
 /* Squared error norms of a finite element function u_h (DOF vector uh) against
  * an exact solution, as the demo drivers compute them
  * (examples/ellbvp_linfe/homDir_linfe_demo.cc:179-188):

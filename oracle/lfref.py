@@ -66,6 +66,10 @@ def lib():
                                               C.c_void_p, _d]),
             "lfref_edge_vectors": (C.c_int, [C.c_int, C.c_int, _d, _i32, C.c_int, C.c_double,
                                              C.c_void_p, _d]),
+            "lfref_nonlocal_gain": (C.c_int, [C.c_int, C.c_int, _d, _i32, _u8, C.c_double,
+                                              C.c_double, _d]),
+            "lfref_nonlocal_loss": (C.c_int, [C.c_int, C.c_int, _d, _i32, _i64, _u8,
+                                              C.c_double, C.c_double, C.c_int, _i32, _d]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -425,6 +429,35 @@ def edge_vectors(xy, edge_v, degree, kind=G_CONST, const=1.0, data=None):
     n = degree + 1
     return _edge_call(lib().lfref_edge_vectors, degree, xy, edge_v, kind, const, data,
                       n).reshape(-1, n)
+
+
+def nonlocal_gain(xy, edge_v, degree, flags, rmin=0.0, rmax=0.0):
+    """FisherKPP's gain h at the segment rule's points of every flagged edge
+    ([nedges, nqp], zero elsewhere; lfref_nonlocal_gain)."""
+    ev = np.ascontiguousarray(edge_v, np.int32)
+    nq = degree + 1
+    out = np.zeros(ev.shape[0] * nq)
+    rc = lib().lfref_nonlocal_gain(degree, ev.shape[0], np.ascontiguousarray(xy, np.float64).ravel(),
+                                   ev.ravel(), np.ascontiguousarray(flags, np.uint8),
+                                   float(rmin), float(rmax), out)
+    assert rc == 0, rc
+    return out.reshape(-1, nq)
+
+
+def nonlocal_loss(xy, edge_v, edge_dofs, degree, flags, rmin=0.0, rmax=0.0):
+    """FisherKPP's loss matrix on the DOFs of the flagged edges: (rows, L)
+    with L [nb, nb] (lfref_nonlocal_loss)."""
+    ev = np.ascontiguousarray(edge_v, np.int32)
+    ed = np.ascontiguousarray(edge_dofs, np.int64)
+    fl = np.ascontiguousarray(flags, np.uint8)
+    cap = int(len(np.unique(ed[fl != 0]))) if np.any(fl) else 0
+    rows = np.zeros(max(cap, 1), np.int32)
+    L = np.zeros(max(cap, 1) ** 2)
+    nb = lib().lfref_nonlocal_loss(degree, ev.shape[0],
+                                   np.ascontiguousarray(xy, np.float64).ravel(), ev.ravel(),
+                                   ed.ravel(), fl, float(rmin), float(rmax), cap, rows, L)
+    assert nb == cap, (nb, cap)
+    return rows[:nb].copy(), L[:nb * nb].reshape(nb, nb)
 
 
 def eta_sub(coef, sel):
