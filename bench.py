@@ -1026,16 +1026,33 @@ def plain_fp64_per_problem(args, capi, ctx, sizes, g, coff, voff):
             "iters_sum": int(sum(its)), "x": x_all}
 
 
-def _fkpp_setup(capi, ctx, n, c=1.2, h=1.0):
+def _fkpp_setup(capi, ctx, n, c=1.2, h=1.0, nonlocal_terms=False):
+    """Mesh, edges, A, M and u0 of the Fisher/KPP workload; with
+    nonlocal_terms the boundary condition of modelproblem_island_main.cc
+    (gain h(x) and loss L of g = 1 / (1 + |x - y|^2)) replaces h."""
     from lehrfempp_amd import fisher_kpp
     xy, cells, _ = capi.tp_triag_mesh(n)
     ev, ce, _ = capi.mesh_edges(xy.shape[0], cells)
     mesh = capi.Mesh(ctx, xy, cells)
     edges = capi.Edges(ctx, mesh, 1, ev)
     ef = fisher_kpp.boundary_edge_flags(ev.shape[0], ce)
-    A, M = fisher_kpp.assemble_galerkin_matrices(ctx, mesh, edges, ef,
-                                                 capi.coeff(capi.COEFF_CONST, c),
-                                                 capi.coeff(capi.COEFF_CONST, h))
+    if nonlocal_terms:
+        ctx.sync()
+        t0 = time.perf_counter()
+        hc, eta = fisher_kpp.nonlocal_gain(ctx, edges, ef)
+        loss = fisher_kpp.nonlocal_loss(ctx, edges, ef)
+        ctx.sync()
+        log("fkpp nonlocal terms: %d boundary DOFs, gain and loss in %.1f ms"
+            % (loss[0].size, 1e3 * (time.perf_counter() - t0)))
+        A, M = fisher_kpp.assemble_galerkin_matrices(ctx, mesh, edges, ef,
+                                                     capi.coeff(capi.COEFF_CONST, c), hc,
+                                                     L=loss)
+        eta.close()
+        loss[1].close()
+    else:
+        A, M = fisher_kpp.assemble_galerkin_matrices(ctx, mesh, edges, ef,
+                                                     capi.coeff(capi.COEFF_CONST, c),
+                                                     capi.coeff(capi.COEFF_CONST, h))
     u0 = np.zeros(xy.shape[0])
     r2 = (xy[:, 0] - 0.3) ** 2 + (xy[:, 1] - 0.4) ** 2
     u0[r2 < 0.01] = 0.3
@@ -1048,7 +1065,9 @@ def run_fkpp(args, capi, ctx):
     h = 1 on the boundary, carrying capacity K = 1, u0 = 0.3 on a disc."""
     from lehrfempp_amd import fisher_kpp
     n, m_steps, T, lam = args.fkpp_n, args.fkpp_steps, 1.0, 2.1
-    mesh, edges, A, M, u0 = _fkpp_setup(capi, ctx, n)
+    mesh, edges, A, M, u0 = _fkpp_setup(capi, ctx, n, nonlocal_terms=args.fkpp_nonlocal)
+    if args.fkpp_nonlocal:
+        args.fkpp_mode, args.fkpp_modes = "fp64", "fp64"  # the block needs FP64
     nv = u0.size
     codes = {"fp64": capi.PCG_FP64, "mixed": capi.PCG_MIXED,
              "mixed_matrix": capi.PCG_MIXED_MATRIX}
@@ -1088,9 +1107,12 @@ def run_fkpp(args, capi, ctx):
     res = {"metric": "Fisher/KPP Strang splitting, ms per time step", "unit": "ms",
            "value": 1e3 * wall / nd, "higher_is_better": False,
            "config": {"workload": "Fisher/KPP on %dx%d P1 structured triangles, c=1.2, "
-                                  "lambda=2.1, Robin h=1, K=1, T=1, %d steps, SDIRK-2 "
+                                  "lambda=2.1, %s, K=1, T=1, %d steps, SDIRK-2 "
                                   "diffusion + exact logistic reaction, stage solves by "
-                                  "Jacobi-PCG to 1e-10" % (n, n, m_steps),
+                                  "Jacobi-PCG to 1e-10"
+                                  % (n, n, "nonlocal gain h and loss L of "
+                                     "g = 1/(1+|x-y|^2)" if args.fkpp_nonlocal
+                                     else "Robin h=1", m_steps),
                       "dofs": nv, "nnz": int(A.info()[2])},
            "pcg_mode": args.fkpp_mode,
            "evolution_s": wall, "device_ms": dev_ms, "diffusion_steps": nd,
@@ -1101,7 +1123,7 @@ def run_fkpp(args, capi, ctx):
         % (n, res["value"], res["iters_per_solve"], max_it))
     for hh in (vK, vu, A, M, edges, mesh):
         hh.close()
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.fkpp_nonlocal:
         res["cpu_baseline"] = fkpp_cpu_sample(capi, ctx, args.fkpp_sample_n, m_steps)
     return res
 
@@ -1345,6 +1367,8 @@ def main():
     ap.add_argument("--fkpp-steps", type=int, default=100)
     ap.add_argument("--fkpp-sample-n", type=int, default=256)
     ap.add_argument("--fkpp-mode", choices=["fp64", "mixed", "mixed_matrix"], default="fp64")
+    ap.add_argument("--fkpp-nonlocal", action="store_true",
+                    help="the nonlocal boundary terms (gain h, loss block L) instead of Robin h=1")
     ap.add_argument("--fkpp-modes", default="fp64,mixed,mixed_matrix",
                     help="fkpp: comma-separated precision modes to run")
     args = ap.parse_args()

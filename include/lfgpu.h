@@ -503,6 +503,47 @@ int lfgpu_csr_clone(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_csr **out);
 int lfgpu_csr_axpby(lfgpu_ctx *ctx, double a, const lfgpu_csr *X, double b,
                     const lfgpu_csr *Y, lfgpu_csr **out);
 
+/* ---- nonlocal boundary terms of the Fisher/KPP model ------------------- */
+/* The kernel g(x, y) = 1 / (1 + |x - y|^2) of the model's nonlocal boundary
+ * condition (projects/fisher_kpp/modelproblem_island_main.cc:76-80); with
+ * rmax > 0 it is 0 unless rmin <= |x - y| <= rmax (the annulus of
+ * strangsplitting_main.cc:160-165 and convergence_tau_main.cc:108-113).
+ * flags [nedges] (host) select the boundary edges (the drivers' edge_pred).
+ *
+ * Gain: eta [nedges x nqe] receives h(x) = localQuadFunction(mesh, {segment
+ * rule}, g(x, .), 1, edge_pred) (strangsplitting.h:78-115) at the segment
+ * rule's points of every flagged edge (0 on the others), the coefficient of
+ * the drivers' MassEdgeMatrixProvider(fe_space, mf_h, edge_pred): pass it as
+ * an LFGPU_COEFF_QP coefficient to lfgpu_assemble_galerkin_bd. */
+int lfgpu_nonlocal_gain(lfgpu_ctx *ctx, const lfgpu_edges *e,
+                        const uint8_t *flags, double rmin, double rmax,
+                        lfgpu_vec *eta);
+/* Loss: the matrix L the drivers build column by column
+ * (modelproblem_island_main.cc:92-126: column j = AssembleVectorLocally(1,
+ * ...) of the edge load of L_j(x) = [AssembleVectorLocally(1, ...) of the
+ * edge load of g(x, .)]_j), restricted to the nb DOFs of the flagged edges
+ * (every other entry is 0): *nb receives nb, rows (host, capacity nb) those
+ * DOFs in ascending order and L (device, >= nb^2 values) the block,
+ * row-major.  rows = L = NULL queries nb only. */
+int lfgpu_nonlocal_loss(lfgpu_ctx *ctx, const lfgpu_edges *e,
+                        const uint8_t *flags, double rmin, double rmax,
+                        int32_t *nb, int32_t *rows, lfgpu_vec *L);
+/* Attaches a dense block to A: A then stands for (its sparse part) +
+ * scale * B, B [nb x nb] row-major on the rows and columns rows[0..nb)
+ * (host, ascending), copied.  This is the dense `A_ds - L` of
+ * strangsplitting.h:166-170 (scale -1, B = L) without densifying the sparse
+ * part.  lfgpu_spmv, lfgpu_pcg in mode LFGPU_PCG_FP64 (two sweeps; its
+ * Jacobi entries include scale * B's diagonal), lfgpu_csr_clone and
+ * lfgpu_csr_axpby (the block of the one operand that has one, scaled by its
+ * factor) honour it; lfgpu_csr_download returns the sparse part; the other
+ * modes, the batch and lfgpu_pcg_noalloc reject such a matrix.  nb = 0
+ * removes the block. */
+int lfgpu_csr_set_dense_block(lfgpu_ctx *ctx, lfgpu_csr *A, int32_t nb,
+                              const int32_t *rows, const lfgpu_vec *B,
+                              double scale);
+/* The attached block's size and scale (0 and 0.0 without one). */
+int lfgpu_csr_dense_block(const lfgpu_csr *A, int32_t *nb, double *scale);
+
 /* ---- small BLAS-1 helpers used by drivers ------------------------------- */
 /* y = a*x + b*y */
 int lfgpu_axpby(lfgpu_ctx *ctx, double a, const lfgpu_vec *x, double b,

@@ -22,7 +22,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LFGPU_ARCH", "gfx950")
 
 SOURCES = ["lfgpu_core.hip", "lfgpu_assemble.hip", "lfgpu_mfma.hip", "lfgpu_solver.hip",
-           "lfgpu_sparse.hip", "lfgpu_post.hip", "lfgpu_edges.hip", "lfgpu_timestep.hip"]
+           "lfgpu_sparse.hip", "lfgpu_post.hip", "lfgpu_edges.hip", "lfgpu_timestep.hip",
+           "lfgpu_nonlocal.hip"]
 HEADERS = ["lfgpu_internal.h", "lfgpu_fe_device.h", "lfgpu_reduce.h"]
 
 FLAGS = [

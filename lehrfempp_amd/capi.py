@@ -128,6 +128,13 @@ SIGNATURES = {
     "lfgpu_strang_evolution": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(StrangInfo)]),
     "lfgpu_csr_clone": (C.c_int, [_P, _P, C.POINTER(_P)]),
     "lfgpu_csr_axpby": (C.c_int, [_P, C.c_double, _P, C.c_double, _P, C.POINTER(_P)]),
+    "lfgpu_nonlocal_gain": (C.c_int, [_P, _P, C.POINTER(C.c_uint8), C.c_double, C.c_double,
+                                      _P]),
+    "lfgpu_nonlocal_loss": (C.c_int, [_P, _P, C.POINTER(C.c_uint8), C.c_double, C.c_double,
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P]),
+    "lfgpu_csr_set_dense_block": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32), _P,
+                                            C.c_double]),
+    "lfgpu_csr_dense_block": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "lfgpu_axpby": (C.c_int, [_P, C.c_double, _P, C.c_double, _P]),
     "lfgpu_dot": (C.c_int, [_P, _P, _P, _d]),
 }
@@ -568,6 +575,49 @@ def csr_clone(ctx, A: Csr) -> Csr:
     h = _P()
     check(lib().lfgpu_csr_clone(ctx.h, A.h, C.byref(h)))
     return Csr(ctx, h, owned=True)
+
+
+def nonlocal_gain(ctx, edges: Edges, flags, rmin=0.0, rmax=0.0) -> Vec:
+    """lfgpu_nonlocal_gain: h at the segment points of the flagged edges, a
+    device vector [nedges * nqe] for an LFGPU_COEFF_QP edge coefficient."""
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    assert fl.size == edges.n
+    nqe = edges.nle  # degree + 1 points for degree 1 and 3
+    eta = Vec(ctx, edges.n * nqe)
+    check(lib().lfgpu_nonlocal_gain(ctx.h, edges.h, _ptr(fl, C.c_uint8), float(rmin),
+                                    float(rmax), eta.h))
+    return eta
+
+
+def nonlocal_loss(ctx, edges: Edges, flags, rmin=0.0, rmax=0.0):
+    """lfgpu_nonlocal_loss: (rows, L) with rows the DOFs of the flagged edges
+    (ascending) and L a device vector holding the nb x nb block row-major."""
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    assert fl.size == edges.n
+    nb = C.c_int32()
+    check(lib().lfgpu_nonlocal_loss(ctx.h, edges.h, _ptr(fl, C.c_uint8), float(rmin),
+                                    float(rmax), C.byref(nb), None, None))
+    rows = np.zeros(max(nb.value, 1), np.int32)
+    L = Vec(ctx, max(nb.value, 1) ** 2)
+    check(lib().lfgpu_nonlocal_loss(ctx.h, edges.h, _ptr(fl, C.c_uint8), float(rmin),
+                                    float(rmax), C.byref(nb), _ptr(rows, C.c_int32), L.h))
+    return rows[:nb.value], L
+
+
+def csr_set_dense_block(ctx, A: Csr, rows, B: Vec | None, scale=1.0):
+    """lfgpu_csr_set_dense_block: A stands for its sparse part + scale * B on
+    rows (ascending); rows of length 0 removes the block."""
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    check(lib().lfgpu_csr_set_dense_block(ctx.h, A.h, int(r.size),
+                                          _ptr(r, C.c_int32) if r.size else None,
+                                          None if B is None else B.h, float(scale)))
+
+
+def csr_dense_block(A: Csr):
+    """(nb, scale) of the dense block attached to A."""
+    nb, sc = C.c_int32(), C.c_double()
+    check(lib().lfgpu_csr_dense_block(A.h, C.byref(nb), C.byref(sc)))
+    return nb.value, sc.value
 
 
 def csr_axpby(ctx, a, X: Csr, b, Y: Csr) -> Csr:

@@ -115,6 +115,13 @@ struct lfgpu_csr {
   bool owned_by_plan = false;
   lfgpu_solver_cache *cache = nullptr;  // built lazily by the solver
   uint64_t val_version = 0;             // bumped whenever val changes
+  // Optional dense block (lfgpu_csr_set_dense_block): the matrix stands for
+  // its sparse part + blk_scale * B, B [blk_n x blk_n] row-major on the rows
+  // and columns blk_rows (ascending), both device arrays owned here.
+  int32_t blk_n = 0;
+  int32_t *blk_rows = nullptr;
+  double *blk = nullptr;
+  double blk_scale = 0.0;
 };
 
 struct lfgpu_plan {
@@ -169,6 +176,13 @@ int element_contract(hipStream_t s, const lfgpu_mesh *m, const lfgpu_fe *fe, con
 // Release the edge records of a plan (lfgpu_edges.hip).
 void plan_free_edges(lfgpu_plan *p);
 inline void csr_values_changed(lfgpu_csr *A) { A->val_version++; }
+// Dense block of a CSR (lfgpu_nonlocal.hip): copy of from's block scaled by
+// `scale` into to (which must have none); release.
+int csr_copy_block(lfgpu_ctx *ctx, const lfgpu_csr *from, double scale, lfgpu_csr *to);
+void csr_free_block(lfgpu_csr *A);
+// y[rows[b]] += scale (B x[rows])_b for the block of A, queued on stream s
+// (CSR order x and y).
+void csr_block_gemv_add(hipStream_t s, const lfgpu_csr *A, const double *x, double *y);
 // y = A x (FP64) on the context stream, without synchronisation; builds the
 // solver's storage of A on first use (lfgpu_solver.hip).
 int spmv_async(lfgpu_ctx *ctx, lfgpu_csr *A, const double *x, double *y);

@@ -112,6 +112,12 @@ struct lfgpu_solver_cache {
   // solver works on permuted copies of x and b
   int32_t *perm = nullptr, *iperm = nullptr;
   double *xp = nullptr, *bp = nullptr;
+  // the CSR's dense block in storage order: blk_srow[c] is the storage row
+  // of block row c, blk_smap[r] the block row of storage row r (-1: none),
+  // blk_t the block product of the current sweep; blk_n 0: no block
+  int32_t blk_n = 0, blk_cap = 0;
+  int32_t *blk_srow = nullptr, *blk_smap = nullptr;
+  double *blk_t = nullptr;
 };
 
 namespace lfgpu {
@@ -126,7 +132,8 @@ void solver_cache_free(lfgpu_csr *A) {
                   (void *)c->r, (void *)c->p, (void *)c->q, (void *)c->r32, (void *)c->p32,
                   (void *)c->q32, (void *)c->xa32, (void *)c->xlo, (void *)c->r64, (void *)c->xa64,
                   (void *)c->part, (void *)c->cg1, (void *)c->rpart, (void *)c->st,
-                  (void *)c->perm, (void *)c->iperm, (void *)c->xp, (void *)c->bp})
+                  (void *)c->perm, (void *)c->iperm, (void *)c->xp, (void *)c->bp,
+                  (void *)c->blk_srow, (void *)c->blk_smap, (void *)c->blk_t})
     dfree(p);
   delete c;
   A->cache = nullptr;
@@ -649,6 +656,88 @@ void fill_vals_enqueue(hipStream_t s, const lfgpu_csr *A, lfgpu_solver_cache *c)
                      c->dinv, c->have32 ? c->dinv32 : nullptr, c->have32 ? c->dinv16 : nullptr);
 }
 
+// The dense block of a CSR (lfgpu_csr_set_dense_block) as the FP64 sweeps
+// see it: the matrix is its sparse part + scale * B on the storage rows
+// srow; t = B v of the sweep's vector v, added to row r's product when
+// smap[r] >= 0.
+struct BlkView {
+  const double *B = nullptr;
+  const int32_t *srow = nullptr, *smap = nullptr;
+  double *t = nullptr;
+  double scale = 0.0;
+  int32_t n = 0;
+};
+
+__global__ void k_fill_i32(int32_t *a, int32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// storage rows of the dense block's rows (and the inverse map)
+__global__ void k_blk_map(const int32_t *rows, int32_t nb, const int32_t *iperm, int32_t *srow,
+                          int32_t *smap) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  const int32_t r = iperm ? iperm[rows[c]] : rows[c];
+  srow[c] = r;
+  smap[r] = c;
+}
+
+// Jacobi entries of the block rows: 1 / (a_rr + scale B_cc), or 1 for a
+// zero diagonal, as k_fill_vals takes 1 / a_rr
+__global__ void k_blk_dinv(const int32_t *row_ptr, const int32_t *col, const double *val,
+                           const int32_t *rows, int32_t nb, const double *B, double scale,
+                           const int32_t *srow, double *dinv) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  const int32_t r = rows[c];
+  double arr = 0.0;
+  for (int32_t p = row_ptr[r]; p < row_ptr[r + 1]; ++p)
+    if (col[p] == r) arr = val[p];
+  const double d = arr + scale * B[(int64_t)c * nb + c];
+  dinv[srow[c]] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// The dense block of A in the storage's row order, and its Jacobi entries
+// (after the values' refresh on the same stream).
+int block_setup(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_solver_cache *c) {
+  hipStream_t s = ctx->stream;
+  const int32_t n = A->nrows, nb = A->blk_n;
+  c->blk_n = nb;
+  if (nb == 0) return LFGPU_OK;
+  if (!c->blk_smap) LFG_TRY(dalloc(&c->blk_smap, (size_t)std::max(n, 1), "block row map"));
+  if (nb > c->blk_cap) {
+    dfree(c->blk_srow);
+    dfree(c->blk_t);
+    c->blk_srow = nullptr;
+    c->blk_t = nullptr;
+    c->blk_cap = 0;
+    LFG_TRY(dalloc(&c->blk_srow, (size_t)nb, "block storage rows"));
+    LFG_TRY(dalloc(&c->blk_t, (size_t)nb, "block product"));
+    c->blk_cap = nb;
+  }
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, c->blk_smap, -1,
+                     (int64_t)n);
+  hipLaunchKernelGGL(k_blk_map, dim3(grid_for(nb, 256)), dim3(256), 0, s, A->blk_rows, nb,
+                     c->iperm, c->blk_srow, c->blk_smap);
+  hipLaunchKernelGGL(k_blk_dinv, dim3(grid_for(nb, 256)), dim3(256), 0, s, A->row_ptr, A->col,
+                     A->val, A->blk_rows, nb, A->blk, A->blk_scale, c->blk_srow, c->dinv);
+  LFG_HIP(hipGetLastError());
+  return LFGPU_OK;
+}
+
+BlkView blk_view(const lfgpu_csr *A, const lfgpu_solver_cache *c) {
+  BlkView b;
+  if (c->blk_n == 0) return b;
+  b.B = A->blk;
+  b.srow = c->blk_srow;
+  b.smap = c->blk_smap;
+  b.t = c->blk_t;
+  b.scale = A->blk_scale;
+  b.n = c->blk_n;
+  return b;
+}
+
 // Builds (or refreshes) the solver's storage of A.  allow_perm: a new
 // sliced-ELL storage may keep its rows in the locality order; the batched
 // solver needs the original row order (its problems own row ranges) and
@@ -741,6 +830,7 @@ int ensure_cache(lfgpu_ctx *ctx, lfgpu_csr *A, bool need32, bool allow_perm) {
     fill_vals_enqueue(s, A, c);
     LFG_HIP(hipGetLastError());
     LFG_TRY(dia_symmetry(ctx, c, n));
+    LFG_TRY(block_setup(ctx, A, c));
     c->built_version = A->val_version;
   }
   return LFGPU_OK;
@@ -1002,11 +1092,46 @@ struct CgArgs {
   double *part;
   PcgState *st;
   Sweep sw;
+  BlkView blk;  // n == 0: none
 };
+
+// t = B v[srow] for the dense block, one wave per block row, each row summed
+// in a fixed order (lanes over the columns, then a butterfly); returns at
+// once when the solve has stopped (st non-null).
+__global__ __launch_bounds__(256) void k_blk_gemv(BlkView b, const double *__restrict__ v,
+                                                  const PcgState *st) {
+  if (st && st->done) return;
+  const int32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= b.n) return;
+  const double *__restrict__ Br = b.B + (int64_t)row * b.n;
+  double acc = 0.0;
+  for (int32_t c = lane; c < b.n; c += 64) acc += Br[c] * v[b.srow[c]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) b.t[row] = acc;
+}
+
+inline void launch_blk_gemv(hipStream_t s, const BlkView &b, const double *v,
+                            const PcgState *st) {
+  hipLaunchKernelGGL(k_blk_gemv, dim3(grid_for(b.n, 4)), dim3(256), 0, s, b, v, st);
+}
+
+template <bool BLK>
+__device__ __forceinline__ double blk_term(const BlkView &b, int32_t r) {
+  if constexpr (BLK) {
+    const int32_t k = b.smap[r];
+    return k >= 0 ? b.scale * b.t[k] : 0.0;
+  } else {
+    return 0.0;
+  }
+}
 
 // setup: r = b - A x, p = D^-1 r; partials of b.b, r.r, r.D^-1 r (the parity
 // 1 slots, which iteration 0 reads)
-template <int FMT, int SB>
+// (BLK: the dense block's product B x of the preceding k_blk_gemv is added
+// to A x)
+template <int FMT, int SB, bool BLK = false>
 __global__ __launch_bounds__(SB) void k_cg_init(CgArgs a, const double *__restrict__ b) {
   __shared__ double sh[4 * (SB / 64)];
   const int32_t r0 = sweep_chunk(blockIdx.x, a.sw.ngroups) * a.sw.rows_per_group;
@@ -1015,7 +1140,7 @@ __global__ __launch_bounds__(SB) void k_cg_init(CgArgs a, const double *__restri
   for (int32_t i = r0 + threadIdx.x; i < r1; i += SB) {
     const double xi = a.x[i];
     const double bi = b[i];
-    const double ri = bi - apply64<FMT>(a.m, a.x, i, xi);
+    const double ri = bi - (apply64<FMT>(a.m, a.x, i, xi) + blk_term<BLK>(a.blk, i));
     a.r[i] = ri;
     const double zi = a.dinv[i] * ri;
     a.p[i] = zi;
@@ -1071,7 +1196,9 @@ __global__ void k_zero_if_bb0(double *x, int32_t n, const PcgState *st) {
 // FP64 iteration k, sweep 1 (par = k & 1): Eigen's convergence test on r_k
 // (||r_k||^2 left by the previous sweep 2), then q = A p with the partials
 // p.q, q.D^-1 r and q.D^-1 q.
-template <int FMT, int SB, bool SYM = false>
+// BLK: the dense block's product B p of the preceding k_blk_gemv is added to
+// the sparse part's.
+template <int FMT, int SB, bool SYM = false, bool BLK = false>
 __global__ __launch_bounds__(SB) void k_cg_spmv(CgArgs a, int par) {
   __shared__ double sh[4 * (SB / 64)];
   PcgState *st = a.st;
@@ -1101,7 +1228,7 @@ __global__ __launch_bounds__(SB) void k_cg_spmv(CgArgs a, int par) {
     // loads before the store: the compiler keeps stores and later loads in
     // program order
     const double pi = a.p[i], di = a.dinv[i], ri = a.r[i];
-    const double qi = apply64<FMT, true, SYM>(a.m, a.p, i, pi);
+    const double qi = apply64<FMT, true, SYM>(a.m, a.p, i, pi) + blk_term<BLK>(a.blk, i);
     // in the 512-thread sweeps (2^21 rows and more) q goes past the caches:
     // it is read once, by the update sweep (2% faster at 2048^2); the same
     // store measured slower in the mixed and row sweeps
@@ -2107,6 +2234,20 @@ void launch_cg_chunk(hipStream_t s, const Sweep &sw, const CgArgs &a, int count,
   const dim3 G(sw.ngroups), B(SB);
   for (int k = 0; k < count; ++k) {
     const int par = k & 1;
+    if (a.blk.n > 0) {
+      // the dense block: t = B p before the product sweep adds it
+      launch_blk_gemv(s, a.blk, a.p, a.st);
+      tmark_begin(s, 0);
+      if (FMT == FMT_DIA && sym)
+        hipLaunchKernelGGL((k_cg_spmv<FMT, SB, true, true>), G, B, 0, s, a, par);
+      else
+        hipLaunchKernelGGL((k_cg_spmv<FMT, SB, false, true>), G, B, 0, s, a, par);
+      tmark_end(s);
+      tmark_begin(s, 1);
+      hipLaunchKernelGGL(k_cg_update<SB>, G, B, 0, s, a, par);
+      tmark_end(s);
+      continue;
+    }
     tmark_begin(s, 0);
     if (FMT == FMT_DIA && sym)
       hipLaunchKernelGGL((k_cg_spmv<FMT, SB, true>), G, B, 0, s, a, par);
@@ -2153,7 +2294,9 @@ bool same_view(const MatView &x, const MatView &y) {
 bool same_args(const CgArgs &x, const CgArgs &y) {
   return same_view(x.m, y.m) && x.dinv == y.dinv && x.x == y.x && x.r == y.r && x.p == y.p &&
          x.q == y.q && x.part == y.part && x.st == y.st && x.sw.n == y.sw.n &&
-         x.sw.rows_per_group == y.sw.rows_per_group && x.sw.ngroups == y.sw.ngroups;
+         x.sw.rows_per_group == y.sw.rows_per_group && x.sw.ngroups == y.sw.ngroups &&
+         x.blk.B == y.blk.B && x.blk.srow == y.blk.srow && x.blk.smap == y.blk.smap &&
+         x.blk.t == y.blk.t && x.blk.scale == y.blk.scale && x.blk.n == y.blk.n;
 }
 
 bool same_args(const RowArgs &x, const RowArgs &y) {
@@ -3420,7 +3563,9 @@ int batch_run_mixed(lfgpu_batch *bt, const double *b, double *x, const lfgpu_pcg
 namespace lfgpu {
 int spmv_async(lfgpu_ctx *ctx, lfgpu_csr *A, const double *x, double *y) {
   LFG_TRY(ensure_cache(ctx, A, false, true));
-  return launch_spmv(ctx->stream, A->cache, A->nrows, x, y);
+  LFG_TRY(launch_spmv(ctx->stream, A->cache, A->nrows, x, y));
+  if (A->blk_n > 0) csr_block_gemv_add(ctx->stream, A, x, y);
+  return LFGPU_OK;
 }
 }  // namespace lfgpu
 
@@ -3433,6 +3578,7 @@ extern "C" int lfgpu_spmv(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *
   LFG_TRY(ensure_cache(ctx, A, false, true));
   LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
   LFG_TRY(launch_spmv(ctx->stream, A->cache, A->nrows, x->d, y->d));
+  if (A->blk_n > 0) csr_block_gemv_add(ctx->stream, A, x->d, y->d);
   LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0.f;
@@ -3451,7 +3597,10 @@ extern "C" int lfgpu_spmv_timed(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu
   LFG_REQUIRE(A->nrows == A->ncols, LFGPU_EINVAL, "SpMV needs a square matrix");
   LFG_TRY(ensure_cache(ctx, A, false, true));
   LFG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-  for (int k = 0; k < reps; ++k) LFG_TRY(launch_spmv(ctx->stream, A->cache, A->nrows, x->d, y->d));
+  for (int k = 0; k < reps; ++k) {
+    LFG_TRY(launch_spmv(ctx->stream, A->cache, A->nrows, x->d, y->d));
+    if (A->blk_n > 0) csr_block_gemv_add(ctx->stream, A, x->d, y->d);
+  }
   LFG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
   LFG_HIP(hipEventSynchronize(ctx->ev1));
   float ms = 0.f;
@@ -3499,6 +3648,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   LFG_REQUIRE(opts->mode == LFGPU_PCG_FP64 || opts->mode == LFGPU_PCG_MIXED ||
                   opts->mode == LFGPU_PCG_MIXED_MATRIX,
               LFGPU_EINVAL, "unknown PCG mode");
+  LFG_REQUIRE(A->blk_n == 0 || (opts->mode == LFGPU_PCG_FP64 && !opts->single_reduction),
+              LFGPU_EINVAL,
+              "a matrix with a dense block is solved in FP64 mode with two sweeps only");
   const bool mixed = opts->mode != LFGPU_PCG_FP64;
   LFG_TRY(ensure_cache(ctx, A, mixed, true));
   lfgpu_solver_cache *c = A->cache;
@@ -3543,7 +3695,9 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   // in the locality order its gathers stay in L2 there too, and it saves
   // the two reduction launches per iteration (2.4M-DOF P3: 82.6 against
   // 90.2 us per iteration; FP64 measured 155.5 against 141.9 us chunked)
-  const bool rowsweep = !single && c->fmt == FMT_ELL && opts->mode != LFGPU_PCG_MIXED;
+  // (a dense block runs the chunked sweeps, which add its product)
+  const bool rowsweep =
+      !single && c->fmt == FMT_ELL && opts->mode != LFGPU_PCG_MIXED && c->blk_n == 0;
   const int nwg = (int)((n + kRB - 1) / kRB);
   if (rowsweep && !c->rpart)
     LFG_TRY(dalloc(&c->rpart, (size_t)3 * std::max(nwg, 1), "row-sweep partials"));
@@ -3660,7 +3814,23 @@ extern "C" int lfgpu_pcg(lfgpu_ctx *ctx, const lfgpu_csr *Ac, const lfgpu_vec *b
   // its true residual from
   double *r_init = opts->mode == LFGPU_PCG_MIXED_MATRIX ? c->r64 : c->r;
   CgArgs a{mv, c->dinv, xd, r_init, c->p, c->q, c->part, c->st, sw};
-  if (sb == kSBMix) {
+  a.blk = blk_view(A, c);
+  if (a.blk.n > 0) {
+    // FP64 only, so the sweeps are 512 wide here unless the matrix is a
+    // DIA one below 2^21 rows
+    launch_blk_gemv(s, a.blk, xd, nullptr);
+    const dim3 B(sb);
+    if (sb == kSBMix && c->fmt == FMT_DIA)
+      hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSBMix, true>), G, B, 0, s, a, bd);
+    else if (sb == kSBMix)
+      hipLaunchKernelGGL((k_cg_init<FMT_ELL, kSBMix, true>), G, B, 0, s, a, bd);
+    else
+      hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSB64, true>), G, B, 0, s, a, bd);
+    if (sb == kSBMix)
+      hipLaunchKernelGGL(k_cg_init_finish<kSBMix>, dim3(1), B, 0, s, a);
+    else
+      hipLaunchKernelGGL(k_cg_init_finish<kSB64>, dim3(1), B, 0, s, a);
+  } else if (sb == kSBMix) {
     const dim3 B(kSBMix);
     if (c->fmt == FMT_DIA)
       hipLaunchKernelGGL((k_cg_init<FMT_DIA, kSBMix>), G, B, 0, s, a, bd);
@@ -3709,6 +3879,7 @@ extern "C" int lfgpu_batch_create(lfgpu_ctx *ctx, lfgpu_csr *A, int nprob,
   *out = nullptr;
   LFG_REQUIRE(A->nrows == A->ncols && row_offsets[0] == 0 && row_offsets[nprob] == A->nrows,
               LFGPU_EINVAL, "row offsets must partition the rows of a square matrix");
+  LFG_REQUIRE(A->blk_n == 0, LFGPU_EINVAL, "a batch cannot be made of a matrix with a dense block");
   for (int p = 0; p < nprob; ++p)
     LFG_REQUIRE(row_offsets[p + 1] >= row_offsets[p], LFGPU_EINVAL, "row offsets must increase");
   lfgpu_batch *bt = new lfgpu_batch();
@@ -4239,6 +4410,8 @@ extern "C" int lfgpu_pcg_noalloc(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfgpu
               "PCG size mismatch");
   LFG_REQUIRE(b->d != x->d, LFGPU_EINVAL, "PCG right-hand side and solution must differ");
   LFG_REQUIRE(opts->rtol >= 0.0 && opts->maxit >= 0, LFGPU_EINVAL, "bad PCG options");
+  LFG_REQUIRE(A->blk_n == 0, LFGPU_EINVAL,
+              "lfgpu_pcg_noalloc does not take a matrix with a dense block");
   LFG_REQUIRE(opts->mode == LFGPU_PCG_FP64 && !opts->single_reduction, LFGPU_EINVAL,
               "lfgpu_pcg_noalloc solves in FP64 with two sweeps per iteration only");
   LFG_REQUIRE(A->nrows <= kNoAllocMaxRows, LFGPU_ERANGE,

@@ -72,6 +72,7 @@ extern "C" int lfgpu_csr_destroy(lfgpu_csr *A) {
   LFG_REQUIRE(!A->owned_by_plan, LFGPU_EINVAL, "this CSR belongs to a plan");
   if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
   solver_cache_free(A);
+  csr_free_block(A);
   dfree(A->row_ptr);
   dfree(A->col);
   dfree(A->val);
@@ -337,8 +338,14 @@ extern "C" int lfgpu_csr_clone(lfgpu_ctx *ctx, const lfgpu_csr *A, lfgpu_csr **o
   LFG_REQUIRE(ctx && A && out, LFGPU_EINVAL, "NULL argument to lfgpu_csr_clone");
   *out = nullptr;
   LFG_TRY(csr_new_like(ctx, A, true, out));
-  LFG_HIP(hipStreamSynchronize(ctx->stream));
-  return LFGPU_OK;
+  int rc = A->blk_n > 0 ? csr_copy_block(ctx, A, A->blk_scale, *out) : LFGPU_OK;
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (rc == LFGPU_OK && e != hipSuccess) rc = hip_fail(e, "lfgpu_csr_clone");
+  if (rc != LFGPU_OK) {
+    lfgpu_csr_destroy(*out);
+    *out = nullptr;
+  }
+  return rc;
 }
 
 extern "C" int lfgpu_csr_axpby(lfgpu_ctx *ctx, double a, const lfgpu_csr *X, double b,
@@ -347,6 +354,8 @@ extern "C" int lfgpu_csr_axpby(lfgpu_ctx *ctx, double a, const lfgpu_csr *X, dou
   *out = nullptr;
   LFG_REQUIRE(X->nrows == Y->nrows && X->ncols == Y->ncols && X->nnz == Y->nnz, LFGPU_EINVAL,
               "lfgpu_csr_axpby needs matrices of equal size and nonzero count");
+  LFG_REQUIRE(X->blk_n == 0 || Y->blk_n == 0, LFGPU_EINVAL,
+              "lfgpu_csr_axpby: at most one of the matrices may carry a dense block");
   hipStream_t s = ctx->stream;
   if (X->row_ptr != Y->row_ptr || X->col != Y->col) {
     int *bad = nullptr;
@@ -371,10 +380,14 @@ extern "C" int lfgpu_csr_axpby(lfgpu_ctx *ctx, double a, const lfgpu_csr *X, dou
     hipLaunchKernelGGL(k_vals_axpby, dim3(grid_for(X->nnz, 256)), dim3(256), 0, s, a, X->val, b,
                        Y->val, Z->val, X->nnz);
   hipError_t e = hipGetLastError();
+  // the block of the operand that has one, scaled by its factor
+  int rc = LFGPU_OK;
+  if (e == hipSuccess && X->blk_n > 0) rc = csr_copy_block(ctx, X, a * X->blk_scale, Z);
+  if (e == hipSuccess && Y->blk_n > 0) rc = csr_copy_block(ctx, Y, b * Y->blk_scale, Z);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
+  if (e != hipSuccess || rc != LFGPU_OK) {
     lfgpu_csr_destroy(Z);
-    return hip_fail(e, "lfgpu_csr_axpby");
+    return rc != LFGPU_OK ? rc : hip_fail(e, "lfgpu_csr_axpby");
   }
   *out = Z;
   return LFGPU_OK;

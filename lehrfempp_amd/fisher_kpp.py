@@ -21,17 +21,57 @@ def boundary_edge_flags(nedges, cell_e):
     return (cnt == 1).astype(np.uint8)
 
 
+def nonlocal_gain(ctx, edges: capi.Edges, bd_edge_flags, rmin=0.0, rmax=0.0):
+    """The gain h of the drivers' nonlocal boundary condition
+    (modelproblem_island_main.cc:70-90: localQuadFunction of g(x, .) over the
+    boundary edges, strangsplitting.h:78-115) at the edge points, as the
+    LFGPU_COEFF_QP coefficient of the edge mass term; g is 1 / (1 + |x-y|^2),
+    cut to rmin <= |x - y| <= rmax when rmax > 0.  Returns (coefficient,
+    device vector); keep the vector while the coefficient is used."""
+    eta = capi.nonlocal_gain(ctx, edges, bd_edge_flags, rmin, rmax)
+    return capi.coeff(capi.COEFF_QP, 1.0, eta), eta
+
+
+def nonlocal_loss(ctx, edges: capi.Edges, bd_edge_flags, rmin=0.0, rmax=0.0):
+    """The loss matrix L of the drivers (modelproblem_island_main.cc:92-126)
+    on the DOFs of the boundary edges: (rows, device vector of the nb x nb
+    block), the form assemble_galerkin_matrices takes for L."""
+    return capi.nonlocal_loss(ctx, edges, bd_edge_flags, rmin, rmax)
+
+
+def _loss_block(ctx, L, n):
+    """(rows, device block, owned) of L given as (rows, block) or as the
+    reference's dense N x N matrix (owned: the block vector was made here);
+    None when L is absent or zero."""
+    if L is None:
+        return None
+    if isinstance(L, tuple):
+        rows, B = L
+        rows = np.ascontiguousarray(rows, np.int32)
+        if isinstance(B, capi.Vec):
+            return rows, B, False
+        return rows, capi.Vec(ctx, np.ascontiguousarray(B, np.float64).ravel()), True
+    Ld = np.asarray(L, np.float64)
+    if Ld.shape != (n, n):
+        raise ValueError("L must be N x N, or (rows, block)")
+    rows = np.nonzero(np.any(Ld != 0, axis=0) | np.any(Ld != 0, axis=1))[0].astype(np.int32)
+    if rows.size == 0:
+        return None
+    return rows, capi.Vec(ctx, np.ascontiguousarray(Ld[np.ix_(rows, rows)]).ravel()), True
+
+
 def assemble_galerkin_matrices(ctx, mesh: capi.Mesh, edges: capi.Edges, bd_edge_flags,
                                c: capi.Coeff, h: capi.Coeff, L=None):
     """(A, M) of assembleGalerkinMatrices (strangsplitting.h:123-174) for the
     P1 space: A = ReactionDiffusionElementMatrixProvider(c, 0) plus
-    MassEdgeMatrixProvider(h) on the boundary edges, M =
-    ReactionDiffusionElementMatrixProvider(0, 1), both from one plan without
-    fixed DOFs, returned as owned capi.Csr copies.  The nonlocal loss matrix
-    L (a dense N x N matrix subtracted from A in the reference) must be None
-    or all zeros."""
-    if L is not None and np.any(np.asarray(L) != 0):
-        raise NotImplementedError("a nonzero dense loss matrix L is not supported")
+    MassEdgeMatrixProvider(h) on the boundary edges, minus the nonlocal loss
+    L, and M = ReactionDiffusionElementMatrixProvider(0, 1), both from one
+    plan without fixed DOFs, returned as owned capi.Csr copies.  L is None,
+    the reference's dense N x N matrix, or (rows, block) as nonlocal_loss
+    returns it; it is attached to A as a dense block with factor -1 (the
+    reference's A_ds - L, :166-170), which the FP64 solver and the time
+    stepper honour."""
+    blk = _loss_block(ctx, L, mesh.ndofs)
     fe = capi.FE(ctx, 1)
     plan = capi.Plan(ctx, mesh, np.zeros(mesh.ndofs, np.uint8))
     try:
@@ -41,9 +81,13 @@ def assemble_galerkin_matrices(ctx, mesh: capi.Mesh, edges: capi.Edges, bd_edge_
         plan.assemble(fe, capi.coeff(capi.COEFF_CONST, 0.0), capi.coeff(capi.COEFF_CONST, 1.0),
                       None, None)
         M = capi.csr_clone(ctx, plan.csr)
+        if blk is not None:
+            capi.csr_set_dense_block(ctx, A, blk[0], blk[1], -1.0)
     finally:
         plan.close()
         fe.close()
+        if blk is not None and blk[2]:
+            blk[1].close()
     return A, M
 
 

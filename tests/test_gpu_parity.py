@@ -1658,3 +1658,137 @@ def test_strang_invalid_arguments(ctx):
     ss.close()
     for h in (small, ok, A, M, E, g):
         h.close()
+
+
+# ----------------------------------------------- Fisher/KPP nonlocal terms
+@pytest.mark.parametrize("degree,radii", [(1, (0.0, 0.0)), (1, (0.25, 0.7)), (3, (0.0, 0.0))])
+def test_nonlocal_gain_and_loss_bitexact(ctx, degree, radii):
+    # the gain h at the boundary points and the loss block L (the drivers'
+    # nested edge loads, modelproblem_island_main.cc:70-126) equal the
+    # oracle's bit for bit, with and without the annulus cut
+    m = jittered_mesh(9, amount=0.3, seed=4)
+    _, ef = m.boundary_flags()
+    g = gpu_mesh(ctx, m, degree)
+    ed = m.edge_dofs(degree)
+    E = capi.Edges(ctx, g, degree, m.edges, None if degree == 1 else ed)
+    eta = capi.nonlocal_gain(ctx, E, ef, *radii)
+    want = L.nonlocal_gain(m.xy, m.edges, degree, ef, *radii)
+    assert np.array_equal(eta.numpy().reshape(want.shape), want)
+    rows, Lv = capi.nonlocal_loss(ctx, E, ef, *radii)
+    wrows, wL = L.nonlocal_loss(m.xy, m.edges, ed, degree, ef, *radii)
+    assert np.array_equal(rows, wrows)
+    assert np.array_equal(Lv.numpy().reshape(wL.shape), wL)
+    if radii[1] > 0:
+        assert np.count_nonzero(wL == 0.0) > 0 and np.count_nonzero(wL) > 0
+    # nothing flagged: no rows, a zero gain
+    none = np.zeros_like(ef)
+    assert capi.nonlocal_loss(ctx, E, none)[0].size == 0
+    z = capi.nonlocal_gain(ctx, E, none)
+    assert not np.any(z.numpy())
+    for h in (z, Lv, eta, E, g):
+        h.close()
+
+
+def _nonlocal_system(ctx, n, c=1.2):
+    """Device A (with the loss block attached) and M of the nonlocal model on
+    a jittered n x n mesh, and the oracle's dense A - L and M (scipy)."""
+    from lehrfempp_amd import fisher_kpp
+    import scipy.sparse as sp
+    m = jittered_mesh(n, amount=0.3, seed=6)
+    _, ef = m.boundary_flags()
+    g = capi.Mesh(ctx, m.xy, m.cells)
+    E = capi.Edges(ctx, g, 1, m.edges)
+    hc, eta = fisher_kpp.nonlocal_gain(ctx, E, ef)
+    loss = fisher_kpp.nonlocal_loss(ctx, E, ef)
+    A, M = fisher_kpp.assemble_galerkin_matrices(ctx, g, E, ef, capi.coeff(capi.COEFF_CONST, c),
+                                                 hc, L=loss)
+    h = L.nonlocal_gain(m.xy, m.edges, 1, ef)
+    rows, Lm = L.nonlocal_loss(m.xy, m.edges, m.edge_dofs(1), 1, ef)
+    (ra, ca, va), (rm, cm, vm) = L.fisher_kpp_matrices(m, c, (L.G_QP, 1.0, h))
+    Ad = _scipy((ra, ca, va), m.nv).toarray()
+    Ad[np.ix_(rows, rows)] -= Lm
+    handles = [loss[1], eta, E, g]
+    return m, A, M, sp.csr_matrix(Ad), _scipy((rm, cm, vm), m.nv), rows, handles
+
+
+def test_dense_block_product_solve_and_combinations(ctx):
+    # A carries the loss block with factor -1: its product, the FP64 PCG on
+    # M + s A (axpby carries the block, scaled) and clone agree with the
+    # oracle's dense A - L; the modes and paths without block support refuse
+    import scipy.sparse.linalg as sla
+    m, A, M, Aref, Mref, rows, handles = _nonlocal_system(ctx, 12)
+    nb, scale = capi.csr_dense_block(A)
+    assert nb == rows.size and scale == -1.0
+    x = np.random.default_rng(8).standard_normal(m.nv)
+    vx, vy = capi.Vec(ctx, x), capi.Vec(ctx, m.nv)
+    capi.spmv(ctx, A, vx, vy)
+    want = Aref @ x
+    scale_ref = np.abs(Aref).toarray() @ np.abs(x)
+    assert np.all(np.abs(vy.numpy() - want) <= 1e-14 * scale_ref + 1e-300)
+    s = 0.37
+    S = capi.csr_axpby(ctx, 1.0, M, s, A)
+    assert capi.csr_dense_block(S) == (nb, -s)
+    b = capi.Vec(ctx, np.random.default_rng(9).standard_normal(m.nv))
+    u = capi.Vec(ctx, m.nv)
+    info = capi.pcg(ctx, S, b, u, rtol=1e-12)
+    assert info.converged
+    Sref = (Mref + s * Aref).tocsc()
+    uref = sla.spsolve(Sref, b.numpy())
+    assert np.linalg.norm(u.numpy() - uref) <= 1e-9 * np.linalg.norm(uref)
+    # the solver's residual is the block-aware product's
+    r = b.numpy() - Sref @ u.numpy()
+    assert np.linalg.norm(r) <= 1.05e-12 * np.linalg.norm(b.numpy())
+    Cl = capi.csr_clone(ctx, S)
+    assert capi.csr_dense_block(Cl) == (nb, -s)
+    u2 = capi.Vec(ctx, m.nv)
+    capi.pcg(ctx, Cl, b, u2, rtol=1e-12)
+    assert np.array_equal(u2.numpy(), u.numpy())
+    for mode in (capi.PCG_MIXED, capi.PCG_MIXED_MATRIX):
+        with pytest.raises(capi.LfgpuError):
+            capi.pcg(ctx, S, b, u, mode=mode)
+    with pytest.raises(capi.LfgpuError):
+        capi.pcg(ctx, S, b, u, single_reduction=True)
+    with pytest.raises(capi.LfgpuError):
+        capi.pcg_noalloc(ctx, S, b, u)
+    with pytest.raises(capi.LfgpuError):
+        capi.csr_axpby(ctx, 1.0, S, 1.0, A)
+    # removing the block restores the sparse operator
+    capi.csr_set_dense_block(ctx, Cl, [], None)
+    assert capi.csr_dense_block(Cl) == (0, 0.0)
+    capi.spmv(ctx, Cl, vx, vy)
+    Ssp = Mref + s * _scipy(A.download(), m.nv)
+    assert np.all(np.abs(vy.numpy() - Ssp @ x) <= 1e-14 * (abs(Ssp) @ np.abs(x)) + 1e-300)
+    for h in (u2, Cl, u, b, S, vy, vx, A, M, *handles):
+        h.close()
+
+
+def test_strang_with_nonlocal_terms_matches_oracle(ctx):
+    # Evolution with the nonlocal gain and loss (strangsplitting.h:120-201
+    # with L != 0) against the oracle's SuperLU stepper on A - L
+    from lehrfempp_amd import fisher_kpp
+    m, A, M, Aref, Mref, rows, handles = _nonlocal_system(ctx, 16)
+    T, steps, lam = 1.0, 8, 2.1
+    K = 0.8 + 0.4 * m.xy[:, 0]
+    u0 = np.exp(-20 * ((m.xy[:, 0] - 0.3) ** 2 + (m.xy[:, 1] - 0.6) ** 2))
+    ss = fisher_kpp.StrangSplit(ctx, A, M, T, steps, lam, rtol=1e-13)
+    vK, vu = capi.Vec(ctx, K), capi.Vec(ctx, u0)
+    out = ss.Evolution(vK, vu)
+    assert ss.last_info.converged
+    want = L.StrangSplit(Aref, Mref, T, steps, lam).evolution(K, u0)
+    got = out.numpy()
+    assert np.abs(got - want).max() < 1e-10 * np.abs(want).max()
+    # pure diffusion (lambda = 0) keeps the mass 1^T M u: gain and loss
+    # balance at the boundary
+    ss0 = fisher_kpp.StrangSplit(ctx, A, M, T, steps, 0.0, rtol=1e-13)
+    cur = capi.Vec(ctx, u0)
+    for k in range(4):
+        nxt = ss0.diffusionEvolutionOperator(k == 0, cur)
+        cur.close()
+        cur = nxt
+    mass0, mass = (Mref @ u0).sum(), (Mref @ cur.numpy()).sum()
+    assert abs(mass - mass0) < 1e-11 * mass0
+    assert np.abs(cur.numpy() - u0).max() > 1e-3
+    for h in (cur, out, vK, vu, A, M, *handles):
+        h.close()
+    ss.close()
+    ss0.close()

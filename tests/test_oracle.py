@@ -668,3 +668,99 @@ def test_strang_diffusion_is_second_order_in_time():
         got = L.StrangSplit(A, M, T, nsteps, 0.0).evolution(np.ones(A.shape[0]), u0)
         errs.append(np.abs(got - exact).max())
     assert 3.3 < errs[0] / errs[1] < 4.7, errs
+
+
+# ----------------------------------------------- Fisher/KPP nonlocal terms
+def _nonlocal_case(n=8, jitter=0.0, rmin=0.0, rmax=0.0):
+    m = L.tp_triag_mesh(n)
+    if jitter:
+        rng = np.random.default_rng(7)
+        nf, _ = m.boundary_flags()
+        xy = m.xy.copy()
+        xy[nf == 0] += jitter / n * (rng.random((int((nf == 0).sum()), 2)) - 0.5)
+        m = L.TriMesh(xy, m.cells)
+    _, ef = m.boundary_flags()
+    h = L.nonlocal_gain(m.xy, m.edges, 1, ef, rmin, rmax)
+    rows, Lm = L.nonlocal_loss(m.xy, m.edges, m.edge_dofs(1), 1, ef, rmin, rmax)
+    return m, ef, h, rows, Lm
+
+
+def _quadrature_matrices(m, ef, rows):
+    """The boundary quadrature as matrices: point coordinates Q, weights W
+    (w_k |e| on flagged edges, 0 elsewhere) and the P1 basis values Phi[dof
+    row, point], built from the Gauss points (3 -+ sqrt 3) / 6 directly."""
+    ev, ed = m.edges, m.edge_dofs(1)
+    g = np.array([(3 - math.sqrt(3)) / 6, (3 + math.sqrt(3)) / 6])
+    pos = {int(r): i for i, r in enumerate(rows)}
+    Q = np.zeros((2 * len(ev), 2))
+    W = np.zeros(2 * len(ev))
+    Phi = np.zeros((len(rows), 2 * len(ev)))
+    for e in range(len(ev)):
+        a, b = m.xy[ev[e, 0]], m.xy[ev[e, 1]]
+        for k in range(2):
+            Q[2 * e + k] = a + (b - a) * g[k]
+            if ef[e]:
+                W[2 * e + k] = 0.5 * np.linalg.norm(b - a)
+                Phi[pos[int(ed[e, 0])], 2 * e + k] = 1 - g[k]
+                Phi[pos[int(ed[e, 1])], 2 * e + k] = g[k]
+    return Q, W, Phi
+
+
+@pytest.mark.parametrize("radii", [(0.0, 0.0), (0.3, 0.8)])
+def test_nonlocal_loss_is_the_double_boundary_quadrature(radii):
+    # L_ij = sum_kl phi_i(x_k) w_k g(x_k, y_l) w_l phi_j(y_l) over the
+    # boundary points: the nested AssembleVectorLocally calls of the drivers
+    # (modelproblem_island_main.cc:92-126) written as one matrix product
+    # from independently built quadrature matrices; a transposed index, a
+    # wrong weight or edge length, or a point on the wrong edge fails this
+    m, ef, h, rows, Lm = _nonlocal_case(8, jitter=0.5, rmin=radii[0], rmax=radii[1])
+    Q, W, Phi = _quadrature_matrices(m, ef, rows)
+    D2 = ((Q[:, None, :] - Q[None, :, :]) ** 2).sum(-1)
+    G = 1.0 / (1.0 + D2)
+    if radii[1] > 0:
+        D = np.sqrt(D2)
+        G = np.where((D >= radii[0]) & (D <= radii[1]), G, 0.0)
+    ref = (Phi * W) @ G @ (Phi * W).T
+    assert np.abs(ref - Lm).max() <= 1e-14 * np.abs(Lm).max()
+    # the gain at the points: h(x_k) = sum_l w_l g(x_k, y_l)
+    hq = G @ W
+    flagged = np.repeat(ef != 0, 2)
+    assert np.abs(hq[flagged] - h.ravel()[flagged]).max() <= 1e-14 * np.abs(hq).max()
+    assert np.all(h[ef == 0] == 0.0)
+    # the boundary DOFs are the vertices of the boundary edges
+    assert np.array_equal(rows, np.unique(m.edge_dofs(1)[ef != 0]))
+
+
+def test_nonlocal_loss_rows_sum_to_the_load_of_the_gain():
+    # sum_j phi_j = 1 on the boundary, so L 1 is the boundary load of h
+    # (ScalarLoadEdgeVectorProvider with f = h), and L is symmetric as g is:
+    # the loss and the gain of the model balance
+    m, ef, h, rows, Lm = _nonlocal_case(10, jitter=0.6)
+    ev_h = L.edge_vectors(m.xy, m.edges, 1, kind=L.G_QP, const=1.0, data=h.ravel())
+    bh = np.zeros(m.nv)
+    ed = m.edge_dofs(1)
+    for e in np.nonzero(ef)[0]:
+        bh[ed[e]] += ev_h[e]
+    assert np.abs(Lm.sum(1) - bh[rows]).max() <= 1e-14 * np.abs(bh).max()
+    assert np.abs(Lm - Lm.T).max() <= 1e-15 * np.abs(Lm).max()
+    assert np.all(Lm > 0)  # g > 0 everywhere without a cut
+
+
+def test_strang_with_nonlocal_terms_conserves_mass():
+    # gain h (edge mass term) and loss L balance: 1^T (A - L) = 0 up to
+    # rounding, so pure diffusion keeps 1^T M u (lambda = 0)
+    import scipy.sparse as sp
+    m, ef, h, rows, Lm = _nonlocal_case(8)
+    (rA, cA, vA), (rM, cM, vM) = L.fisher_kpp_matrices(m, 1.2, (L.G_QP, 1.0, h))
+    A = sp.csr_matrix((vA, cA, rA), shape=(m.nv, m.nv)).toarray()
+    A[np.ix_(rows, rows)] -= Lm
+    A = sp.csr_matrix(A)
+    M = sp.csr_matrix((vM, cM, rM), shape=(m.nv, m.nv))
+    assert np.abs(np.ones(m.nv) @ A).max() < 1e-13
+    ss = L.StrangSplit(A, M, 1.0, 10, 0.0)
+    u0 = np.random.default_rng(3).random(m.nv)
+    u = u0
+    for k in range(6):
+        u = ss.diffusion(k == 0, u)
+    assert abs((M @ u).sum() - (M @ u0).sum()) < 1e-13
+    assert np.abs(u - u0).max() > 1e-3
