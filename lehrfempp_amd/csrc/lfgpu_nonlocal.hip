@@ -155,8 +155,9 @@ __global__ __launch_bounds__(256) void k_nl_loss(int32_t nb, const int32_t *bptr
   L[t] = acc;
 }
 
-// y[rows[b]] += scale (B x[rows])_b, one wave per block row (columns over the
-// lanes, then a butterfly: a fixed order)
+// y[rows[b]] += scale (B x[rows])_b, one wave per block row in the order of
+// the solver's k_blk_gemv (columns over the lanes, 8 loads in flight per
+// lane, then a butterfly)
 __global__ __launch_bounds__(256) void k_blk_gemv_add(const double *B, int32_t nb,
                                                       const int32_t *rows, double scale,
                                                       const double *__restrict__ x, double *y) {
@@ -165,7 +166,19 @@ __global__ __launch_bounds__(256) void k_blk_gemv_add(const double *B, int32_t n
   if (row >= nb) return;
   const double *__restrict__ Br = B + (int64_t)row * nb;
   double acc = 0.0;
-  for (int32_t c = lane; c < nb; c += 64) acc += Br[c] * x[rows[c]];
+  for (int32_t c0 = lane; c0 < nb; c0 += 8 * 64) {
+    double bv[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int32_t c = c0 + 64 * u;
+      const bool in = c < nb;
+      bv[u] = in ? Br[c] : 0.0;
+      xv[u] = in ? x[rows[c]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (c0 + 64 * u < nb) acc += bv[u] * xv[u];
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if (lane == 0) y[rows[row]] += scale * acc;

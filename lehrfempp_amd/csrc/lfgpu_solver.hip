@@ -1096,8 +1096,10 @@ struct CgArgs {
 };
 
 // t = B v[srow] for the dense block, one wave per block row, each row summed
-// in a fixed order (lanes over the columns, then a butterfly); returns at
-// once when the solve has stopped (st non-null).
+// in a fixed order (lane l adds the columns l, l + 64, ... in turn, then a
+// butterfly); the loads of 8 columns per lane are issued before their
+// products are added.  Returns at once when the solve has stopped (st
+// non-null).
 __global__ __launch_bounds__(256) void k_blk_gemv(BlkView b, const double *__restrict__ v,
                                                   const PcgState *st) {
   if (st && st->done) return;
@@ -1106,7 +1108,19 @@ __global__ __launch_bounds__(256) void k_blk_gemv(BlkView b, const double *__res
   if (row >= b.n) return;
   const double *__restrict__ Br = b.B + (int64_t)row * b.n;
   double acc = 0.0;
-  for (int32_t c = lane; c < b.n; c += 64) acc += Br[c] * v[b.srow[c]];
+  for (int32_t c0 = lane; c0 < b.n; c0 += 8 * 64) {
+    double bv[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int32_t c = c0 + 64 * u;
+      const bool in = c < b.n;
+      bv[u] = in ? Br[c] : 0.0;
+      xv[u] = in ? v[b.srow[c]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (c0 + 64 * u < b.n) acc += bv[u] * xv[u];
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if (lane == 0) b.t[row] = acc;
