@@ -1689,12 +1689,18 @@ def test_nonlocal_gain_and_loss_bitexact(ctx, degree, radii):
         h.close()
 
 
-def _nonlocal_system(ctx, n, c=1.2):
+def _nonlocal_system(ctx, n, c=1.2, shuffle=False):
     """Device A (with the loss block attached) and M of the nonlocal model on
-    a jittered n x n mesh, and the oracle's dense A - L and M (scipy)."""
+    a jittered n x n mesh (vertices renumbered at random with shuffle), and
+    the oracle's dense A - L and M (scipy)."""
     from lehrfempp_amd import fisher_kpp
     import scipy.sparse as sp
     m = jittered_mesh(n, amount=0.3, seed=6)
+    if shuffle:
+        new = np.random.default_rng(12).permutation(m.nv).astype(np.int32)
+        xy = np.empty_like(m.xy)
+        xy[new] = m.xy
+        m = L.TriMesh(xy, new[m.cells], new[m.edges])
     _, ef = m.boundary_flags()
     g = capi.Mesh(ctx, m.xy, m.cells)
     E = capi.Edges(ctx, g, 1, m.edges)
@@ -1709,6 +1715,34 @@ def _nonlocal_system(ctx, n, c=1.2):
     Ad[np.ix_(rows, rows)] -= Lm
     handles = [loss[1], eta, E, g]
     return m, A, M, sp.csr_matrix(Ad), _scipy((rm, cm, vm), m.nv), rows, handles
+
+
+def test_dense_block_on_sliced_ell(ctx):
+    # vertices numbered at random: the stage matrix is stored as sliced ELL
+    # in the locality order, so the block rows reach the sweeps through the
+    # storage permutation; product and solve against the oracle's A - L
+    import scipy.sparse.linalg as sla
+    m, A, M, Aref, Mref, rows, handles = _nonlocal_system(ctx, 14, shuffle=True)
+    s = 0.21
+    S = capi.csr_axpby(ctx, 1.0, M, s, A)
+    b = capi.Vec(ctx, np.random.default_rng(10).standard_normal(m.nv))
+    u = capi.Vec(ctx, m.nv)
+    info = capi.pcg(ctx, S, b, u, rtol=1e-12)
+    assert info.converged
+    st, _, _ = capi.solver_storage(S)
+    assert st in (capi.STORAGE_ELL, capi.STORAGE_ELL16), st
+    Sref = (Mref + s * Aref).tocsc()
+    uref = sla.spsolve(Sref, b.numpy())
+    assert np.linalg.norm(u.numpy() - uref) <= 1e-9 * np.linalg.norm(uref)
+    assert (np.linalg.norm(b.numpy() - Sref @ u.numpy())
+            <= 1.05e-12 * np.linalg.norm(b.numpy()))
+    x = np.random.default_rng(11).standard_normal(m.nv)
+    vx, vy = capi.Vec(ctx, x), capi.Vec(ctx, m.nv)
+    capi.spmv(ctx, S, vx, vy)
+    want = Sref @ x
+    assert np.all(np.abs(vy.numpy() - want) <= 1e-14 * (abs(Sref) @ np.abs(x)) + 1e-300)
+    for h in (vy, vx, u, b, S, A, M, *handles):
+        h.close()
 
 
 def test_dense_block_product_solve_and_combinations(ctx):
