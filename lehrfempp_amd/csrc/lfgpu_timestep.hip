@@ -178,6 +178,10 @@ extern "C" int lfgpu_strang_create(lfgpu_ctx *ctx, const lfgpu_csr *A, const lfg
   LFG_REQUIRE(std::isfinite(T) && std::isfinite(lambda), LFGPU_EINVAL,
               "T and lambda must be finite");
   LFG_REQUIRE(opts->rtol >= 0.0 && opts->maxit >= 0, LFGPU_EINVAL, "bad PCG options");
+  LFG_REQUIRE((A->blk_n == 0 && M->blk_n == 0) ||
+                  (opts->mode == LFGPU_PCG_FP64 && !opts->single_reduction),
+              LFGPU_EINVAL,
+              "stage matrices with a dense block are solved in FP64 mode with two sweeps only");
   lfgpu_strang *st = new lfgpu_strang();
   st->ctx = ctx;
   st->n = A->nrows;

@@ -1822,6 +1822,10 @@ def test_strang_with_nonlocal_terms_matches_oracle(ctx):
     mass0, mass = (Mref @ u0).sum(), (Mref @ cur.numpy()).sum()
     assert abs(mass - mass0) < 1e-11 * mass0
     assert np.abs(cur.numpy() - u0).max() > 1e-3
+    # the mixed modes do not take the block: refused when the stepper is made
+    for mode in (capi.PCG_MIXED, capi.PCG_MIXED_MATRIX):
+        with pytest.raises(capi.LfgpuError):
+            fisher_kpp.StrangSplit(ctx, A, M, T, steps, lam, mode=mode)
     for h in (cur, out, vK, vu, A, M, *handles):
         h.close()
     ss.close()
