@@ -170,9 +170,11 @@ struct lfgpu_edges {
 namespace lfgpu {
 void solver_cache_free(lfgpu_csr *A);
 // Element matrices (and, if has_f, element vectors) of every cell through the
-// reference-tensor contraction on the matrix cores (lfgpu_mfma.hip).
+// reference-tensor contraction on the matrix cores (lfgpu_mfma.hip): ke is
+// [cell][i][j], or with rowmajor_i [i][cell][j].
 int element_contract(hipStream_t s, const lfgpu_mesh *m, const lfgpu_fe *fe, const CoeffD &alpha,
-                     const CoeffD &gamma, const CoeffD *f, double *ke, double *fvec);
+                     const CoeffD &gamma, const CoeffD *f, double *ke, double *fvec,
+                     bool rowmajor_i);
 // Release the edge records of a plan (lfgpu_edges.hip).
 void plan_free_edges(lfgpu_plan *p);
 inline void csr_values_changed(lfgpu_csr *A) { A->val_version++; }

@@ -47,7 +47,7 @@ template <int NLOC, int NQP, int KT>
 __global__ __launch_bounds__(kBlock) void k_element_contract(
     const double *xy, const int32_t *cell_v, int64_t ncells, const double *tab,
     const double *rten, CoeffD alpha, CoeffD gamma, CoeffD f, int has_f, double *ke,
-    double *fvec) {
+    double *fvec, int rowmajor_i) {
   constexpr int NE = NLOC * NLOC;
   constexpr int NEP = (NE + 15) / 16 * 16;
   constexpr int NT = 4 * KT;          // contraction length
@@ -110,10 +110,14 @@ __global__ __launch_bounds__(kBlock) void k_element_contract(
                                                    acc, 0, 0, 0);
       const int e = ct * 16 + ci;
       if (e < NE) {
+        // cell-major: ke[cell][i][j]; rowmajor_i: ke[i][cell][j], so the rows
+        // i of consecutive cells are adjacent for the row-owner gather
+        const int ei = e / NLOC, ej = e - ei * NLOC;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int64_t rc = wbase + q + 4 * v;
-          if (rc < ncells) ke[rc * NE + e] = acc[v];
+          if (rc < ncells)
+            ke[rowmajor_i ? ((int64_t)ei * ncells + rc) * NLOC + ej : rc * NE + e] = acc[v];
         }
       }
     }
@@ -128,17 +132,18 @@ bool cellwise(const CoeffD &c) {
 
 template <int NLOC, int NQP>
 int launch_contract(hipStream_t s, const lfgpu_mesh *m, const lfgpu_fe *fe, const CoeffD &a,
-                    const CoeffD &g, const CoeffD *f, double *ke, double *fvec) {
+                    const CoeffD &g, const CoeffD *f, double *ke, double *fvec, int rowmajor_i) {
   const int64_t nb = (m->ncells + kCellsPerBlock - 1) / kCellsPerBlock;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(nb, kMaxGrid));
   const CoeffD fz = f ? *f : CoeffD{LFGPU_COEFF_CONST, 0.0, nullptr};
   if (cellwise(a) && cellwise(g))
     hipLaunchKernelGGL((k_element_contract<NLOC, NQP, 1>), dim3(grid), dim3(kBlock), 0, s, m->xy,
-                       m->cell_v, m->ncells, fe->tab, fe->rten, a, g, fz, f ? 1 : 0, ke, fvec);
+                       m->cell_v, m->ncells, fe->tab, fe->rten, a, g, fz, f ? 1 : 0, ke, fvec,
+                       rowmajor_i);
   else
     hipLaunchKernelGGL((k_element_contract<NLOC, NQP, NQP>), dim3(grid), dim3(kBlock), 0, s,
                        m->xy, m->cell_v, m->ncells, fe->tab, fe->rten, a, g, fz, f ? 1 : 0, ke,
-                       fvec);
+                       fvec, rowmajor_i);
   LFG_HIP(hipGetLastError());
   return LFGPU_OK;
 }
@@ -148,10 +153,12 @@ int launch_contract(hipStream_t s, const lfgpu_mesh *m, const lfgpu_fe *fe, cons
 namespace lfgpu {
 
 int element_contract(hipStream_t s, const lfgpu_mesh *m, const lfgpu_fe *fe, const CoeffD &alpha,
-                     const CoeffD &gamma, const CoeffD *f, double *ke, double *fvec) {
+                     const CoeffD &gamma, const CoeffD *f, double *ke, double *fvec,
+                     bool rowmajor_i) {
   if (m->ncells == 0) return LFGPU_OK;
-  if (fe->nloc == 3) return launch_contract<3, 3>(s, m, fe, alpha, gamma, f, ke, fvec);
-  return launch_contract<10, 12>(s, m, fe, alpha, gamma, f, ke, fvec);
+  const int rm = rowmajor_i ? 1 : 0;
+  if (fe->nloc == 3) return launch_contract<3, 3>(s, m, fe, alpha, gamma, f, ke, fvec, rm);
+  return launch_contract<10, 12>(s, m, fe, alpha, gamma, f, ke, fvec, rm);
 }
 
 }  // namespace lfgpu
@@ -167,5 +174,5 @@ extern "C" int lfgpu_element_matrices_mfma(lfgpu_ctx *ctx, const lfgpu_mesh *m,
   CoeffD a, g;
   LFG_TRY(resolve_coeff(alpha, m->ncells, fe->nqp, false, &a, "alpha"));
   LFG_TRY(resolve_coeff(gamma, m->ncells, fe->nqp, false, &g, "gamma"));
-  return element_contract(ctx->stream, m, fe, a, g, nullptr, out->d, nullptr);
+  return element_contract(ctx->stream, m, fe, a, g, nullptr, out->d, nullptr, false);
 }
