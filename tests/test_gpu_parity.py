@@ -145,6 +145,35 @@ def test_galerkin_csr_matches_oracle(ctx, degree, n):
     assert np.array_equal(b, bo)
 
 
+@pytest.mark.parametrize("degree,n", [(1, 1), (1, 2), (1, 3), (3, 1), (3, 2)])
+def test_smallest_meshes(ctx, degree, n):
+    # the smallest structured meshes: every row fixed (P1, n = 1) or a handful
+    # of free rows, fewer rows than one workgroup and than the DIA
+    # detection's minimum; nonzero prescribed values make every system
+    # nontrivial.  Assembly bit-exact; the PCG solution equals the direct
+    # solve and the prescribed values
+    m = L.tp_triag_mesh(n)
+    rng = np.random.default_rng(40 + n)
+    alpha = np.exp(rng.standard_normal(m.ne))
+    _, nd = m.dofs(degree)
+    fixed_vals = rng.standard_normal(nd)
+    (rp, col, val, b), (rpo, colo, valo, bo) = _assemble_both(ctx, m, degree, alpha, 1.0,
+                                                              L.F_CONST, fixed_vals=fixed_vals)
+    assert np.array_equal(rp, rpo) and np.array_equal(col, colo)
+    assert np.array_equal(val, valo) and np.array_equal(b, bo)
+    rows = np.repeat(np.arange(nd), np.diff(rp))
+    A = capi.coo_to_csr(ctx, rows, col, val, nd, nd)
+    vb, x = capi.Vec(ctx, b), capi.Vec(ctx, nd)
+    info = capi.pcg(ctx, A, vb, x, rtol=1e-12)
+    assert info.converged
+    xd = L.band_cholesky_solve(rp, col, val, b)
+    assert np.linalg.norm(x.numpy() - xd) <= 1e-10 * np.linalg.norm(xd)
+    fl = m.dirichlet_flags(degree) != 0
+    assert np.allclose(x.numpy()[fl], fixed_vals[fl], rtol=1e-10, atol=0)
+    for h in (x, vb, A):
+        h.close()
+
+
 def test_galerkin_sinsin_rhs_tolerance(ctx):
     m = L.tp_triag_mesh(48)
     alpha = np.exp(np.random.default_rng(0).standard_normal(m.ne))
