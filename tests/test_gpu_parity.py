@@ -601,6 +601,42 @@ def test_batched_pcg_matches_per_problem_solves(ctx):
         h.close()
 
 
+@pytest.mark.parametrize("mode", [capi.PCG_FP64, capi.PCG_MIXED, capi.PCG_MIXED_MATRIX])
+def test_batched_with_trivial_problems(ctx, mode):
+    # problems of side 1 (every row fixed, zero right-hand side: Eigen's early
+    # exit) and 2 (one free row) next to larger ones: the trivial ones stop at
+    # iteration 0 with x = 0, every other one matches its direct solve
+    sizes = [1, 16, 2, 33, 1]
+    xy, cells, flags, voff, coff = capi.batched_tp_mesh(sizes)
+    rng = np.random.default_rng(23)
+    alpha = np.exp(rng.standard_normal(cells.shape[0]))
+    g = capi.Mesh(ctx, xy, cells)
+    fe = capi.FE(ctx, 1)
+    plan = capi.Plan(ctx, g, flags)
+    va = capi.Vec(ctx, alpha)
+    b = capi.Vec(ctx, g.ndofs)
+    plan.assemble(fe, capi.coeff(capi.COEFF_CELL, 1.0, va), capi.coeff(capi.COEFF_CONST, 0.0),
+                  capi.coeff(capi.COEFF_CONST, 1.0), b)
+    batch = capi.Batch(ctx, plan.csr, voff)
+    x = capi.Vec(ctx, g.ndofs)
+    x.fill(1.0)
+    summ, infos = batch.pcg(b, x, rtol=1e-10, mode=mode)
+    assert summ.converged
+    xg, bh = x.numpy(), b.numpy()
+    for k, n in enumerate(sizes):
+        m = L.tp_triag_mesh(n)
+        rp, col, val, bo = L.galerkin_system(m, 1, alpha[coff[k]:coff[k + 1]], 0.0, L.F_CONST,
+                                             1.0)
+        xk = xg[voff[k]:voff[k + 1]]
+        if n == 1:
+            assert not np.any(bo) and infos[k].iters == 0 and not np.any(xk), k
+            continue
+        xd = L.band_cholesky_solve(rp, col, val, bo)
+        assert np.linalg.norm(xk - xd) <= 1e-8 * np.linalg.norm(xd), k
+    for h in (x, batch, b, va, plan, fe, g):
+        h.close()
+
+
 @pytest.mark.parametrize("mode", [capi.PCG_MIXED, capi.PCG_MIXED_MATRIX])
 def test_batched_mixed_pcg_matches_direct_solves(ctx, mode):
     # per-problem reliable updates: each problem stops on its own FP64 true
